@@ -1,0 +1,221 @@
+"""ctypes binding of the glw C ABI (include/glw.h) and the device plan.
+
+The library is built in-tree as ``_lib/libglw.so``.  Nothing here computes a
+field: a missing library or device raises, there is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_size_t, c_uint8, c_void_p
+from pathlib import Path
+
+import numpy as np
+
+from .plan import HostPlan
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libglw.so"
+
+_DP = POINTER(c_double)
+_U8P = POINTER(c_uint8)
+
+
+class GlwError(RuntimeError):
+    """Raised when the native library reports an error."""
+
+
+class CBox(ctypes.Structure):
+    _fields_ = [("mx0", c_int32), ("my0", c_int32), ("wx", c_int32), ("wy", c_int32)]
+
+
+class CBoxTables(ctypes.Structure):
+    _fields_ = [(n, _DP) for n in ("qx", "qy", "nu", "inu", "q2", "a")]
+
+
+class COrder2Desc(ctypes.Structure):
+    _fields_ = [("eta_rank", c_int32), ("eta_damp", _DP), ("eta_cosh", _DP), ("eta_sinhr", _DP),
+                ("eta_coef", _DP), ("channels", c_int32), ("psi_damp", _DP), ("psi_factor", _DP),
+                ("psi_coef", _DP), ("psi_sign", _DP)]
+
+
+class CPlanDesc(ctypes.Structure):
+    _fields_ = [("nx", c_int32), ("ny", c_int32), ("depth", c_double), ("gravity", c_double),
+                ("floor_level", c_double), ("precision", c_int32), ("max_instants", c_int32),
+                ("order", c_int32), ("p1", CBox), ("p2", CBox), ("p3", CBox), ("g2x", c_int32),
+                ("g2y", c_int32), ("g3x", c_int32), ("g3y", c_int32), ("t1", CBoxTables), ("t2", CBoxTables),
+                ("t3", CBoxTables), ("omega", _DP), ("mask2", _U8P), ("mask3", _U8P), ("inner_rank", c_int32),
+                ("outer_rank", c_int32), ("damp_outer", _DP), ("damp_inner", _DP), ("inner_cosh", _DP),
+                ("inner_sinhr", _DP), ("inner_coef", _DP), ("outer_cosh", _DP), ("outer_sinhr", _DP),
+                ("outer_coef", _DP), ("o2", COrder2Desc)]
+
+
+#: every symbol include/glw.h declares
+EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
+           "glw_order3_execute", "glw_plan_buffer", "glw_fft_length_supported", "glw_last_error",
+           "glw_version")
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None):
+    """Load libglw.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise GlwError(f"native library {p} is missing: run __graft_entry__.build() (hipcc, gfx950)")
+    lib = ctypes.CDLL(str(p))
+    lib.glw_plan_create.argtypes = [POINTER(CPlanDesc), POINTER(c_void_p)]
+    lib.glw_plan_create.restype = c_int
+    lib.glw_plan_destroy.argtypes = [c_void_p]
+    lib.glw_plan_destroy.restype = c_int
+    for name in ("glw_linear_execute", "glw_order2_execute", "glw_order3_execute"):
+        fn = getattr(lib, name)
+        fn.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
+        fn.restype = c_int
+    lib.glw_plan_buffer.argtypes = [c_void_p, c_char_p, POINTER(c_void_p), POINTER(c_size_t)]
+    lib.glw_plan_buffer.restype = c_int
+    lib.glw_fft_length_supported.argtypes = [c_int32]
+    lib.glw_fft_length_supported.restype = c_int
+    lib.glw_last_error.restype = c_char_p
+    lib.glw_version.restype = c_char_p
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = load_library().glw_last_error().decode()
+        raise GlwError(f"glw error {rc}: {msg}")
+
+
+def _dp(a: np.ndarray | None, keep: list):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    keep.append(a)
+    return a.ctypes.data_as(_DP)
+
+
+def _u8p(a: np.ndarray | None, keep: list):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    keep.append(a)
+    return a.ctypes.data_as(_U8P)
+
+
+def _cbox(b) -> CBox:
+    if b is None:
+        return CBox(0, 0, 0, 0)
+    return CBox(b.mx0, b.my0, b.wx, b.wy)
+
+
+def _ctab(t, keep) -> CBoxTables:
+    if t is None:
+        return CBoxTables()
+    return CBoxTables(*(_dp(getattr(t, n), keep) for n in ("qx", "qy", "nu", "inu", "q2", "a")))
+
+
+def describe(plan: HostPlan, max_instants: int) -> tuple[CPlanDesc, list]:
+    """Fill the C descriptor from a host plan (arrays kept alive in the list)."""
+    keep: list = []
+    d = CPlanDesc()
+    d.nx, d.ny = plan.nx, plan.ny
+    d.depth, d.gravity, d.floor_level = plan.depth, plan.gravity, plan.floor_level
+    d.precision, d.max_instants, d.order = plan.precision, max_instants, plan.order
+    d.p1, d.p2, d.p3 = _cbox(plan.box1), _cbox(plan.box2), _cbox(plan.box3)
+    d.g2x, d.g2y = plan.g2 if plan.g2 else (0, 0)
+    d.g3x, d.g3y = plan.g3 if plan.g3 else (0, 0)
+    d.t1, d.t2, d.t3 = _ctab(plan.t1, keep), _ctab(plan.t2, keep), _ctab(plan.t3, keep)
+    d.omega = _dp(plan.omega, keep)
+    d.mask2 = _u8p(plan.mask2, keep)
+    d.mask3 = _u8p(plan.mask3, keep)
+    if plan.order >= 3:
+        o3 = plan.o3
+        d.inner_rank, d.outer_rank = o3["inner_rank"], o3["outer_rank"]
+        for n in ("damp_outer", "damp_inner", "inner_cosh", "inner_sinhr", "inner_coef", "outer_cosh",
+                  "outer_sinhr", "outer_coef"):
+            setattr(d, n, _dp(o3[n], keep))
+    if plan.order >= 2:
+        o2 = plan.o2
+        d.o2 = COrder2Desc(o2["eta_rank"], _dp(o2["eta_damp"], keep), _dp(o2["eta_cosh"], keep),
+                           _dp(o2["eta_sinhr"], keep), _dp(o2["eta_coef"], keep), o2["channels"],
+                           _dp(o2["psi_damp"], keep), _dp(o2["psi_factor"], keep), _dp(o2["psi_coef"], keep),
+                           _dp(o2["psi_sign"], keep))
+    return d, keep
+
+
+class DevicePlan:
+    """A glw plan on the current HIP device plus the t=0 parent spectrum.
+
+    ``run(kind, times)`` evaluates ``kind`` in {"linear", "order2", "order3"}
+    at the given instants and returns two complex128 torch tensors
+    ``[nt, ny, nx]`` on the device.
+    """
+
+    def __init__(self, host: HostPlan, s0_box: np.ndarray, *, max_instants: int = 16):
+        import torch
+
+        if not torch.cuda.is_available():
+            raise GlwError("a HIP device is required (torch.cuda.is_available() is False)")
+        self.lib = load_library()
+        self.host = host
+        self.max_instants = int(max_instants)
+        desc, keep = describe(host, self.max_instants)
+        handle = c_void_p()
+        _check(self.lib.glw_plan_create(ctypes.byref(desc), ctypes.byref(handle)))
+        del keep
+        self._handle = handle
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        s0 = np.ascontiguousarray(s0_box, dtype=np.complex128)
+        if s0.shape != (host.box1.wy, host.box1.wx):
+            raise ValueError("parent spectrum must be given on the plan's parent box")
+        self.s0 = torch.from_numpy(s0).to(self.device)
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            self.lib.glw_plan_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def times_tensor(self, times):
+        import torch
+
+        if isinstance(times, torch.Tensor):
+            return times.to(device=self.device, dtype=torch.float64).contiguous()
+        return torch.as_tensor(np.atleast_1d(np.asarray(times, dtype=float)), dtype=torch.float64,
+                               device=self.device)
+
+    def run(self, kind: str, times, out=None):
+        import torch
+
+        t = self.times_tensor(times)
+        nt = int(t.numel())
+        shape = (nt, self.host.ny, self.host.nx)
+        if out is None:
+            out = (torch.empty(shape, dtype=torch.complex128, device=self.device),
+                   torch.empty(shape, dtype=torch.complex128, device=self.device))
+        a, b = out
+        for x in (a, b):
+            if x.shape != shape or x.dtype != torch.complex128 or not x.is_contiguous() or x.device != self.device:
+                raise ValueError(f"outputs must be contiguous complex128 {shape} on {self.device}")
+        fn = {"linear": self.lib.glw_linear_execute, "order2": self.lib.glw_order2_execute,
+              "order3": self.lib.glw_order3_execute}[kind]
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _check(fn(self._handle, c_void_p(self.s0.data_ptr()), c_void_p(t.data_ptr()), nt,
+                  c_void_p(a.data_ptr()), c_void_p(b.data_ptr()), c_void_p(stream)))
+        return a, b
+
+    def buffer(self, name: str):
+        ptr, size = c_void_p(), c_size_t()
+        _check(self.lib.glw_plan_buffer(self._handle, name.encode(), ctypes.byref(ptr), ctypes.byref(size)))
+        return ptr.value, size.value

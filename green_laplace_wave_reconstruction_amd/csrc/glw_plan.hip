@@ -1,0 +1,1093 @@
+// Plan, workspace, accumulation kernels and C ABI of the glw library.
+//
+// Build: hipcc --offload-arch=gfx950 -O3 -shared -fPIC
+//        -Xclang -target-feature -Xclang -packed-fp32-ops
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "glw.h"
+#include "glw_kernels.hpp"
+#include "glw_ops.hpp"
+
+using namespace glw;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define GLW_HIP(call)                                                                \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return fail(GLW_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+// ----------------------------------------------------------------------------
+// FP64 elementwise kernels
+// ----------------------------------------------------------------------------
+__global__ void k_fill_u64(uint64_t* p, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// S_t = S0 exp(-i omega t)   (python/src/green_laplace/spatial.py:70-72;
+// src/gl_spectral_surface.m:145-147)
+__global__ void k_phase(const cpx<double>* s0, const double* omega, const double* times, int box,
+                        cpx<double>* st) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.y;
+  if (b >= box) return;
+  double sn, cs;
+  sincos(omega[b] * times[t], &sn, &cs);
+  const cpx<double> s = s0[b];
+  st[(size_t)t * box + b] = {s.x * cs + s.y * sn, s.y * cs - s.x * sn};
+}
+
+__device__ __forceinline__ cpx<double> floored(cpx<double> x, double level) {
+  return hypot(x.x, x.y) < level ? cpx<double>{0.0, 0.0} : x;
+}
+__device__ __forceinline__ double bits_to_d(uint64_t b) { return __longlong_as_double((long long)b); }
+
+// Inner GL accumulation of the pair state (_order3.py:240-259).
+struct AccInnerArgs {
+  const cpx<double>* spec;  // [nto*I][4][box]
+  const uint64_t* fmax;     // [nto*I][4]
+  double floor_level;
+  int I, box;
+  const double* a;
+  const uint8_t* mask;
+  const double* ch;         // [I][box]
+  const double* sh;
+  const double* coef;       // [I]
+  cpx<double>* out;         // [nto][4][box]: eta, eta_t, phi, phi_t
+  uint64_t* etmax;          // [nto]
+};
+
+__global__ __launch_bounds__(NT) void k_acc_inner(AccInnerArgs g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int to = blockIdx.y;
+  cpx<double> e = {0, 0}, et = {0, 0}, p = {0, 0}, pt = {0, 0};
+  const bool on = b < g.box && g.mask[b];
+  if (on) {
+    const double A = g.a[b];
+    for (int i = 0; i < g.I; ++i) {
+      const size_t task = (size_t)to * g.I + i;
+      cpx<double> s[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        s[k] = floored(g.spec[(task * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 4 + k]));
+      const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
+      // e += c (-A sh sd + ch sk)
+      const cpx<double> ze = {c * (-A * shv * s[0].x + chv * s[1].x), c * (-A * shv * s[0].y + chv * s[1].y)};
+      e = cadd(e, ze);
+      // et -= i c (-A sh sdt + ch skt)
+      const cpx<double> zt = {c * (-A * shv * s[2].x + chv * s[3].x), c * (-A * shv * s[2].y + chv * s[3].y)};
+      et = {et.x + zt.y, et.y - zt.x};
+      // p += i c (ch sd - sh sk)
+      const cpx<double> zp = {c * (chv * s[0].x - shv * s[1].x), c * (chv * s[0].y - shv * s[1].y)};
+      p = {p.x - zp.y, p.y + zp.x};
+      // pt += c (ch sdt - sh skt)
+      pt = {pt.x + c * (chv * s[2].x - shv * s[3].x), pt.y + c * (chv * s[2].y - shv * s[3].y)};
+    }
+  }
+  if (b < g.box) {
+    cpx<double>* o = g.out + (size_t)to * 4 * g.box + b;
+    o[0] = e;
+    o[g.box] = et;
+    o[2 * (size_t)g.box] = p;
+    o[3 * (size_t)g.box] = pt;
+  }
+  block_max_atomic(on ? hypot(et.x, et.y) : 0.0, g.etmax + to);
+}
+
+// Floors of the cubic forcing spectra and the maxima of the psi combinations
+// (_order3.py:103-122).
+struct ComboArgs {
+  const cpx<double>* spec;  // [nto][4][box]: F_K, F_D, C, C_t
+  const uint64_t* fmax;     // [nto][4]
+  double floor_level;
+  int box;
+  const double* a;
+  const uint8_t* mask;
+  uint64_t* cmax;           // [nto][2]
+};
+
+__global__ __launch_bounds__(NT) void k_combo_outer(ComboArgs g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int to = blockIdx.y;
+  double m0 = 0.0, m1 = 0.0;
+  if (b < g.box && g.mask[b]) {
+    cpx<double> s[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      s[k] = floored(g.spec[((size_t)to * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)to * 4 + k]));
+    const double A = g.a[b];
+    const cpx<double> kp = {s[0].x + A * s[2].x, s[0].y + A * s[2].y};
+    const cpx<double> dp = csub(s[1], s[3]);
+    m0 = hypot(kp.x, kp.y);
+    m1 = hypot(dp.x, dp.y);
+  }
+  block_max_atomic(m0, g.cmax + (size_t)to * 2);
+  __syncthreads();
+  block_max_atomic(m1, g.cmax + (size_t)to * 2 + 1);
+}
+
+// Outer GL accumulation of eta33, psi33 spectra (_order3.py:117-138).
+struct AccOuterArgs {
+  const cpx<double>* spec;  // [nt*O][4][box]
+  const uint64_t* fmax;     // [nt*O][4]
+  const uint64_t* cmax;     // [nt*O][2]
+  double floor_level;
+  int O, box;
+  const double* a;
+  const uint8_t* mask;
+  const double* ch;         // [O][box]
+  const double* sh;
+  const double* coef;       // [O]
+  cpx<double>* out;         // [nt][2][box]
+};
+
+__global__ __launch_bounds__(NT) void k_acc_outer(AccOuterArgs g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int t = blockIdx.y;
+  if (b >= g.box) return;
+  cpx<double> eta = {0, 0}, psi = {0, 0};
+  if (g.mask[b]) {
+    const double A = g.a[b];
+    for (int o = 0; o < g.O; ++o) {
+      const size_t to = (size_t)t * g.O + o;
+      cpx<double> s[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        s[k] = floored(g.spec[(to * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[to * 4 + k]));
+      const cpx<double> kp = floored(cpx<double>{s[0].x + A * s[2].x, s[0].y + A * s[2].y},
+                                     g.floor_level * bits_to_d(g.cmax[to * 2]));
+      const cpx<double> dp = floored(csub(s[1], s[3]), g.floor_level * bits_to_d(g.cmax[to * 2 + 1]));
+      const double c = g.coef[o], chv = g.ch[(size_t)o * g.box + b], shv = g.sh[(size_t)o * g.box + b];
+      // eta += c (A sh F_D - i ch F_K)
+      eta.x += c * (A * shv * s[1].x + chv * s[0].y);
+      eta.y += c * (A * shv * s[1].y - chv * s[0].x);
+      // psi += c (-i ch F_Dpsi - sh F_Kpsi)
+      psi.x += c * (chv * dp.y - shv * kp.x);
+      psi.y += c * (-chv * dp.x - shv * kp.y);
+    }
+  }
+  g.out[(size_t)t * 2 * g.box + b] = eta;
+  g.out[((size_t)t * 2 + 1) * g.box + b] = psi;
+}
+
+// eta22 accumulation (spatial.py:192-221).
+struct AccEta22Args {
+  const cpx<double>* spec;  // [nt*J][2][box]
+  const uint64_t* fmax;     // [nt*J][2]
+  double floor_level;
+  int J, box;
+  const double* a;
+  const double* ch;         // [J][box]
+  const double* sh;
+  const double* coef;       // [J]
+  cpx<double>* out;         // [nt][2][box] slot 0
+};
+
+__global__ __launch_bounds__(NT) void k_acc_eta22(AccEta22Args g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int t = blockIdx.y;
+  if (b >= g.box) return;
+  const double A = g.a[b];
+  cpx<double> acc = {0, 0};
+  for (int j = 0; j < g.J; ++j) {
+    const size_t task = (size_t)t * g.J + j;
+    const cpx<double> sd = floored(g.spec[(task * 2) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2]));
+    const cpx<double> sk = floored(g.spec[(task * 2 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2 + 1]));
+    const double c = g.coef[j], chv = g.ch[(size_t)j * g.box + b], shv = g.sh[(size_t)j * g.box + b];
+    acc.x += c * (-A * shv * sd.x + chv * sk.x);
+    acc.y += c * (-A * shv * sd.y + chv * sk.y);
+  }
+  g.out[(size_t)t * 2 * g.box + b] = acc;
+}
+
+// psi22 accumulation (spatial.py:276-308).
+struct AccPsi22Args {
+  const cpx<double>* spec;    // [nt*4][8][box]  (branch-node task, pair p: sd = 2p, sk = 2p+1)
+  const cpx<double>* contact; // [nt][1][box]
+  int box;
+  const double* inu;
+  const uint8_t* mask;
+  const double* factor;       // [4][4][box]
+  const double* coef;         // [4]
+  const double* sign;         // [2]
+  cpx<double>* out;           // [nt][2][box] slot 1
+};
+
+__global__ __launch_bounds__(NT) void k_acc_psi22(AccPsi22Args g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int t = blockIdx.y;
+  if (b >= g.box) return;
+  cpx<double> res = {0, 0};
+  const double inu = g.inu[b];
+  for (int bn = 0; bn < 4; ++bn) {
+    const double sgn = g.sign[bn / 2];
+    const size_t task = (size_t)t * 4 + bn;
+    cpx<double> resp = {0, 0};
+    for (int p = 0; p < 4; ++p) {
+      const double f = g.factor[((size_t)bn * 4 + p) * g.box + b];
+      const cpx<double> sd = g.spec[(task * 8 + 2 * p) * g.box + b];
+      const cpx<double> sk = g.spec[(task * 8 + 2 * p + 1) * g.box + b];
+      resp.x += f * (sd.x + sgn * sk.x * inu);
+      resp.y += f * (sd.y + sgn * sk.y * inu);
+    }
+    const double c = g.coef[bn];
+    res = {res.x - c * resp.y, res.y + c * resp.x};
+  }
+  const cpx<double> ct = g.contact[(size_t)t * g.box + b];
+  cpx<double> v = cadd(res, ct);
+  if (!g.mask[b]) v = {0, 0};
+  g.out[((size_t)t * 2 + 1) * g.box + b] = v;
+}
+
+// ----------------------------------------------------------------------------
+// Dispatch over FFT lengths
+// ----------------------------------------------------------------------------
+#define GLW_LENGTHS(X) \
+  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768) X(1024)
+
+constexpr int kLdsBudget = 48 * 1024;  // column kernels: keep several workgroups per CU
+constexpr int kLdsMax = 160 * 1024;    // gfx950 per-workgroup LDS
+
+template <typename T, int L>
+constexpr int col_width() {
+  int cw = kLdsBudget / (L * (int)sizeof(cpx<T>));
+  int p = 1;
+  while (p * 2 <= cw && p < 32) p *= 2;
+  return p;
+}
+
+bool length_supported(int n) {
+  switch (n) {
+#define GLW_CASE(L) case L:
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+template <typename T>
+int launch_colinv(int gy, ColInvArgs a, int ntask, const void* tw, hipStream_t st) {
+  switch (gy) {
+#define GLW_CASE(L)                                                                       \
+  case L: {                                                                               \
+    constexpr int CW = col_width<T, L>();                                                 \
+    hipLaunchKernelGGL((k_colinv<T, L, CW>), dim3((a.wx + CW - 1) / CW, a.nfield, ntask), \
+                       dim3(NT), 0, st, a, (const cpx<T>*)tw);                            \
+    break;                                                                                \
+  }
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return fail(GLW_EUNSUPPORTED, "column length " + std::to_string(gy) + " not instantiated");
+  }
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+template <typename T>
+int launch_colfwd(int gy, ColFwdArgs a, int ntask, const void* tw, hipStream_t st) {
+  switch (gy) {
+#define GLW_CASE(L)                                                                        \
+  case L: {                                                                                \
+    constexpr int CW = col_width<T, L>();                                                  \
+    hipLaunchKernelGGL((k_colfwd<T, L, CW>), dim3((a.owx + CW - 1) / CW, a.nsrc, ntask),  \
+                       dim3(NT), 0, st, a, (const cpx<T>*)tw);                             \
+    break;                                                                                 \
+  }
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return fail(GLW_EUNSUPPORTED, "column length " + std::to_string(gy) + " not instantiated");
+  }
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+// Rows per workgroup: as many as fit a 64 KB tile (at most 4), at least 1.
+template <typename T, int NIN, int L>
+constexpr int row_ry() {
+  int per = NIN * L * (int)sizeof(cpx<T>);
+  int ry = (64 * 1024) / per;
+  if (ry >= 4) return 4;
+  if (ry >= 2) return 2;
+  return 1;
+}
+
+template <typename T, int NIN, int NOUT, class Op>
+int launch_row(int gx, RowArgs a, int ntask, const void* tw, hipStream_t st) {
+  switch (gx) {
+#define GLW_CASE(L)                                                                         \
+  case L: {                                                                                 \
+    constexpr int RY = row_ry<T, NIN, L>();                                                 \
+    if constexpr (NIN * RY * L * sizeof(cpx<T>) <= kLdsMax) {                               \
+      if (a.gy % RY) return fail(GLW_EINVAL, "row count not divisible by the row tile");    \
+      hipLaunchKernelGGL((k_row<T, L, NIN, NOUT, RY, Op>), dim3(a.gy / RY, ntask), dim3(NT), \
+                         0, st, a, (const cpx<T>*)tw);                                      \
+    } else {                                                                                \
+      return fail(GLW_EUNSUPPORTED, "row tile exceeds LDS at length " + std::to_string(L)); \
+    }                                                                                       \
+    break;                                                                                  \
+  }
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return fail(GLW_EUNSUPPORTED, "row length " + std::to_string(gx) + " not instantiated");
+  }
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+int launch_synth(int gx, SynthArgs a, int nseq, const void* tw, hipStream_t st) {
+  switch (gx) {
+#define GLW_CASE(L)                                                                          \
+  case L: {                                                                                  \
+    constexpr int RY = (L <= 256) ? 4 : 2;                                                   \
+    if (a.gy % RY) return fail(GLW_EINVAL, "row count not divisible by the synthesis tile"); \
+    hipLaunchKernelGGL((k_synth<double, L, RY>), dim3(a.gy / RY, nseq), dim3(NT), 0, st, a,  \
+                       (const cpx<double>*)tw);                                              \
+    break;                                                                                   \
+  }
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return fail(GLW_EUNSUPPORTED, "synthesis length " + std::to_string(gx) + " not instantiated");
+  }
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Plan
+// ----------------------------------------------------------------------------
+struct DevTables {
+  double *qx = nullptr, *qy = nullptr, *nu = nullptr, *inu = nullptr, *q2 = nullptr, *a = nullptr;
+  BoxTables view() const { return BoxTables{qx, qy, nu, inu, q2, a}; }
+};
+
+int box_size(const glw_box& b) { return b.wx * b.wy; }
+
+}  // namespace
+
+struct glw_plan_s {
+  glw_plan_desc d{};
+  int device = 0;
+  std::vector<void*> allocs;
+  std::map<int, void*> tw64, tw32;
+  DevTables t1, t2, t3;
+  double *omega = nullptr, *damp_o = nullptr, *damp_oi = nullptr;
+  uint8_t *mask2 = nullptr, *mask3 = nullptr;
+  double *in_ch = nullptr, *in_sh = nullptr, *in_coef = nullptr;
+  double *out_ch = nullptr, *out_sh = nullptr, *out_coef = nullptr;
+  // order 2
+  double *e22_damp = nullptr, *e22_ch = nullptr, *e22_sh = nullptr, *e22_coef = nullptr;
+  double *p22_damp = nullptr, *p22_fac = nullptr, *p22_coef = nullptr, *p22_sign = nullptr;
+  // workspace: name -> (ptr, bytes)
+  std::map<std::string, std::pair<void*, size_t>> ws;
+
+  ~glw_plan_s() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+
+  template <typename X>
+  int upload(X** dst, const X* src, size_t n, bool required = true) {
+    *dst = nullptr;
+    if (!src) {
+      if (required) return fail(GLW_EINVAL, "missing required plan table");
+      return GLW_OK;
+    }
+    void* p = nullptr;
+    GLW_HIP(hipMalloc(&p, n * sizeof(X)));
+    allocs.push_back(p);
+    GLW_HIP(hipMemcpy(p, src, n * sizeof(X), hipMemcpyHostToDevice));
+    *dst = (X*)p;
+    return GLW_OK;
+  }
+
+  int upload_tables(DevTables& dt, const glw_box_tables& h, size_t n) {
+    int rc;
+    if ((rc = upload(&dt.qx, h.qx, n))) return rc;
+    if ((rc = upload(&dt.qy, h.qy, n))) return rc;
+    if ((rc = upload(&dt.nu, h.nu, n))) return rc;
+    if ((rc = upload(&dt.inu, h.inu, n))) return rc;
+    if ((rc = upload(&dt.q2, h.q2, n))) return rc;
+    return upload(&dt.a, h.a, n);
+  }
+
+  int twiddles(int n) {
+    if (tw64.count(n)) return GLW_OK;
+    std::vector<cpx<double>> w(n);
+    std::vector<cpx<float>> wf(n);
+    for (int m = 0; m < n; ++m) {
+      long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+      w[m] = {(double)cosl(ang), (double)sinl(ang)};
+      wf[m] = {(float)cosl(ang), (float)sinl(ang)};
+    }
+    cpx<double>* p = nullptr;
+    int rc = upload(&p, w.data(), n);
+    if (rc) return rc;
+    tw64[n] = p;
+    cpx<float>* pf = nullptr;
+    rc = upload(&pf, wf.data(), n);
+    if (rc) return rc;
+    tw32[n] = pf;
+    return GLW_OK;
+  }
+
+  int reserve(const char* name, size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 256;
+    GLW_HIP(hipMalloc(&p, bytes));
+    allocs.push_back(p);
+    ws[name] = {p, bytes};
+    return GLW_OK;
+  }
+  template <typename X>
+  X* buf(const char* name) {
+    return (X*)ws.at(name).first;
+  }
+  const void* tw(int n, bool fp32) { return fp32 ? tw32.at(n) : tw64.at(n); }
+};
+
+namespace {
+
+int validate(const glw_plan_desc* d) {
+  if (!d) return fail(GLW_EINVAL, "null descriptor");
+  if (d->nx < 8 || d->ny < 8 || d->nx % 2 || d->ny % 2) return fail(GLW_EINVAL, "grid must be even and >= 8");
+  if (!length_supported(d->nx) || !length_supported(d->ny))
+    return fail(GLW_EUNSUPPORTED, "output grid length not supported by the FFT kernels");
+  if (d->order < 1 || d->order > 3) return fail(GLW_EINVAL, "order must be 1, 2 or 3");
+  if (d->precision != GLW_PREC_FP64 && d->precision != GLW_PREC_FP32) return fail(GLW_EINVAL, "bad precision");
+  if (d->max_instants < 1) return fail(GLW_EINVAL, "max_instants must be positive");
+  auto box_ok = [&](const glw_box& b, int gx, int gy) {
+    return b.wx >= 1 && b.wy >= 1 && b.wx <= gx && b.wy <= gy;
+  };
+  if (!box_ok(d->p1, d->nx, d->ny)) return fail(GLW_EINVAL, "parent box does not fit the grid");
+  if (d->order >= 2) {
+    if (!length_supported(d->g2x) || !length_supported(d->g2y))
+      return fail(GLW_EUNSUPPORTED, "quadratic working grid length not supported");
+    if (2 * d->p1.wx - 1 > d->g2x || 2 * d->p1.wy - 1 > d->g2y)
+      return fail(GLW_EINVAL, "quadratic working grid narrower than the pair support");
+    if (!box_ok(d->p2, d->g2x, d->g2y) || !box_ok(d->p2, d->nx, d->ny))
+      return fail(GLW_EINVAL, "pair box does not fit the working grid");
+  }
+  if (d->order >= 3) {
+    if (!length_supported(d->g3x) || !length_supported(d->g3y))
+      return fail(GLW_EUNSUPPORTED, "cubic working grid length not supported");
+    if (3 * d->p1.wx - 2 > d->g3x || 3 * d->p1.wy - 2 > d->g3y)
+      return fail(GLW_EINVAL, "cubic working grid narrower than the triple support");
+    if (!box_ok(d->p3, d->g3x, d->g3y) || !box_ok(d->p3, d->nx, d->ny))
+      return fail(GLW_EINVAL, "triple box does not fit the working grid");
+    if (d->inner_rank < 1 || d->inner_rank > 16 || d->outer_rank < 1 || d->outer_rank > 16)
+      return fail(GLW_EINVAL, "GL ranks must be 1..16");
+    if (d->p2.wx > d->g3x || d->p2.wy > d->g3y) return fail(GLW_EINVAL, "pair box wider than the cubic grid");
+  }
+  if (d->order >= 2 && (d->o2.eta_rank < 1 || d->o2.eta_rank > 16 || d->o2.channels != 2))
+    return fail(GLW_EINVAL, "order-2 tables: eta rank 1..16 and two psi22 channels required");
+  return GLW_OK;
+}
+
+int build(glw_plan* P) {
+  const glw_plan_desc& d = P->d;
+  const size_t b1 = box_size(d.p1), b2 = box_size(d.p2), b3 = box_size(d.p3);
+  const bool f32 = d.precision == GLW_PREC_FP32;
+  const size_t csz = f32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  const size_t zsz = sizeof(cpx<double>);
+  const size_t nt = d.max_instants;
+  int rc;
+  if ((rc = P->upload_tables(P->t1, d.t1, b1))) return rc;
+  if ((rc = P->upload(&P->omega, d.omega, b1))) return rc;
+  if ((rc = P->twiddles(d.nx))) return rc;
+  if ((rc = P->twiddles(d.ny))) return rc;
+  int wfin = d.p1.wx;
+  if ((rc = P->reserve("st", nt * b1 * zsz))) return rc;
+  if (d.order >= 2) {
+    wfin = std::max(wfin, (int)d.p2.wx);
+    if ((rc = P->upload_tables(P->t2, d.t2, b2))) return rc;
+    if ((rc = P->upload(&P->mask2, d.mask2, b2))) return rc;
+    if ((rc = P->twiddles(d.g2x))) return rc;
+    if ((rc = P->twiddles(d.g2y))) return rc;
+    const int J = d.o2.eta_rank;
+    if ((rc = P->upload(&P->e22_damp, d.o2.eta_damp, (size_t)J * b1))) return rc;
+    if ((rc = P->upload(&P->e22_ch, d.o2.eta_cosh, (size_t)J * b2))) return rc;
+    if ((rc = P->upload(&P->e22_sh, d.o2.eta_sinhr, (size_t)J * b2))) return rc;
+    if ((rc = P->upload(&P->e22_coef, d.o2.eta_coef, (size_t)J))) return rc;
+    if ((rc = P->upload(&P->p22_damp, d.o2.psi_damp, (size_t)8 * b1))) return rc;
+    if ((rc = P->upload(&P->p22_fac, d.o2.psi_factor, (size_t)16 * b2))) return rc;
+    if ((rc = P->upload(&P->p22_coef, d.o2.psi_coef, (size_t)4))) return rc;
+    if ((rc = P->upload(&P->p22_sign, d.o2.psi_sign, (size_t)2))) return rc;
+    const size_t g2 = (size_t)d.g2y;
+    if ((rc = P->reserve("e22_t", nt * J * 8 * g2 * d.p1.wx * csz))) return rc;
+    if ((rc = P->reserve("e22_u", nt * J * 2 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("e22_s", nt * J * 2 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("e22_max", nt * J * 2 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("p22_t", nt * 4 * 16 * g2 * d.p1.wx * csz))) return rc;
+    if ((rc = P->reserve("p22_u", nt * 4 * 8 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("p22_s", nt * 4 * 8 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("c22_t", nt * 2 * g2 * d.p1.wx * csz))) return rc;
+    if ((rc = P->reserve("c22_u", nt * 1 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("c22_s", nt * 1 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("ep22", nt * 2 * b2 * zsz))) return rc;
+  }
+  if (d.order >= 3) {
+    wfin = std::max(wfin, (int)d.p3.wx);
+    const size_t I = d.inner_rank, O = d.outer_rank;
+    if ((rc = P->upload_tables(P->t3, d.t3, b3))) return rc;
+    if ((rc = P->upload(&P->mask3, d.mask3, b3))) return rc;
+    if ((rc = P->twiddles(d.g3x))) return rc;
+    if ((rc = P->twiddles(d.g3y))) return rc;
+    if ((rc = P->upload(&P->damp_o, d.damp_outer, O * b1))) return rc;
+    if ((rc = P->upload(&P->damp_oi, d.damp_inner, O * I * b1))) return rc;
+    if ((rc = P->upload(&P->in_ch, d.inner_cosh, I * b2))) return rc;
+    if ((rc = P->upload(&P->in_sh, d.inner_sinhr, I * b2))) return rc;
+    if ((rc = P->upload(&P->in_coef, d.inner_coef, I))) return rc;
+    if ((rc = P->upload(&P->out_ch, d.outer_cosh, O * b3))) return rc;
+    if ((rc = P->upload(&P->out_sh, d.outer_sinhr, O * b3))) return rc;
+    if ((rc = P->upload(&P->out_coef, d.outer_coef, O))) return rc;
+    const size_t g2 = d.g2y, g3 = d.g3y;
+    const size_t wmax = std::max(d.p1.wx, d.p2.wx);
+    if ((rc = P->reserve("tin", nt * O * I * 12 * g2 * d.p1.wx * csz))) return rc;
+    if ((rc = P->reserve("uin", nt * O * I * 4 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("sin", nt * O * I * 4 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("max_in", nt * O * I * 4 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("pair", nt * O * 4 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("max_et", nt * O * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("tout", nt * O * 21 * g3 * wmax * csz))) return rc;
+    if ((rc = P->reserve("uout", nt * O * 4 * g3 * d.p3.wx * csz))) return rc;
+    if ((rc = P->reserve("sout", nt * O * 4 * b3 * zsz))) return rc;
+    if ((rc = P->reserve("max_out", nt * O * 4 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("max_combo", nt * O * 2 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("ep33", nt * 2 * b3 * zsz))) return rc;
+  }
+  if ((rc = P->reserve("tfin", nt * 2 * (size_t)d.ny * wfin * zsz))) return rc;
+  return GLW_OK;
+}
+
+int fill_zero(uint64_t* p, size_t n, hipStream_t st) {
+  if (n == 0) return GLW_OK;
+  hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+ColInvArgs colinv_base(const cpx<double>* spec, int nsrc, int sdiv, const glw_box& box, const DevTables& t) {
+  ColInvArgs a;
+  memset(&a, 0, sizeof(a));
+  a.spec = spec;
+  a.nsrc = nsrc;
+  a.sdiv = sdiv;
+  a.mx0 = box.mx0;
+  a.my0 = box.my0;
+  a.wx = box.wx;
+  a.wy = box.wy;
+  a.tab = t.view();
+  a.dmod = 1;
+  a.dmul = 1;
+  a.fmax_stride = nsrc;
+  return a;
+}
+
+// Spectra on `box` ([nt][nsrc][box]) -> physical [nt][ny][nx] fields, one per
+// (src, scale, multiplier) triple.
+int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& box, const DevTables& tab,
+               int nt, const int* srcs, const int* codes, const double* scales, double** outs, int nout,
+               hipStream_t st) {
+  const glw_plan_desc& d = P->d;
+  ColInvArgs a = colinv_base(spec, nsrc, 1, box, tab);
+  a.nfield = nout;
+  for (int f = 0; f < nout; ++f) {
+    a.fsrc[f] = srcs[f];
+    a.fcode[f] = codes[f];
+    a.fscale[f] = scales[f];
+  }
+  cpx<double>* tfin = P->buf<cpx<double>>("tfin");
+  a.out = tfin;
+  a.out_nfield = nout;
+  a.out_foff = 0;
+  a.ostride = box.wx;
+  int rc = launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), st);
+  if (rc) return rc;
+  for (int f = 0; f < nout; ++f) {
+    // rows of field f of every instant: sequence t starts at (t*nout + f)*ny*wx
+    SynthArgs s;
+    s.in = tfin + (size_t)f * d.ny * box.wx;
+    s.in_seq = (size_t)nout * d.ny * box.wx;
+    s.istride = box.wx;
+    s.mx0 = box.mx0;
+    s.wx = box.wx;
+    s.gy = d.ny;
+    s.out = (cpx<double>*)outs[f];
+    if ((rc = launch_synth(d.nx, s, nt, P->tw(d.nx, false), st))) return rc;
+  }
+  return GLW_OK;
+}
+
+int phase(glw_plan* P, const double* s0, const double* times, int nt, hipStream_t st) {
+  const int b1 = box_size(P->d.p1);
+  hipLaunchKernelGGL(k_phase, dim3((b1 + 255) / 256, nt), dim3(256), 0, st, (const cpx<double>*)s0, P->omega,
+                     times, b1, P->buf<cpx<double>>("st"));
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+template <typename T>
+int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) {
+  const glw_plan_desc& d = P->d;
+  const bool f32 = std::is_same<T, float>::value;
+  const int I = d.inner_rank, O = d.outer_rank;
+  const int b2 = box_size(d.p2), b3 = box_size(d.p3);
+  const int n_in = nt * O * I, n_to = nt * O;
+  int rc;
+  const cpx<double>* stp = P->buf<cpx<double>>("st");
+  // ---- inner pair stage: tasks (t, o, i)
+  {
+    ColInvArgs a = colinv_base(stp, 1, O * I, d.p1, P->t1);
+    a.damp = P->damp_oi;
+    a.dmod = O * I;
+    static const int codes[12] = {M_ONE, M_NU, M_NU2, M_NU3, M_QX_NU, M_QY_NU, M_QX, M_QY, M_Q2_NU, M_Q2, M_NU_QX, M_NU_QY};
+    const double sc = 1.0 / ((double)d.nx * d.ny);
+    a.nfield = 12;
+    for (int f = 0; f < 12; ++f) {
+      a.fsrc[f] = 0;
+      a.fcode[f] = codes[f];
+      a.fscale[f] = sc;
+    }
+    a.out = P->ws.at("tin").first;
+    a.out_nfield = 12;
+    a.ostride = d.p1.wx;
+    if ((rc = launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), st))) return rc;
+  }
+  {
+    RowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.in = P->ws.at("tin").first;
+    r.nin = 12;
+    r.istride = d.p1.wx;
+    for (int f = 0; f < 12; ++f) {
+      r.fmx0[f] = d.p1.mx0;
+      r.fwx[f] = d.p1.wx;
+    }
+    r.out = P->ws.at("uin").first;
+    r.nout = 4;
+    r.ostride = d.p2.wx;
+    r.omx0 = d.p2.mx0;
+    r.owx = d.p2.wx;
+    r.gy = d.g2y;
+    if ((rc = launch_row<T, 12, 4, InnerOp>(d.g2x, r, n_in, P->tw(d.g2x, f32), st))) return rc;
+  }
+  if ((rc = fill_zero(P->buf<uint64_t>("max_in"), (size_t)n_in * 4, st))) return rc;
+  {
+    ColFwdArgs c;
+    c.in = P->ws.at("uin").first;
+    c.nsrc = 4;
+    c.istride = d.p2.wx;
+    c.omx0 = d.p2.mx0;
+    c.owx = d.p2.wx;
+    c.omy0 = d.p2.my0;
+    c.owy = d.p2.wy;
+    c.mask = P->mask2;
+    c.scale = (double)d.nx * d.ny / ((double)d.g2x * d.g2y);
+    c.out = P->buf<cpx<double>>("sin");
+    c.fmax = P->buf<uint64_t>("max_in");
+    if ((rc = launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), st))) return rc;
+  }
+  if ((rc = fill_zero(P->buf<uint64_t>("max_et"), (size_t)n_to, st))) return rc;
+  {
+    AccInnerArgs g;
+    g.spec = P->buf<cpx<double>>("sin");
+    g.fmax = P->buf<uint64_t>("max_in");
+    g.floor_level = d.floor_level;
+    g.I = I;
+    g.box = b2;
+    g.a = P->t2.a;
+    g.mask = P->mask2;
+    g.ch = P->in_ch;
+    g.sh = P->in_sh;
+    g.coef = P->in_coef;
+    g.out = P->buf<cpx<double>>("pair");
+    g.etmax = P->buf<uint64_t>("max_et");
+    hipLaunchKernelGGL(k_acc_inner, dim3((b2 + NT - 1) / NT, n_to), dim3(NT), 0, st, g);
+    GLW_HIP(hipGetLastError());
+  }
+  // ---- outer stage: tasks (t, o)
+  const int wmax = std::max(d.p1.wx, d.p2.wx);
+  {
+    ColInvArgs a = colinv_base(stp, 1, O, d.p1, P->t1);
+    a.damp = P->damp_o;
+    a.dmod = O;
+    static const int codes[12] = {M_ONE,  M_I_QX, M_I_QY,     M_QX_NU,   M_QY_NU, M_MI_NU,
+                                  M_NU_QX, M_NU_QY, M_MI_Q2_NU, M_MI_Q2NU, M_MA,    M_MQ2};
+    const double sc = 1.0 / ((double)d.nx * d.ny);
+    a.nfield = 12;
+    for (int f = 0; f < 12; ++f) {
+      a.fsrc[f] = 0;
+      a.fcode[f] = codes[f];
+      a.fscale[f] = sc;
+    }
+    a.out = P->ws.at("tout").first;
+    a.out_nfield = 21;
+    a.out_foff = 0;
+    a.ostride = wmax;
+    if ((rc = launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st))) return rc;
+  }
+  {
+    ColInvArgs a = colinv_base(P->buf<cpx<double>>("pair"), 4, 1, d.p2, P->t2);
+    a.fmax = P->buf<uint64_t>("max_et");
+    a.fmax_stride = 1;          // one maximum per (t, o)
+    a.floor_src_mask = 1 << 1;  // eta_t is floored after accumulation (_order3.py:259)
+    a.floor_level = d.floor_level;
+    static const int srcs[9] = {0, 0, 0, 2, 2, 2, 2, 3, 1};
+    static const int codes[9] = {M_ONE, M_I_QX, M_I_QY, M_I_QX, M_I_QY, M_A, M_Q2, M_A, M_ONE};
+    const double sc = 1.0 / ((double)d.nx * d.ny);
+    a.nfield = 9;
+    for (int f = 0; f < 9; ++f) {
+      a.fsrc[f] = srcs[f];
+      a.fcode[f] = codes[f];
+      a.fscale[f] = sc;
+    }
+    a.out = P->ws.at("tout").first;
+    a.out_nfield = 21;
+    a.out_foff = 12;
+    a.ostride = wmax;
+    if ((rc = launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st))) return rc;
+  }
+  {
+    RowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.in = P->ws.at("tout").first;
+    r.nin = 21;
+    r.istride = wmax;
+    for (int f = 0; f < 21; ++f) {
+      r.fmx0[f] = f < 12 ? d.p1.mx0 : d.p2.mx0;
+      r.fwx[f] = f < 12 ? d.p1.wx : d.p2.wx;
+    }
+    r.out = P->ws.at("uout").first;
+    r.nout = 4;
+    r.ostride = d.p3.wx;
+    r.omx0 = d.p3.mx0;
+    r.owx = d.p3.wx;
+    r.gy = d.g3y;
+    if ((rc = launch_row<T, 21, 4, OuterOp>(d.g3x, r, n_to, P->tw(d.g3x, f32), st))) return rc;
+  }
+  if ((rc = fill_zero(P->buf<uint64_t>("max_out"), (size_t)n_to * 4, st))) return rc;
+  if ((rc = fill_zero(P->buf<uint64_t>("max_combo"), (size_t)n_to * 2, st))) return rc;
+  {
+    ColFwdArgs c;
+    c.in = P->ws.at("uout").first;
+    c.nsrc = 4;
+    c.istride = d.p3.wx;
+    c.omx0 = d.p3.mx0;
+    c.owx = d.p3.wx;
+    c.omy0 = d.p3.my0;
+    c.owy = d.p3.wy;
+    c.mask = P->mask3;
+    c.scale = (double)d.nx * d.ny / ((double)d.g3x * d.g3y);
+    c.out = P->buf<cpx<double>>("sout");
+    c.fmax = P->buf<uint64_t>("max_out");
+    if ((rc = launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), st))) return rc;
+  }
+  {
+    ComboArgs g;
+    g.spec = P->buf<cpx<double>>("sout");
+    g.fmax = P->buf<uint64_t>("max_out");
+    g.floor_level = d.floor_level;
+    g.box = b3;
+    g.a = P->t3.a;
+    g.mask = P->mask3;
+    g.cmax = P->buf<uint64_t>("max_combo");
+    hipLaunchKernelGGL(k_combo_outer, dim3((b3 + NT - 1) / NT, n_to), dim3(NT), 0, st, g);
+    GLW_HIP(hipGetLastError());
+  }
+  {
+    AccOuterArgs g;
+    g.spec = P->buf<cpx<double>>("sout");
+    g.fmax = P->buf<uint64_t>("max_out");
+    g.cmax = P->buf<uint64_t>("max_combo");
+    g.floor_level = d.floor_level;
+    g.O = O;
+    g.box = b3;
+    g.a = P->t3.a;
+    g.mask = P->mask3;
+    g.ch = P->out_ch;
+    g.sh = P->out_sh;
+    g.coef = P->out_coef;
+    g.out = P->buf<cpx<double>>("ep33");
+    hipLaunchKernelGGL(k_acc_outer, dim3((b3 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
+    GLW_HIP(hipGetLastError());
+  }
+  const double h = d.depth, pot = d.depth * std::sqrt(d.gravity * d.depth), nn = (double)d.nx * d.ny;
+  const int srcs[2] = {0, 1}, codes[2] = {M_ONE, M_ONE};
+  const double scales[2] = {h / nn, pot / nn};
+  double* outs[2] = {eta, psi};
+  return synthesize(P, P->buf<cpx<double>>("ep33"), 2, d.p3, P->t3, nt, srcs, codes, scales, outs, 2, st);
+}
+
+template <typename T>
+int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) {
+  const glw_plan_desc& d = P->d;
+  const bool f32 = std::is_same<T, float>::value;
+  const int J = d.o2.eta_rank;
+  const int b2 = box_size(d.p2);
+  const double sc = 1.0 / ((double)d.nx * d.ny);
+  const double up = (double)d.nx * d.ny / ((double)d.g2x * d.g2y);
+  const cpx<double>* stp = P->buf<cpx<double>>("st");
+  int rc;
+  auto row_args = [&](const char* tin, int nin, const char* uout, int nout) {
+    RowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.in = P->ws.at(tin).first;
+    r.nin = nin;
+    r.istride = d.p1.wx;
+    for (int f = 0; f < nin; ++f) {
+      r.fmx0[f] = d.p1.mx0;
+      r.fwx[f] = d.p1.wx;
+    }
+    r.out = P->ws.at(uout).first;
+    r.nout = nout;
+    r.ostride = d.p2.wx;
+    r.omx0 = d.p2.mx0;
+    r.owx = d.p2.wx;
+    r.gy = d.g2y;
+    return r;
+  };
+  auto fwd_args = [&](const char* uin, int nsrc, const char* sout, uint64_t* fmax, const uint8_t* mask) {
+    ColFwdArgs c;
+    c.in = P->ws.at(uin).first;
+    c.nsrc = nsrc;
+    c.istride = d.p2.wx;
+    c.omx0 = d.p2.mx0;
+    c.owx = d.p2.wx;
+    c.omy0 = d.p2.my0;
+    c.owy = d.p2.wy;
+    c.mask = mask;
+    c.scale = up;
+    c.out = P->buf<cpx<double>>(sout);
+    c.fmax = fmax;
+    return c;
+  };
+  static const int e_codes[8] = {M_ONE, M_NU, M_NU2, M_QX_NU, M_QY_NU, M_Q2_NU, M_QX, M_QY};
+  // ---- eta22: tasks (t, j)
+  {
+    ColInvArgs a = colinv_base(stp, 1, J, d.p1, P->t1);
+    a.damp = P->e22_damp;
+    a.dmod = J;
+    a.nfield = 8;
+    for (int f = 0; f < 8; ++f) {
+      a.fsrc[f] = 0;
+      a.fcode[f] = e_codes[f];
+      a.fscale[f] = sc;
+    }
+    a.out = P->ws.at("e22_t").first;
+    a.out_nfield = 8;
+    a.ostride = d.p1.wx;
+    if ((rc = launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), st))) return rc;
+    if ((rc = launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), st)))
+      return rc;
+    if ((rc = fill_zero(P->buf<uint64_t>("e22_max"), (size_t)nt * J * 2, st))) return rc;
+    if ((rc = launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
+                               P->tw(d.g2y, f32), st)))
+      return rc;
+    AccEta22Args g;
+    g.spec = P->buf<cpx<double>>("e22_s");
+    g.fmax = P->buf<uint64_t>("e22_max");
+    g.floor_level = d.floor_level;
+    g.J = J;
+    g.box = b2;
+    g.a = P->t2.a;
+    g.ch = P->e22_ch;
+    g.sh = P->e22_sh;
+    g.coef = P->e22_coef;
+    g.out = P->buf<cpx<double>>("ep22");
+    hipLaunchKernelGGL(k_acc_eta22, dim3((b2 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
+    GLW_HIP(hipGetLastError());
+  }
+  // ---- psi22: tasks (t, branch*2 + node), two channels of 8 fields each
+  {
+    ColInvArgs a = colinv_base(stp, 1, 4, d.p1, P->t1);
+    static const int p_codes[8] = {M_ONE, M_NU, M_NU2, M_QX_NU, M_QY_NU, M_Q2_NU, M_QX, M_QY};
+    // damping rows: [branch][node][channel]; a task (t, bn) uses rows 2*bn + ch.
+    // colinv picks one damping row per task, so run the two channels as two
+    // launches writing field offsets 0 and 8.
+    for (int ch = 0; ch < 2; ++ch) {
+      a.damp = P->p22_damp;
+      a.dmod = 4;
+      a.dmul = 2;
+      a.dadd = ch;
+      a.nfield = 8;
+      for (int f = 0; f < 8; ++f) {
+        a.fsrc[f] = 0;
+        a.fcode[f] = p_codes[f];
+        a.fscale[f] = sc;
+      }
+      a.out = P->ws.at("p22_t").first;
+      a.out_nfield = 16;
+      a.out_foff = 8 * ch;
+      a.ostride = d.p1.wx;
+      if ((rc = launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), st))) return rc;
+    }
+    if ((rc = launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), st)))
+      return rc;
+    if ((rc = launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), st)))
+      return rc;
+  }
+  // ---- psi22 Taylor contact: tasks t
+  {
+    ColInvArgs a = colinv_base(stp, 1, 1, d.p1, P->t1);
+    a.nfield = 2;
+    a.fsrc[0] = 0;
+    a.fcode[0] = M_ONE;
+    a.fscale[0] = sc;
+    a.fsrc[1] = 0;
+    a.fcode[1] = M_MI_NU;
+    a.fscale[1] = sc;
+    a.out = P->ws.at("c22_t").first;
+    a.out_nfield = 2;
+    a.ostride = d.p1.wx;
+    if ((rc = launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), st))) return rc;
+    if ((rc = launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), st)))
+      return rc;
+    if ((rc = launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), st)))
+      return rc;
+  }
+  {
+    AccPsi22Args g;
+    g.spec = P->buf<cpx<double>>("p22_s");
+    g.contact = P->buf<cpx<double>>("c22_s");
+    g.box = b2;
+    g.inu = P->t2.inu;
+    g.mask = P->mask2;
+    g.factor = P->p22_fac;
+    g.coef = P->p22_coef;
+    g.sign = P->p22_sign;
+    g.out = P->buf<cpx<double>>("ep22");
+    hipLaunchKernelGGL(k_acc_psi22, dim3((b2 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
+    GLW_HIP(hipGetLastError());
+  }
+  const double h = d.depth, pot = d.depth * std::sqrt(d.gravity * d.depth), nn = (double)d.nx * d.ny;
+  const int srcs[2] = {0, 1}, codes[2] = {M_ONE, M_ONE};
+  const double scales[2] = {h / nn, pot / nn};
+  double* outs[2] = {eta, psi};
+  return synthesize(P, P->buf<cpx<double>>("ep22"), 2, d.p2, P->t2, nt, srcs, codes, scales, outs, 2, st);
+}
+
+typedef int (*chunk_fn)(glw_plan*, int, double*, double*, hipStream_t);
+
+int run_chunks(glw_plan* P, const double* s0, const double* times, int nt, double* a, double* b, void* stream,
+               chunk_fn fn) {
+  if (!P) return fail(GLW_EINVAL, "null plan");
+  if (nt < 0) return fail(GLW_EINVAL, "negative instant count");
+  if (nt == 0) return GLW_OK;
+  if (!s0 || !times || !a || !b) return fail(GLW_EINVAL, "null array argument");
+  GLW_HIP(hipSetDevice(P->device));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t plane = (size_t)P->d.ny * P->d.nx * 2;  // doubles per output field
+  for (int t0 = 0; t0 < nt; t0 += P->d.max_instants) {
+    const int n = std::min(P->d.max_instants, nt - t0);
+    int rc = phase(P, s0, times + t0, n, st);
+    if (rc) return rc;
+    rc = fn(P, n, a + (size_t)t0 * plane, b + (size_t)t0 * plane, st);
+    if (rc) return rc;
+  }
+  return GLW_OK;
+}
+
+int linear_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) {
+  const glw_plan_desc& d = P->d;
+  const double h = d.depth, pot = d.depth * std::sqrt(d.gravity * d.depth), nn = (double)d.nx * d.ny;
+  const int srcs[2] = {0, 0}, codes[2] = {M_ONE, M_MI_INU};
+  const double scales[2] = {h / nn, pot / nn};
+  double* outs[2] = {eta, psi};
+  return synthesize(P, P->buf<cpx<double>>("st"), 1, d.p1, P->t1, nt, srcs, codes, scales, outs, 2, st);
+}
+
+int order3_dispatch(glw_plan* P, int nt, double* a, double* b, hipStream_t st) {
+  return P->d.precision == GLW_PREC_FP32 ? order3_chunk<float>(P, nt, a, b, st) : order3_chunk<double>(P, nt, a, b, st);
+}
+int order2_dispatch(glw_plan* P, int nt, double* a, double* b, hipStream_t st) {
+  return P->d.precision == GLW_PREC_FP32 ? order2_chunk<float>(P, nt, a, b, st) : order2_chunk<double>(P, nt, a, b, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int glw_plan_create(const glw_plan_desc* desc, glw_plan** plan) {
+  if (!plan) return fail(GLW_EINVAL, "null plan pointer");
+  *plan = nullptr;
+  int rc = validate(desc);
+  if (rc) return rc;
+  glw_plan* P = new glw_plan_s();
+  P->d = *desc;
+  if (hipGetDevice(&P->device) != hipSuccess) {
+    delete P;
+    return fail(GLW_EDEVICE, "no HIP device");
+  }
+  rc = build(P);
+  if (rc) {
+    delete P;
+    return rc;
+  }
+  // host table pointers are not kept past creation
+  P->d.t1 = P->d.t2 = P->d.t3 = glw_box_tables{};
+  *plan = P;
+  return GLW_OK;
+}
+
+int glw_plan_destroy(glw_plan* plan) {
+  delete plan;
+  return GLW_OK;
+}
+
+int glw_linear_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta11,
+                       double* psi11, void* stream) {
+  return run_chunks(plan, s0, times, nt, eta11, psi11, stream, linear_chunk);
+}
+
+int glw_order2_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta22,
+                       double* psi22, void* stream) {
+  if (plan && plan->d.order < 2) return fail(GLW_EINVAL, "plan built without order-2 tables");
+  return run_chunks(plan, s0, times, nt, eta22, psi22, stream, order2_dispatch);
+}
+
+int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta33,
+                       double* psi33, void* stream) {
+  if (plan && plan->d.order < 3) return fail(GLW_EINVAL, "plan built without order-3 tables");
+  return run_chunks(plan, s0, times, nt, eta33, psi33, stream, order3_dispatch);
+}
+
+int glw_plan_buffer(glw_plan* plan, const char* name, void** ptr, size_t* bytes) {
+  if (!plan || !name || !ptr || !bytes) return fail(GLW_EINVAL, "null argument");
+  auto it = plan->ws.find(name);
+  if (it == plan->ws.end()) return fail(GLW_EINVAL, std::string("no workspace buffer ") + name);
+  *ptr = it->second.first;
+  *bytes = it->second.second;
+  return GLW_OK;
+}
+
+int glw_fft_length_supported(int32_t n) { return length_supported(n) ? 1 : 0; }
+
+const char* glw_last_error(void) { return g_err.c_str(); }
+
+const char* glw_version(void) { return "glw 0.1.0 gfx950"; }
+
+}  // extern "C"

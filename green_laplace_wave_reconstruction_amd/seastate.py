@@ -1,0 +1,109 @@
+"""Seeded synthetic directional sea states on an FFT grid.
+
+The components follow the MF12-style input of the reference API
+(src/gl_spectral_coefficients.m:1): amplitudes ``a, b`` with
+``a + i b = sqrt(2 S(kx, ky) dkx dky) e^{i theta}`` on strict-forward grid
+cells, JONSWAP frequency spectrum times a ``cos^{2s}((theta - theta0)/2)``
+spreading, truncated at ``|k| < kmax_factor * kp`` and scaled so that the
+discrete variance equals ``(Hs/4)^2``.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+
+def peak_wavenumber(tp: float, depth: float, gravity: float = 9.81) -> float:
+    """Solve ``(2 pi / Tp)^2 = g k tanh(k h)`` by Newton iteration."""
+    w2h_g = (2.0 * math.pi / tp) ** 2 * depth / gravity
+    q = max(math.sqrt(w2h_g), w2h_g)
+    for _ in range(100):
+        t = math.tanh(q)
+        f = q * t - w2h_g
+        q_new = q - f / (t + q * (1.0 - t * t))
+        if abs(q_new - q) < 1e-15 * max(1.0, q):
+            q = q_new
+            break
+        q = q_new
+    return q / depth
+
+
+@dataclass(frozen=True)
+class SeaState:
+    """Components plus the grid they were drawn on."""
+
+    a: np.ndarray
+    b: np.ndarray
+    kx: np.ndarray
+    ky: np.ndarray
+    gravity: float
+    depth: float
+    lx: float
+    ly: float
+    nx: int
+    ny: int
+    kp: float
+
+    @property
+    def count(self) -> int:
+        return int(self.a.size)
+
+
+def jonswap_sea_state(*, nx: int, ny: int, lx: float, ly: float, depth: float, hs: float, tp: float,
+                      gamma: float = 3.3, spread_s: float = 10.0, theta0: float = 0.0, gravity: float = 9.81,
+                      kmax_factor: float = 4.0, min_kh: float = 0.0, seed: int = 0) -> SeaState:
+    """Draw a seeded strict-forward JONSWAP sea state on the ``[ny, nx]`` FFT grid.
+
+    Cells kept: ``mx >= 1`` (kx > 0), ``|my| < ny/2``, ``|k| < kmax_factor*kp`` and
+    ``k h > min_kh``.  Phases are ``U[0, 2 pi)`` from ``numpy.random.default_rng(seed)``
+    drawn in row-major cell order.
+    """
+    kp = peak_wavenumber(tp, depth, gravity)
+    dkx, dky = 2.0 * math.pi / lx, 2.0 * math.pi / ly
+    mx = np.arange(1, nx // 2)
+    my = np.arange(-(ny // 2) + 1, ny // 2)
+    MX, MY = np.meshgrid(mx, my)
+    KX, KY = MX * dkx, MY * dky
+    K = np.hypot(KX, KY)
+    keep = (K < kmax_factor * kp) & (K * depth > min_kh)
+    kx, ky, k = KX[keep], KY[keep], K[keep]
+    omega = np.sqrt(gravity * k * np.tanh(k * depth))
+    wp = 2.0 * math.pi / tp
+    sigma = np.where(omega <= wp, 0.07, 0.09)
+    peak = gamma ** np.exp(-((omega - wp) ** 2) / (2.0 * sigma**2 * wp**2))
+    s_omega = omega**-5.0 * np.exp(-1.25 * (wp / omega) ** 4) * peak
+    # d omega / d k for the finite-depth dispersion relation
+    th = np.tanh(k * depth)
+    domega_dk = gravity * (th + k * depth * (1.0 - th**2)) / (2.0 * omega)
+    theta = np.arctan2(ky, kx)
+    spread = np.cos(0.5 * (theta - theta0)) ** (2.0 * spread_s)
+    density = s_omega * domega_dk / k * spread  # S(kx, ky) up to a constant
+    variance = float(np.sum(density) * dkx * dky)
+    if variance <= 0:
+        raise ValueError("empty sea state: no energetic strict-forward cells")
+    density *= (hs / 4.0) ** 2 / variance
+    amp = np.sqrt(2.0 * density * dkx * dky)
+    rng = np.random.default_rng(seed)
+    phase = rng.uniform(0.0, 2.0 * math.pi, size=amp.size)
+    return SeaState(a=amp * np.cos(phase), b=amp * np.sin(phase), kx=kx, ky=ky, gravity=gravity, depth=depth,
+                    lx=lx, ly=ly, nx=nx, ny=ny, kp=kp)
+
+
+def order2_plain_128(seed: int = 1) -> SeaState:
+    """Plain GL order 2 at 128^2: L=1000 m, h=50 m, Hs=4 m, Tp=10 s, gamma=3.3, s=10."""
+    return jonswap_sea_state(nx=128, ny=128, lx=1000.0, ly=1000.0, depth=50.0, hs=4.0, tp=10.0, gamma=3.3,
+                             spread_s=10.0, seed=seed)
+
+
+def order3_headline_512(seed: int = 2) -> SeaState:
+    """GL order 3 at 512^2: L=2000 m, h=30 m, Hs=6 m, Tp=12 s, gamma=3.3, s=15, parents kh > 0.5."""
+    return jonswap_sea_state(nx=512, ny=512, lx=2000.0, ly=2000.0, depth=30.0, hs=6.0, tp=12.0, gamma=3.3,
+                             spread_s=15.0, min_kh=0.5, seed=seed)
+
+
+def headline_instants(tp: float = 12.0, count: int = 64) -> np.ndarray:
+    """The 64 instants uniform over [0, 10 Tp]."""
+    return np.linspace(0.0, 10.0 * tp, count)

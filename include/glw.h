@@ -1,0 +1,135 @@
+/* glw -- MI355X (gfx950) Green--Laplace bound-wave reconstruction, C ABI.
+ *
+ * The reference has no FFI: its hot path is a NumPy/MATLAB call graph.  Each
+ * entry point below replaces one reference function for one sea-state plan
+ * (a fixed grid, parent support and GL rank set) and many time instants:
+ *
+ *   glw_plan_create / glw_plan_destroy
+ *       replace the per-call setup of the reference graphs: supports, balance
+ *       shifts, GL rules, multipliers (python/src/green_laplace/_order3.py:28-55,
+ *       spatial.py:162-190, 241-275) and the MATLAB plan cache
+ *       (src/internal/gl_no_stokes_eta33.m:535-621).  The host builds every
+ *       FP64 table; the plan uploads them and owns the device workspace.
+ *   glw_linear_execute
+ *       eta11+ and psi11+ (spatial.py:74-79; src/gl_spectral_surface.m:26-31).
+ *   glw_order2_execute
+ *       eta22+ (spatial.py:156-231; src/internal/green_laplace_eta22.m) and
+ *       psi22+ (spatial.py:234-414;
+ *       src/internal/finite_depth_directional_dual_branch_green_laplace_psi22.m).
+ *   glw_order3_execute
+ *       eta33+ and psi33+ (_order3.py:18-148; src/internal/gl_no_stokes_eta33.m).
+ *
+ * Conventions: every array argument of an execute call is a device pointer.
+ * Complex arrays are interleaved complex128.  Spectra are given on the parent
+ * box (p1) of the plan, [p1.wy][p1.wx] row-major over signed mode numbers
+ * (mx0.., my0..) of the ny x nx FFT grid.  Outputs are [nt][ny][nx] complex128
+ * physical fields, row = y, already scaled to metres (eta) and m^2/s (psi),
+ * identical in meaning to the reference's *_plus components.  Execute calls
+ * allocate nothing, never synchronise the host, and run on `stream`
+ * (a hipStream_t, or NULL for the default stream).  A plan must not be used
+ * by two host threads or two streams at once.  Every function returns 0 on
+ * success and a nonzero code on error; glw_last_error() returns the text
+ * (thread-local).
+ */
+#ifndef GLW_H
+#define GLW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLW_OK 0
+#define GLW_EINVAL 1
+#define GLW_EDEVICE 2
+#define GLW_EUNSUPPORTED 3
+
+#define GLW_PREC_FP64 0 /* FFTs and products in FP64 */
+#define GLW_PREC_FP32 1 /* FFTs and products in FP32, FP64 accumulation */
+
+typedef struct glw_plan_s glw_plan;
+
+/* Rectangle of signed mode numbers: mx0..mx0+wx-1, my0..my0+wy-1. */
+typedef struct glw_box {
+  int32_t mx0, my0, wx, wy;
+} glw_box;
+
+/* Host FP64 tables on a box, [wy][wx] row-major: qx=h*kx, qy=h*ky, nu =
+ * sqrt(q tanh q), inu = 1/max(nu, DBL_MIN), q2 = q^2, a = q tanh q. */
+typedef struct glw_box_tables {
+  const double *qx, *qy, *nu, *inu, *q2, *a;
+} glw_box_tables;
+
+/* Order-2 stage tables (optional; order >= 2). */
+typedef struct glw_order2_desc {
+  int32_t eta_rank;             /* J of eta22 */
+  const double* eta_damp;       /* [J][p1] exp(-tau_j (nu - c)) */
+  const double* eta_cosh;       /* [J][p2] cosh(sqrt(A) tau_j) */
+  const double* eta_sinhr;      /* [J][p2] sinh(sqrt(A) tau_j)/sqrt(A) */
+  const double* eta_coef;       /* [J] w e^x / lambda / 4 * exp(-2 c tau) */
+  int32_t channels;             /* psi22 radial channels (1 or 2) */
+  const double* psi_damp;       /* [2 branches][2 nodes][channels][p1] channel mask * exp(-tau (nu - c_ch)) */
+  const double* psi_factor;     /* [2][2][channels*channels][p2] output factor, zero off the channel-pair support */
+  const double* psi_coef;       /* [2][2] w e^x / lambda / 8 */
+  const double* psi_sign;       /* [2] sign of K/sqrt(A) per branch */
+} glw_order2_desc;
+
+typedef struct glw_plan_desc {
+  int32_t nx, ny;               /* FFT grid */
+  double depth, gravity;
+  double floor_level;           /* 16 eps log2(nx ny): roundoff floor relative to the support max */
+  int32_t precision;            /* GLW_PREC_* */
+  int32_t max_instants;         /* workspace capacity (instants per internal batch) */
+  int32_t order;                /* 1, 2 or 3 */
+  glw_box p1, p2, p3;           /* parent, pair and triple boxes */
+  int32_t g2x, g2y;             /* working grid of quadratic products */
+  int32_t g3x, g3y;             /* working grid of cubic products */
+  glw_box_tables t1, t2, t3;    /* tables on p1, p2, p3 */
+  const double* omega;          /* [p1] angular frequency (rad/s) of the time phase */
+  const uint8_t* mask2;         /* [p2] pair support */
+  const uint8_t* mask3;         /* [p3] triple support */
+  /* order 3 */
+  int32_t inner_rank, outer_rank;
+  const double* damp_outer;     /* [outer][p1] exp(-tau_o (nu - c3)) */
+  const double* damp_inner;     /* [outer*inner][p1] exp(-tau_o (nu - c3)) exp(-tau_i (nu - c2)) */
+  const double* inner_cosh;     /* [inner][p2] */
+  const double* inner_sinhr;    /* [inner][p2] */
+  const double* inner_coef;     /* [inner] w e^x / lambda2 * exp(-2 c2 tau) / 4 */
+  const double* outer_cosh;     /* [outer][p3] */
+  const double* outer_sinhr;    /* [outer][p3] */
+  const double* outer_coef;     /* [outer] w e^x / lambda3 * exp(-3 c3 tau) */
+  glw_order2_desc o2;
+} glw_plan_desc;
+
+int glw_plan_create(const glw_plan_desc* desc, glw_plan** plan);
+int glw_plan_destroy(glw_plan* plan);
+
+/* eta11+, psi11+ at nt instants.  s0: [p1] complex128 analytic spectrum at
+ * t = 0; times: [nt] seconds. */
+int glw_linear_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt,
+                       double* eta11, double* psi11, void* stream);
+
+/* eta22+, psi22+ at nt instants (plan order >= 2). */
+int glw_order2_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt,
+                       double* eta22, double* psi22, void* stream);
+
+/* eta33+, psi33+ at nt instants (plan order 3). */
+int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt,
+                       double* eta33, double* psi33, void* stream);
+
+/* Device pointer and size of a named workspace buffer (for stage-level tests). */
+int glw_plan_buffer(glw_plan* plan, const char* name, void** ptr, size_t* bytes);
+
+/* Largest FFT length the kernels are instantiated for, and whether a length is one. */
+int glw_fft_length_supported(int32_t n);
+
+const char* glw_last_error(void);
+const char* glw_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLW_H */

@@ -1,0 +1,156 @@
+"""Pins the FP64 oracle (CPU): reference fixtures, closed forms, brute force, homogeneity."""
+
+import json
+import math
+
+import numpy as np
+import pytest
+
+from cases import GOLDEN, fixture, rel
+from oracle import gl_oracle as O
+
+
+def _run_fixture(name, order):
+    d = fixture(name)
+    t = float(d["time"]) if "time" in d else 0.0
+    rank = int(d["rank"]) if "rank" in d else 6
+    r = O.reconstruct_directional_field(d["eta1"], lx=float(d["lx"]), ly=float(d["ly"]), depth=float(d["depth"]),
+                                        gravity=float(d["gravity"]), time=t, order=order, eta22_rank=rank)
+    return d, r["components"]
+
+
+def test_fixture_eta22():
+    # python/tests/test_spatial.py:16-34
+    d, c = _run_fixture("eta22", 2)
+    assert rel(c["eta11"], d["eta1"]) < 2e-13
+    assert rel(c["eta22"], d["eta22"]) < 2e-12
+    assert rel(c["psi11"], d["psi11"]) < 2e-13
+    assert rel(c["psi22"], d["psi22"]) < 3e-12
+
+
+def test_fixture_order3():
+    # python/tests/test_spatial.py:37-49
+    d, c = _run_fixture("order3", 3)
+    assert rel(c["eta33"], d["eta33"]) < 5e-11
+    assert rel(c["psi33"], d["psi33"]) < 5e-11
+
+
+def test_fixture_nonzero_time():
+    # python/tests/test_spatial.py:52-65
+    d, c = _run_fixture("nonzero_time", 3)
+    for k in ("eta11", "psi11", "eta22", "psi22", "eta33", "psi33"):
+        assert rel(c[k], d[k]) < 8e-11, k
+
+
+@pytest.mark.parametrize("rank", list(range(1, 17)))
+def test_gauss_laguerre_factorial_moments(rank):
+    # python/tests/test_quadrature.py:8-14 (all ranks the graphs use)
+    x, w = O.gauss_laguerre_rule(rank)
+    assert np.all(x > 0) and np.all(w > 0)
+    for d in range(2 * rank):
+        assert np.isclose(np.sum(w * x**d), math.factorial(d), rtol=3e-11)
+
+
+def test_gl8_matches_frozen_interface():
+    frozen = json.loads((GOLDEN / "frozen_gl8.json").read_text())
+    x, w = O.gauss_laguerre_rule(8)
+    np.testing.assert_allclose(x, frozen["nodes"], rtol=1e-13)
+    np.testing.assert_allclose(w, frozen["weights"], rtol=1e-11)
+
+
+def test_psi22_two_node_rule_is_gl2():
+    # closed form nodes 2 -/+ sqrt(2) are the Laguerre L2 roots; weights (2 +/- sqrt 2)/4
+    x, w = O.psi22_two_node_rule()
+    xg, wg = O.gauss_laguerre_rule(2)
+    np.testing.assert_allclose(x, xg, rtol=1e-14)
+    np.testing.assert_allclose(w, wg, rtol=1e-14)
+    np.testing.assert_allclose(w, [(2 + math.sqrt(2)) / 4, (2 - math.sqrt(2)) / 4], rtol=1e-14)
+
+
+def test_linear_closed_form():
+    # tests/run_release_tests.m:31-41
+    r = O.gl_spectral_surface(1, 9.81, 1.0, [0.01], [0.0], [2.0], [0.0], 2 * np.pi, 2 * np.pi, 32, 32, 0.0)
+    x = np.arange(32) * (2 * np.pi / 32)
+    assert rel(r["eta"], np.tile(0.01 * np.cos(2 * x), (32, 1))) < 5e-14
+    om = math.sqrt(9.81 * 2 * math.tanh(2.0))
+    assert rel(r["psi"], np.tile(np.real(-1j * 9.81 / om * 0.01 * np.exp(2j * x)), (32, 1))) < 5e-14
+
+
+def _ordered_pair_eta22(spec, qx, qy, peak_q, nodes, weights):
+    """Direct O(M^2) evaluation of the eta22 GL kernel per ordered pair of parents.
+
+    Independent restatement of tests/finite_depth_directional_pure_gl8_ordered_pair.m:20-59:
+    no FFT, no balance shift, scalar products of the two parents.
+    """
+    ny, nx = spec.shape
+    nn = nx * ny
+    active = np.flatnonzero(np.abs(spec) > 1e-13 * max(1.0, np.max(np.abs(spec))))
+    rows, cols = np.unravel_index(active, spec.shape)
+    amp = spec.flat[active] / nn
+    px, py = qx.flat[active], qy.flat[active]
+    q = np.hypot(px, py)
+    nu = np.sqrt(q * np.tanh(q))
+    lam = math.sqrt((2 * math.sqrt(peak_q * math.tanh(peak_q))) ** 2 - 2 * peak_q * math.tanh(2 * peak_q))
+    out = np.zeros_like(spec)
+    for i in range(active.size):
+        for j in range(active.size):
+            dot = px[i] * px[j] + py[i] * py[j]
+            Q = math.hypot(px[i] + px[j], py[i] + py[j])
+            A = Q * math.tanh(Q)
+            a = math.sqrt(A)
+            s = nu[i] + nu[j]
+            sd = nu[i] ** 2 + nu[i] * nu[j] + nu[j] ** 2 - dot / (nu[i] * nu[j])
+            sk = (q[i] ** 2 + dot) / nu[i] + (q[j] ** 2 + dot) / nu[j]
+            ker = 0.0
+            for x, w in zip(nodes, weights):
+                t = x / lam
+                sh = t if a == 0 else math.sinh(a * t) / a
+                ker += w * math.exp(x) / lam * math.exp(-s * t) / 4 * (-A * sd * sh + sk * math.cosh(a * t))
+            out[(rows[i] + rows[j]) % ny, (cols[i] + cols[j]) % nx] += nn * amp[i] * amp[j] * ker
+    return np.fft.ifft2(out)
+
+
+def _release_pair_case():
+    # tests/run_release_tests.m:43-62
+    n = 64
+    mode = np.fft.fftfreq(n) * n
+    mx, my = np.meshgrid(mode, mode)
+    qx, qy = 0.25 * mx, 0.25 * my
+    spec = np.zeros((n, n), dtype=complex)
+    modes = [(3, 0), (4, 1), (5, -1), (6, 2)]
+    amps = [0.017 * np.exp(0.13j), 0.012 * np.exp(-0.41j), 0.009 * np.exp(0.77j), 0.006 * np.exp(-1.04j)]
+    for (a, b), v in zip(modes, amps):
+        spec[b % n, a % n] = n * n * v
+    return spec, qx, qy
+
+
+@pytest.mark.parametrize("rank", [6, 8])
+def test_eta22_ordered_pair_bruteforce(rank):
+    spec, qx, qy = _release_pair_case()
+    fast, _ = O.eta22(spec, qx, qy, rank, peak_q=1.0)
+    x, w = O.gauss_laguerre_rule(rank)
+    direct = _ordered_pair_eta22(spec, qx, qy, 1.0, x, w)
+    assert rel(fast, direct) < 3e-11
+
+
+def test_eta22_ordered_pair_sensitivity():
+    """The brute-force check fails for a plausible mistake: a perturbed node scale (a
+    wrong lambda, 1e-7 relative) or a wrong sign of the dynamic term."""
+    spec, qx, qy = _release_pair_case()
+    fast, _ = O.eta22(spec, qx, qy, 8, peak_q=1.0)
+    x, w = O.gauss_laguerre_rule(8)
+    assert rel(fast, _ordered_pair_eta22(spec, qx, qy, 1.0 + 1e-7, x, w)) > 3e-11
+    assert rel(fast, _ordered_pair_eta22(spec, qx, qy, 1.0, x, -w)) > 1.0
+
+
+@pytest.mark.parametrize("order,power", [(2, 2), (3, 3)])
+def test_homogeneity(order, power):
+    d = fixture("nonzero_time")
+    kw = dict(lx=float(d["lx"]), ly=float(d["ly"]), depth=float(d["depth"]), gravity=float(d["gravity"]),
+              time=0.3, order=order, eta22_rank=8)
+    # scale by a power of two so the check is exact in FP64 (support max stays >= 1)
+    a = O.reconstruct_directional_field(d["eta1"] * 4.0, **kw)["components"]
+    b = O.reconstruct_directional_field(d["eta1"] * 8.0, **kw)["components"]
+    keys = ("eta22", "psi22") if order == 2 else ("eta33", "psi33")
+    for k in keys:
+        assert rel(b[k], 2.0**power * a[k]) < 1e-14, k
