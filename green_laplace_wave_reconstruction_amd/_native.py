@@ -52,8 +52,8 @@ class CPlanDesc(ctypes.Structure):
 
 #: every symbol include/glw.h declares
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
-           "glw_order3_execute", "glw_plan_buffer", "glw_fft_length_supported", "glw_last_error",
-           "glw_version")
+           "glw_order3_execute", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
+           "glw_fft_length_supported", "glw_last_error", "glw_version")
 
 _lib = None
 
@@ -75,6 +75,11 @@ def load_library(path: str | os.PathLike | None = None):
         fn = getattr(lib, name)
         fn.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
         fn.restype = c_int
+    lib.glw_plan_profile.argtypes = [c_void_p, c_int32]
+    lib.glw_plan_profile.restype = c_int
+    lib.glw_plan_stage_times.argtypes = [c_void_p, c_int32, c_char_p, c_size_t, _DP, POINTER(c_int32),
+                                         POINTER(c_int32)]
+    lib.glw_plan_stage_times.restype = c_int
     lib.glw_plan_buffer.argtypes = [c_void_p, c_char_p, POINTER(c_void_p), POINTER(c_size_t)]
     lib.glw_plan_buffer.restype = c_int
     lib.glw_fft_length_supported.argtypes = [c_int32]
@@ -214,6 +219,21 @@ class DevicePlan:
         _check(fn(self._handle, c_void_p(self.s0.data_ptr()), c_void_p(t.data_ptr()), nt,
                   c_void_p(a.data_ptr()), c_void_p(b.data_ptr()), c_void_p(stream)))
         return a, b
+
+    def profile(self, enable: bool = True) -> None:
+        """Bracket every kernel launch of later runs with HIP events on the run's stream."""
+        _check(self.lib.glw_plan_profile(self._handle, int(enable)))
+
+    def stage_times(self) -> dict[str, tuple[float, int]]:
+        """{stage: (total ms, launches)} since the last call (waits for the last event)."""
+        n = 64
+        names = ctypes.create_string_buffer(4096)
+        ms = (c_double * n)()
+        launches = (c_int32 * n)()
+        count = c_int32()
+        _check(self.lib.glw_plan_stage_times(self._handle, n, names, 4096, ms, launches, ctypes.byref(count)))
+        keys = names.value.decode().split(",") if count.value else []
+        return {k: (ms[i], launches[i]) for i, k in enumerate(keys)}
 
     def buffer(self, name: str):
         ptr, size = c_void_p(), c_size_t()

@@ -404,9 +404,31 @@ struct glw_plan_s {
   double *p22_damp = nullptr, *p22_fac = nullptr, *p22_coef = nullptr, *p22_sign = nullptr;
   // workspace: name -> (ptr, bytes)
   std::map<std::string, std::pair<void*, size_t>> ws;
+  // stage timing
+  std::vector<hipEvent_t> ev;
+  std::vector<const char*> ev_name;
+  size_t ev_used = 0;
+  size_t ev_dropped = 0;
 
   ~glw_plan_s() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
+  }
+
+  // Bracket one launch with two events (no-op unless profiling is enabled).
+  void mark_begin(const char* name, hipStream_t st) {
+    if (ev.empty()) return;
+    if (ev_used + 2 > ev.size()) {
+      ++ev_dropped;
+      return;
+    }
+    ev_name[ev_used / 2] = name;
+    (void)hipEventRecord(ev[ev_used], st);
+  }
+  void mark_end(hipStream_t st) {
+    if (ev.empty() || ev_used + 2 > ev.size()) return;
+    (void)hipEventRecord(ev[ev_used + 1], st);
+    ev_used += 2;
   }
 
   template <typename X>
@@ -590,6 +612,20 @@ int fill_zero(uint64_t* p, size_t n, hipStream_t st) {
   return GLW_OK;
 }
 
+// Launch with stage-timing brackets; evaluates to the launch's return code.
+#define GLW_STAGE(P, name, st, expr) \
+  ([&]() -> int {                    \
+    (P)->mark_begin(name, st);       \
+    int rc_ = (expr);                \
+    (P)->mark_end(st);               \
+    return rc_;                      \
+  }())
+
+int launch_plain(hipError_t e) {
+  if (e != hipSuccess) return fail(GLW_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+  return GLW_OK;
+}
+
 ColInvArgs colinv_base(const cpx<double>* spec, int nsrc, int sdiv, const glw_box& box, const DevTables& t) {
   ColInvArgs a;
   memset(&a, 0, sizeof(a));
@@ -625,7 +661,7 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
   a.out_nfield = nout;
   a.out_foff = 0;
   a.ostride = box.wx;
-  int rc = launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), st);
+  int rc = GLW_STAGE(P, "synth_colinv", st, launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), st));
   if (rc) return rc;
   for (int f = 0; f < nout; ++f) {
     // rows of field f of every instant: sequence t starts at (t*nout + f)*ny*wx
@@ -637,15 +673,17 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
     s.wx = box.wx;
     s.gy = d.ny;
     s.out = (cpx<double>*)outs[f];
-    if ((rc = launch_synth(d.nx, s, nt, P->tw(d.nx, false), st))) return rc;
+    if ((rc = GLW_STAGE(P, "synth_row", st, launch_synth(d.nx, s, nt, P->tw(d.nx, false), st)))) return rc;
   }
   return GLW_OK;
 }
 
 int phase(glw_plan* P, const double* s0, const double* times, int nt, hipStream_t st) {
   const int b1 = box_size(P->d.p1);
+  P->mark_begin("phase", st);
   hipLaunchKernelGGL(k_phase, dim3((b1 + 255) / 256, nt), dim3(256), 0, st, (const cpx<double>*)s0, P->omega,
                      times, b1, P->buf<cpx<double>>("st"));
+  P->mark_end(st);
   GLW_HIP(hipGetLastError());
   return GLW_OK;
 }
@@ -675,7 +713,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("tin").first;
     a.out_nfield = 12;
     a.ostride = d.p1.wx;
-    if ((rc = launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), st))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_colinv", st, launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), st)))) return rc;
   }
   {
     RowArgs r;
@@ -693,7 +731,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.omx0 = d.p2.mx0;
     r.owx = d.p2.wx;
     r.gy = d.g2y;
-    if ((rc = launch_row<T, 12, 4, InnerOp>(d.g2x, r, n_in, P->tw(d.g2x, f32), st))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_row", st, (launch_row<T, 12, 4, InnerOp>(d.g2x, r, n_in, P->tw(d.g2x, f32), st))))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_in"), (size_t)n_in * 4, st))) return rc;
   {
@@ -709,7 +747,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g2x * d.g2y);
     c.out = P->buf<cpx<double>>("sin");
     c.fmax = P->buf<uint64_t>("max_in");
-    if ((rc = launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), st))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), st)))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_et"), (size_t)n_to, st))) return rc;
   {
@@ -726,7 +764,9 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.coef = P->in_coef;
     g.out = P->buf<cpx<double>>("pair");
     g.etmax = P->buf<uint64_t>("max_et");
+    P->mark_begin("o3_inner_acc", st);
     hipLaunchKernelGGL(k_acc_inner, dim3((b2 + NT - 1) / NT, n_to), dim3(NT), 0, st, g);
+    P->mark_end(st);
     GLW_HIP(hipGetLastError());
   }
   // ---- outer stage: tasks (t, o)
@@ -748,7 +788,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out_nfield = 21;
     a.out_foff = 0;
     a.ostride = wmax;
-    if ((rc = launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st)))) return rc;
   }
   {
     ColInvArgs a = colinv_base(P->buf<cpx<double>>("pair"), 4, 1, d.p2, P->t2);
@@ -769,7 +809,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out_nfield = 21;
     a.out_foff = 12;
     a.ostride = wmax;
-    if ((rc = launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st)))) return rc;
   }
   {
     RowArgs r;
@@ -787,7 +827,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.omx0 = d.p3.mx0;
     r.owx = d.p3.wx;
     r.gy = d.g3y;
-    if ((rc = launch_row<T, 21, 4, OuterOp>(d.g3x, r, n_to, P->tw(d.g3x, f32), st))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_row", st, (launch_row<T, 21, 4, OuterOp>(d.g3x, r, n_to, P->tw(d.g3x, f32), st))))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_out"), (size_t)n_to * 4, st))) return rc;
   if ((rc = fill_zero(P->buf<uint64_t>("max_combo"), (size_t)n_to * 2, st))) return rc;
@@ -804,7 +844,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g3x * d.g3y);
     c.out = P->buf<cpx<double>>("sout");
     c.fmax = P->buf<uint64_t>("max_out");
-    if ((rc = launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), st))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), st)))) return rc;
   }
   {
     ComboArgs g;
@@ -815,7 +855,9 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.a = P->t3.a;
     g.mask = P->mask3;
     g.cmax = P->buf<uint64_t>("max_combo");
+    P->mark_begin("o3_outer_combo", st);
     hipLaunchKernelGGL(k_combo_outer, dim3((b3 + NT - 1) / NT, n_to), dim3(NT), 0, st, g);
+    P->mark_end(st);
     GLW_HIP(hipGetLastError());
   }
   {
@@ -832,7 +874,9 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.sh = P->out_sh;
     g.coef = P->out_coef;
     g.out = P->buf<cpx<double>>("ep33");
+    P->mark_begin("o3_outer_acc", st);
     hipLaunchKernelGGL(k_acc_outer, dim3((b3 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
+    P->mark_end(st);
     GLW_HIP(hipGetLastError());
   }
   const double h = d.depth, pot = d.depth * std::sqrt(d.gravity * d.depth), nn = (double)d.nx * d.ny;
@@ -900,13 +944,13 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("e22_t").first;
     a.out_nfield = 8;
     a.ostride = d.p1.wx;
-    if ((rc = launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), st))) return rc;
-    if ((rc = launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), st)))
-      return rc;
+    if ((rc = GLW_STAGE(P, "o2_eta_colinv", st, launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o2_eta_row", st, (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), st)))
+     )) return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("e22_max"), (size_t)nt * J * 2, st))) return rc;
-    if ((rc = launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
+    if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
                                P->tw(d.g2y, f32), st)))
-      return rc;
+     ) return rc;
     AccEta22Args g;
     g.spec = P->buf<cpx<double>>("e22_s");
     g.fmax = P->buf<uint64_t>("e22_max");
@@ -918,7 +962,9 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.sh = P->e22_sh;
     g.coef = P->e22_coef;
     g.out = P->buf<cpx<double>>("ep22");
+    P->mark_begin("o2_eta_acc", st);
     hipLaunchKernelGGL(k_acc_eta22, dim3((b2 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
+    P->mark_end(st);
     GLW_HIP(hipGetLastError());
   }
   // ---- psi22: tasks (t, branch*2 + node), two channels of 8 fields each
@@ -943,12 +989,12 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
       a.out_nfield = 16;
       a.out_foff = 8 * ch;
       a.ostride = d.p1.wx;
-      if ((rc = launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), st))) return rc;
+      if ((rc = GLW_STAGE(P, "o2_psi_colinv", st, launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), st)))) return rc;
     }
-    if ((rc = launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), st)))
-      return rc;
-    if ((rc = launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), st)))
-      return rc;
+    if ((rc = GLW_STAGE(P, "o2_psi_row", st, (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), st)))
+     )) return rc;
+    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), st)))
+     ) return rc;
   }
   // ---- psi22 Taylor contact: tasks t
   {
@@ -963,11 +1009,11 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("c22_t").first;
     a.out_nfield = 2;
     a.ostride = d.p1.wx;
-    if ((rc = launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), st))) return rc;
-    if ((rc = launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), st)))
-      return rc;
-    if ((rc = launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), st)))
-      return rc;
+    if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o2_contact_row", st, (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), st)))
+     )) return rc;
+    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), st)))
+     ) return rc;
   }
   {
     AccPsi22Args g;
@@ -980,7 +1026,9 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.coef = P->p22_coef;
     g.sign = P->p22_sign;
     g.out = P->buf<cpx<double>>("ep22");
+    P->mark_begin("o2_psi_acc", st);
     hipLaunchKernelGGL(k_acc_psi22, dim3((b2 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
+    P->mark_end(st);
     GLW_HIP(hipGetLastError());
   }
   const double h = d.depth, pot = d.depth * std::sqrt(d.gravity * d.depth), nn = (double)d.nx * d.ny;
@@ -1073,6 +1121,58 @@ int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, in
                        double* psi33, void* stream) {
   if (plan && plan->d.order < 3) return fail(GLW_EINVAL, "plan built without order-3 tables");
   return run_chunks(plan, s0, times, nt, eta33, psi33, stream, order3_dispatch);
+}
+
+int glw_plan_profile(glw_plan* plan, int32_t enable) {
+  if (!plan) return fail(GLW_EINVAL, "null plan");
+  GLW_HIP(hipSetDevice(plan->device));
+  for (hipEvent_t e : plan->ev) (void)hipEventDestroy(e);
+  plan->ev.clear();
+  plan->ev_name.clear();
+  plan->ev_used = plan->ev_dropped = 0;
+  if (!enable) return GLW_OK;
+  const size_t n = 8192;
+  plan->ev.resize(n);
+  plan->ev_name.resize(n / 2);
+  for (size_t i = 0; i < n; ++i) GLW_HIP(hipEventCreate(&plan->ev[i]));
+  return GLW_OK;
+}
+
+int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_len, double* ms,
+                         int32_t* launches, int32_t* count) {
+  if (!plan || !names || !ms || !launches || !count || max < 1 || names_len < 1) return fail(GLW_EINVAL, "null argument");
+  *count = 0;
+  names[0] = 0;
+  if (plan->ev.empty()) return fail(GLW_EINVAL, "profiling is not enabled on this plan");
+  std::vector<std::string> order;
+  std::map<std::string, std::pair<double, int>> agg;
+  if (plan->ev_used) GLW_HIP(hipEventSynchronize(plan->ev[plan->ev_used - 1]));
+  for (size_t i = 0; i + 1 < plan->ev_used; i += 2) {
+    float t = 0.f;
+    GLW_HIP(hipEventElapsedTime(&t, plan->ev[i], plan->ev[i + 1]));
+    std::string nm = plan->ev_name[i / 2];
+    if (!agg.count(nm)) order.push_back(nm);
+    agg[nm].first += t;
+    agg[nm].second += 1;
+  }
+  std::string csv;
+  int k = 0;
+  for (const std::string& nm : order) {
+    if (k >= max) break;
+    ms[k] = agg[nm].first;
+    launches[k] = agg[nm].second;
+    csv += (k ? "," : "") + nm;
+    ++k;
+  }
+  if (csv.size() + 1 > names_len) return fail(GLW_EINVAL, "stage name buffer too small");
+  memcpy(names, csv.c_str(), csv.size() + 1);
+  *count = k;
+  plan->ev_used = 0;
+  if (plan->ev_dropped) {
+    plan->ev_dropped = 0;
+    return fail(GLW_EINVAL, "event pool exhausted: some launches were not timed");
+  }
+  return GLW_OK;
 }
 
 int glw_plan_buffer(glw_plan* plan, const char* name, void** ptr, size_t* bytes) {

@@ -119,6 +119,17 @@ int glw_order2_execute(glw_plan* plan, const double* s0, const double* times, in
 int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt,
                        double* eta33, double* psi33, void* stream);
 
+/* Stage timing (instrumentation; no reference counterpart).  enable != 0
+ * creates a pool of HIP events; every kernel launch of later execute calls is
+ * bracketed by two events on the execute stream.  glw_plan_stage_times waits
+ * for the last event, aggregates elapsed times per stage name and resets the
+ * pool: names is filled with "name,name,..." (NUL-terminated), ms[i] and
+ * launches[i] receive the total time and launch count of stage i, *count the
+ * number of stages (at most max). */
+int glw_plan_profile(glw_plan* plan, int32_t enable);
+int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_len, double* ms,
+                         int32_t* launches, int32_t* count);
+
 /* Device pointer and size of a named workspace buffer (for stage-level tests). */
 int glw_plan_buffer(glw_plan* plan, const char* name, void** ptr, size_t* bytes);
 
