@@ -1,0 +1,195 @@
+"""Headline benchmark: eta33+psi33 fields/s at 512^2 on one MI355X.
+
+Workload (one "step"): the GL order-3 graph (inner GL4 x outer GL4) of one
+seeded JONSWAP sea state -- 512^2, L=2000 m, h=30 m, Hs=6 m, Tp=12 s,
+gamma=3.3, s=15, 3120 strict-forward parents with kh > 0.5 -- at 64 instants
+uniform over [0, 10 Tp], from the t=0 analytic spectrum resident in HBM to
+the eta33+ and psi33+ physical fields in HBM.  One unit = one (eta33, psi33)
+pair at one instant.
+
+Roofline: algorithmic bytes follow the reference's transform count: 358
+full-grid 2-D transforms per field pair, each = 2 sweeps x (1 read + 1 write)
+of N^2 complex values; a kernel is credited with the sweeps of the reference
+graph it performs.  The kernels evaluate those sweeps on alias-free reduced
+grids, so the achieved algorithmic bandwidth can exceed the HBM peak; the
+measured HBM traffic (rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/) is reported
+beside it when available.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64|fp32] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+INSTANTS = 64
+
+# reference-graph sweeps performed per launch unit of each stage (see module doc)
+SWEEPS = {
+    "o3_inner_colinv": ("inner", 12), "o3_inner_row": ("inner", 16), "o3_inner_colfwd": ("inner", 4),
+    "o3_outer_colinv": ("outer", 21), "o3_outer_row": ("outer", 25), "o3_outer_colfwd": ("outer", 4),
+    "synth_colinv": ("instant", 2), "synth_row": ("instant", 2),
+}
+
+
+def workload():
+    from green_laplace_wave_reconstruction_amd import api, plan as PL, seastate
+
+    s = seastate.order3_headline_512()
+    import green_laplace_wave_reconstruction_amd as G
+
+    c = G.gl_spectral_coefficients(3, s.gravity, s.depth, s.a, s.b, s.kx, s.ky, 0, 0)
+    kxg, kyg = api._wavenumber_grid(s.lx, s.ly, s.nx, s.ny)
+    spec, omega = api.deposit_analytic_input(c, s.lx, s.ly, s.nx, s.ny)
+    return s, c, kxg, kyg, spec, omega
+
+
+def traffic_from_profiles(kernel: str):
+    """Measured HBM bytes per dispatch of `kernel` from a committed PMC summary, or None."""
+    p = ROOT / "profiles" / "pmc_traffic.json"
+    if not p.exists():
+        return None
+    try:
+        data = json.loads(p.read_text())
+        return data.get(kernel, {}).get("hbm_bytes_per_dispatch")
+    except Exception:
+        return None
+
+
+def cpu_baseline(s, c, kxg, kyg, times_sample, gpu_fields=None):
+    """FP64 NumPy oracle (1 core: numpy.fft is single-threaded) on a bounded sample of instants."""
+    from oracle import gl_oracle as O
+
+    k = np.hypot(s.kx, s.ky)
+    om = np.sqrt(s.gravity * k * np.tanh(s.depth * k))
+    qx, qy = s.depth * kxg, s.depth * kyg
+    pot = s.depth * math.sqrt(s.gravity * s.depth)
+    elapsed = 0.0
+    errs = {"eta33": 0.0, "psi33": 0.0}
+    for i, t in enumerate(times_sample):
+        spec = O.deposit(s.a, s.b, s.kx, s.ky, om, s.depth, s.lx, s.ly, s.nx, s.ny, float(t))
+        t0 = time.perf_counter()
+        e, p, _ = O.order3(spec, qx, qy, peak_q=c["peak_depth"])
+        elapsed += time.perf_counter() - t0
+        if gpu_fields is not None:
+            ge, gp = gpu_fields[i]
+            errs["eta33"] = max(errs["eta33"], float(np.linalg.norm(ge - s.depth * e) / np.linalg.norm(s.depth * e)))
+            errs["psi33"] = max(errs["psi33"], float(np.linalg.norm(gp - pot * p) / np.linalg.norm(pot * p)))
+    return len(times_sample) / elapsed, errs
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"))
+    ap.add_argument("--cpu-sample", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs on one GPU only: the GL graph of one sea state is timed on one "
+                                   "MI355X; --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from green_laplace_wave_reconstruction_amd import plan as PL
+    from green_laplace_wave_reconstruction_amd._native import DevicePlan
+
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no HIP device"}))
+        sys.exit(1)
+    s, c, kxg, kyg, spec, omega = workload()
+    prec = PL.PREC_FP64 if args.precision == "fp64" else PL.PREC_FP32
+    host = PL.build_plan(spec, s.depth * kxg, s.depth * kyg, depth=s.depth, gravity=s.gravity, order=3,
+                         peak_q=c["peak_depth"], precision=prec)
+    host.omega = host.box1.take(omega)
+    dp = DevicePlan(host, host.box1.take(spec), max_instants=INSTANTS)
+    times = torch.linspace(0.0, 10.0 * 12.0, INSTANTS, dtype=torch.float64, device=dp.device)
+    shape = (INSTANTS, s.ny, s.nx)
+    eta = torch.empty(shape, dtype=torch.complex128, device=dp.device)
+    psi = torch.empty(shape, dtype=torch.complex128, device=dp.device)
+    for _ in range(args.warmup):
+        dp.run("order3", times, out=(eta, psi))
+    torch.cuda.synchronize()
+    dp.profile(True)
+    dp.stage_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dp.run("order3", times, out=(eta, psi))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stages = dp.stage_times()
+    ms_per_step = wall / args.steps * 1e3
+    value = INSTANTS * args.steps / wall
+
+    b = 16 if args.precision == "fp64" else 8
+    n2 = s.nx * s.ny
+    sweep_bytes = 2 * n2 * b
+    units = {"inner": INSTANTS * 16, "outer": INSTANTS * 4, "instant": INSTANTS}
+    dom = max((k for k in stages if k in SWEEPS), key=lambda k: stages[k][0])
+    dom_ms, dom_launches = stages[dom]
+    dom_avg_s = dom_ms / dom_launches / 1e3
+    unit_kind, sweeps = SWEEPS[dom]
+    alg_bytes = sweeps * units[unit_kind] * sweep_bytes
+    achieved = alg_bytes / dom_avg_s / 1e9
+    traffic = traffic_from_profiles(dom)
+
+    sample_idx = np.linspace(0, INSTANTS - 1, max(1, args.cpu_sample)).round().astype(int)
+    cpu = None
+    errs = None
+    if not args.no_cpu:
+        tt = times.cpu().numpy()
+        gpu_fields = [(eta[i].cpu().numpy(), psi[i].cpu().numpy()) for i in sample_idx]
+        cpu_rate, errs = cpu_baseline(s, c, kxg, kyg, tt[sample_idx], gpu_fields)
+        cpu = {"value": cpu_rate, "unit": "fields/s", "cores": 1, "kind": "port",
+               "sample": f"{len(sample_idx)} of the {INSTANTS} instants (indices {sample_idx.tolist()}), "
+                         "FP64 NumPy oracle restatement of green_laplace_order3, numpy.fft single-threaded"}
+    result = {
+        "metric": "eta33+psi33 fields/s at 512², FP32-acc vs FP64 at rel-L2≤1e-5; %HBM roofline",
+        "value": value,
+        "unit": "fields/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "data": "synthetic seeded JONSWAP sea state (numpy default_rng seed 2)",
+        "config": {
+            "workload": "GL order 3 (inner GL4 x outer GL4) eta33+psi33, 512x512, L=2000 m, h=30 m, Hs=6 m, "
+                        "Tp=12 s, gamma=3.3, s=15, 3120 parents kh>0.5, 64 instants over [0, 120 s]",
+            "instants_per_step": INSTANTS, "grid": [s.ny, s.nx], "precision": args.precision,
+            "working_grids": {"quadratic": list(host.g2[::-1]), "cubic": list(host.g3[::-1])},
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_avg_s * 1e3,
+        },
+        "pipeline_alg_GBps": 716 * sweep_bytes * value / 1e9,
+        "stages_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
+        "rel_l2_vs_oracle": errs,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()
