@@ -52,7 +52,8 @@ class CPlanDesc(ctypes.Structure):
 
 #: every symbol include/glw.h declares
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
-           "glw_order3_execute", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
+           "glw_order3_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
+           "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
            "glw_fft_length_supported", "glw_last_error", "glw_version")
 
 _lib = None

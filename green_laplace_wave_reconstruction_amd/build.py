@@ -10,7 +10,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SRC = PKG / "csrc" / "glw_plan.hip"
+SOURCES = [PKG / "csrc" / "glw_plan.hip", PKG / "csrc" / "glw_wit.hip"]
 OUT = PKG / "_lib" / "libglw.so"
 
 HIPCC_FLAGS = [
@@ -31,7 +31,7 @@ def hipcc() -> str:
 def build(verbose: bool = False) -> Path:
     OUT.parent.mkdir(parents=True, exist_ok=True)
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), str(SRC), "-o", str(tmp)]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), *[str(x) for x in SOURCES], "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

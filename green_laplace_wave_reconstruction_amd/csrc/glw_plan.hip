@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "glw.h"
+#include "glw_internal.hpp"
 #include "glw_kernels.hpp"
 #include "glw_ops.hpp"
 
@@ -388,6 +389,8 @@ struct DevTables {
 int box_size(const glw_box& b) { return b.wx * b.wy; }
 
 }  // namespace
+
+int glw_report_error(int code, const std::string& msg) { return fail(code, msg); }
 
 struct glw_plan_s {
   glw_plan_desc d{};

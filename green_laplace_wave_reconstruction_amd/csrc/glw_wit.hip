@@ -1,0 +1,615 @@
+// WIT sector-44 quartet sum on gfx950.
+//
+// The reference (paper/order4/cpp/directional_eta4_openmp.cpp:157-453)
+// evaluates, for every unordered quartet, the states of all 15 subsets of its
+// four parents in increasing degree.  A subset's state depends only on the
+// multiset of its parents, so this build tabulates the states of every single,
+// pair (i<=j) and triple (i<=j<=k) once and evaluates only the degree-4 level
+// per quartet.
+//
+// Partition sums.  The reference sums over ordered set partitions with
+// labelled blocks: block 0 carries the potential phi, blocks 1..n the
+// elevations eta, weighted 1/n!.  For a fixed potential block P the n!
+// labellings of one unordered partition of the complement C = S\P cancel the
+// 1/n!, so with E_n(C) = sum over unordered partitions of C into n blocks of
+// prod eta[block]:
+//   taylor    = sum_P phi[P] sum_n zd(K_P, n)   E_n(C)
+//   lowerVert = sum_P phi[P] sum_n zd(K_P, n+1) E_n(C)
+//   lowerTime = sum_P (-i Omega_P) phi[P] sum_n zd(K_P, n) E_n(C)
+//   surfaceH  = sum_{P subset of S} i K_P zd(K_P, n) phi[P] E_n(C), sum over n (n=0 for P=S)
+//   surfaceV  = sum_{P subset of S} zd(K_P, n+1) phi[P] E_n(C)
+// The ordinary source (:264-282) keeps the reference's ordered two-block sum.
+// In sector 44 every parent sign is +1, so charge = degree: the retained-V3
+// push (:284-298) and the y31 coordinate (:300-336) vanish; only the
+// degree-4 near-resonant branch (:393-411) retains a V3 term.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "glw.h"
+#include "glw_fft.hpp"
+#include "glw_internal.hpp"
+
+using glw::cpx;
+
+namespace {
+
+int wfail(int code, const std::string& m) { return glw_report_error(code, m); }
+#define W_HIP(call)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) return wfail(GLW_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int WNT = 256;
+
+struct Rec {
+  cpx<double> eta, phi, sh0, sh1, sv;
+  double kx, ky, om, q, k2;
+  double zero_src;  // strict-zero source magnitude of this subset (0 if not strict zero)
+  double zero;      // 1 if this subset is a strict zero, else 0
+};
+
+__device__ __forceinline__ cpx<double> cm(cpx<double> a, cpx<double> b) { return glw::cmul(a, b); }
+__device__ __forceinline__ cpx<double> ca(cpx<double> a, cpx<double> b) { return glw::cadd(a, b); }
+__device__ __forceinline__ cpx<double> cs(cpx<double> a, double s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ cpx<double> ci(double s, cpx<double> a) { return {-s * a.y, s * a.x}; }  // (i s) a
+__device__ __forceinline__ double cabs2(cpx<double> a) { return hypot(a.x, a.y); }
+
+__host__ __device__ constexpr int popc(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1); }
+__host__ __device__ constexpr int lowbit(int m) { return m & (-m); }
+
+__device__ __forceinline__ double zd(const Rec& r, int n) {
+  switch (n) {
+    case 0: return 1.0;
+    case 1: return r.q;
+    case 2: return r.k2;
+    case 3: return r.k2 * r.q;
+    default: return r.k2 * r.k2;
+  }
+}
+
+struct Model {
+  double g, h, thr;
+  __device__ double q(double kx, double ky, double* k2) const {
+    const double k = hypot(kx, ky);
+    *k2 = k * k;
+    return k * tanh(h * k);
+  }
+};
+
+struct TopOut {
+  cpx<double> eta, psi, phi, src0, src1, retained;
+  double rho;
+  bool zero, near;
+  double zero_src;
+};
+
+// State of the full set FULL = (1<<D)-1 from the records of its proper
+// subsets.  `rec(m)` returns the record of local sub-mask m.
+template <int D, class Acc>
+__device__ __forceinline__ void subset_state(const Acc& rec, const Rec& S, const Model& md, TopOut& o) {
+  constexpr int FULL = (1 << D) - 1;
+  cpx<double> horiz = {0, 0}, kin = {0, 0}, taylor = {0, 0}, lv = {0, 0}, lt = {0, 0};
+#pragma unroll
+  for (int L = 1; L < FULL; ++L) {
+    const int R = FULL ^ L;
+    const Rec& a = rec(L);
+    const Rec& b = rec(R);
+    horiz = ca(horiz, cm(ca(ci(b.kx, a.sh0), ci(b.ky, a.sh1)), b.eta));
+    kin = ca(kin, cs(ca(ca(cm(a.sh0, b.sh0), cm(a.sh1, b.sh1)), cm(a.sv, b.sv)), 0.5));
+  }
+#pragma unroll
+  for (int P = 1; P < FULL; ++P) {
+    const int C = FULL ^ P;
+    const Rec& p = rec(P);
+    cpx<double> t = {0, 0}, v = {0, 0};
+    const cpx<double> e1 = rec(C).eta;
+    t = ca(t, cs(e1, zd(p, 1)));
+    v = ca(v, cs(e1, zd(p, 2)));
+    if constexpr (true) {
+      if (popc(C) == 2) {
+        const int a = lowbit(C), b = C ^ a;
+        const cpx<double> e2 = cm(rec(a).eta, rec(b).eta);
+        t = ca(t, cs(e2, zd(p, 2)));
+        v = ca(v, cs(e2, zd(p, 3)));
+      } else if (popc(C) == 3) {
+        const int a = lowbit(C), b = lowbit(C ^ a), c = C ^ a ^ b;
+        const cpx<double> e2 = ca(ca(cm(rec(a).eta, rec(b | c).eta), cm(rec(b).eta, rec(a | c).eta)),
+                                  cm(rec(c).eta, rec(a | b).eta));
+        const cpx<double> e3 = cm(cm(rec(a).eta, rec(b).eta), rec(c).eta);
+        t = ca(ca(t, cs(e2, zd(p, 2))), cs(e3, zd(p, 3)));
+        v = ca(ca(v, cs(e2, zd(p, 3))), cs(e3, zd(p, 4)));
+      }
+    }
+    const cpx<double> pt = cm(p.phi, t);
+    taylor = ca(taylor, pt);
+    lv = ca(lv, cm(p.phi, v));
+    lt = ca(lt, ci(-p.om, pt));
+  }
+  // source = [-(horiz - lv) - q taylor, -(lt + kin) - i Omega taylor]
+  o.src0 = {-(horiz.x - lv.x) - S.q * taylor.x, -(horiz.y - lv.y) - S.q * taylor.y};
+  const cpx<double> it = ci(S.om, taylor);
+  o.src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
+  const double mag = sqrt(S.k2);
+  o.zero = mag <= 100.0 * 2.220446049250313e-16 * fmax(mag, 1.0);
+  o.near = false;
+  o.rho = 0.0;
+  o.retained = {0, 0};
+  o.zero_src = 0.0;
+  if (o.zero) {
+    o.eta = o.psi = {0, 0};
+    o.zero_src = hypot(cabs2(o.src0), cabs2(o.src1));
+  } else {
+    bool solve = true;
+    if constexpr (D == 4) {
+      const double lin = md.g * S.q, tmp = S.om * S.om;
+      o.rho = fabs(lin - tmp) / fmax(lin + tmp, 2.2250738585072014e-308);
+      if (o.rho < md.thr) {
+        const double w = sqrt(lin), ms = sqrt(w / (2.0 * md.g)), ns = sqrt(md.g / (2.0 * w));
+        const double sg = S.om >= 0.0 ? 1.0 : -1.0;
+        const cpx<double> a0 = cs(o.src0, 1.0 / (2.0 * ms));
+        const cpx<double> a1 = ci(sg, cs(o.src1, 1.0 / (2.0 * ns)));
+        o.retained = ca(a0, a1);
+        const cpx<double> comp = glw::csub(a0, a1);
+        const double den = S.om + sg * w;
+        o.eta = cs(ci(ms, comp), 1.0 / den);
+        o.psi = cs(comp, -sg * ns / den);
+        o.near = true;
+        solve = false;
+      }
+    }
+    if (solve) {
+      const double den = S.om * S.om - md.g * S.q;
+      const cpx<double> a = glw::csub(ci(S.om, o.src0), cs(o.src1, S.q));
+      const cpx<double> b = ca(cs(o.src0, md.g), ci(S.om, o.src1));
+      o.eta = cs(a, 1.0 / den);
+      o.psi = cs(b, 1.0 / den);
+    }
+  }
+  o.phi = glw::csub(o.psi, taylor);
+}
+
+// Surface derivatives of the full set (degree < 4), reference :202-225.
+template <int D, class Acc>
+__device__ __forceinline__ void surface(const Acc& rec, const Rec& S, cpx<double> phiS, Rec& out) {
+  constexpr int FULL = (1 << D) - 1;
+  cpx<double> h0 = ci(S.kx, phiS), h1 = ci(S.ky, phiS), sv = cs(phiS, S.q);
+#pragma unroll
+  for (int P = 1; P < FULL; ++P) {
+    const int C = FULL ^ P;
+    const Rec& p = rec(P);
+    cpx<double> e[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    e[1] = rec(C).eta;
+    if (popc(C) == 2) {
+      const int a = lowbit(C), b = C ^ a;
+      e[2] = cm(rec(a).eta, rec(b).eta);
+    } else if (popc(C) == 3) {
+      const int a = lowbit(C), b = lowbit(C ^ a), c = C ^ a ^ b;
+      e[2] = ca(ca(cm(rec(a).eta, rec(b | c).eta), cm(rec(b).eta, rec(a | c).eta)), cm(rec(c).eta, rec(a | b).eta));
+      e[3] = cm(cm(rec(a).eta, rec(b).eta), rec(c).eta);
+    }
+#pragma unroll
+    for (int n = 1; n <= 3; ++n) {
+      if (n > popc(C)) break;
+      const cpx<double> c = cm(p.phi, e[n]);
+      const double z = zd(p, n);
+      h0 = ca(h0, ci(p.kx * z, c));
+      h1 = ca(h1, ci(p.ky * z, c));
+      sv = ca(sv, cs(c, zd(p, n + 1)));
+    }
+  }
+  out.sh0 = h0;
+  out.sh1 = h1;
+  out.sv = sv;
+}
+
+struct WitDev {
+  int M, nx, ny;
+  Model md;
+  const int* modes;      // [M][2]
+  Rec* single;           // [M]
+  Rec* pair;             // [M(M+1)/2]
+  Rec* triple;           // [C(M+2,3)]
+  const long long* offP; // [M+1]  pairs with first index < i
+  const long long* offT; // [M+1]  triples with first index < i
+};
+
+__device__ __forceinline__ long long pidx(const WitDev& w, int i, int j) { return w.offP[i] + (j - i); }
+__device__ __forceinline__ long long tidx(const WitDev& w, int i, int j, int k) {
+  return w.offT[i] + (w.offP[j] - w.offP[i]) + (k - j);
+}
+
+__device__ void base_record(const WitDev& w, const Rec** sub, int D, Rec& S) {
+  double kx = 0, ky = 0, om = 0;
+  for (int s = 0; s < D; ++s) {
+    kx += sub[s]->kx;
+    ky += sub[s]->ky;
+    om += sub[s]->om;
+  }
+  S.kx = kx;
+  S.ky = ky;
+  S.om = om;
+  S.q = w.md.q(kx, ky, &S.k2);
+}
+
+__global__ void k_wit_single_fill(WitDev w, double dkx, double dky) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= w.M) return;
+  Rec r;
+  r.kx = w.modes[2 * i] * dkx;
+  r.ky = w.modes[2 * i + 1] * dky;
+  r.q = w.md.q(r.kx, r.ky, &r.k2);
+  r.om = sqrt(w.md.g * r.q);
+  r.eta = {0.5, 0.0};
+  r.phi = {0.0, -r.om / (2.0 * r.q)};
+  r.sh0 = ci(r.kx, r.phi);
+  r.sh1 = ci(r.ky, r.phi);
+  r.sv = cs(r.phi, r.q);
+  r.zero = 0.0;
+  r.zero_src = 0.0;
+  w.single[i] = r;
+}
+
+struct Acc2 {
+  const Rec* r[4];
+  __device__ const Rec& operator()(int m) const { return *r[m]; }
+};
+struct Acc3 {
+  const Rec* r[8];
+  __device__ const Rec& operator()(int m) const { return *r[m]; }
+};
+struct Acc4 {
+  const Rec* r[16];
+  __device__ const Rec& operator()(int m) const { return *r[m]; }
+};
+
+__global__ void k_wit_pairs(WitDev w) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long np = (long long)w.M * (w.M + 1) / 2;
+  if (p >= np) return;
+  // unrank p -> (i, j): largest i with offP[i] <= p
+  int lo = 0, hi = w.M - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (w.offP[mid] <= p) lo = mid; else hi = mid - 1;
+  }
+  const int i = lo, j = i + (int)(p - w.offP[i]);
+  Acc2 a;
+  a.r[1] = &w.single[i];
+  a.r[2] = &w.single[j];
+  const Rec* sub[2] = {a.r[1], a.r[2]};
+  Rec S;
+  base_record(w, sub, 2, S);
+  TopOut o;
+  subset_state<2>(a, S, w.md, o);
+  S.eta = o.eta;
+  S.phi = o.phi;
+  S.zero = o.zero ? 1.0 : 0.0;
+  S.zero_src = o.zero_src;
+  surface<2>(a, S, o.phi, S);
+  w.pair[p] = S;
+}
+
+__global__ void k_wit_triples(WitDev w, long long nt) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  int lo = 0, hi = w.M - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (w.offT[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const int i = lo;
+  // within first index i: pair rank among (j,k), i<=j<=k
+  const long long r = t - w.offT[i] + w.offP[i];
+  int lj = i, hj = w.M - 1;
+  while (lj < hj) {
+    const int mid = (lj + hj + 1) / 2;
+    if (w.offP[mid] <= r) lj = mid; else hj = mid - 1;
+  }
+  const int j = lj, k = j + (int)(r - w.offP[j]);
+  Acc3 a;
+  a.r[1] = &w.single[i];
+  a.r[2] = &w.single[j];
+  a.r[4] = &w.single[k];
+  a.r[3] = &w.pair[pidx(w, i, j)];
+  a.r[5] = &w.pair[pidx(w, i, k)];
+  a.r[6] = &w.pair[pidx(w, j, k)];
+  const Rec* sub[3] = {a.r[1], a.r[2], a.r[4]};
+  Rec S;
+  base_record(w, sub, 3, S);
+  TopOut o;
+  subset_state<3>(a, S, w.md, o);
+  S.eta = o.eta;
+  S.phi = o.phi;
+  S.zero = o.zero ? 1.0 : 0.0;
+  S.zero_src = o.zero_src;
+  surface<3>(a, S, o.phi, S);
+  w.triple[t] = S;
+}
+
+struct QuartetArgs {
+  WitDev w;
+  const cpx<double>* amps;  // [cases][M]
+  int cases;
+  double* out;              // [5][cases][ny][nx] complex
+  unsigned long long* cnt;  // kept, near, zero_out, zero_int
+  unsigned long long* dmax; // bits: max residual, max zero source, min detuning
+};
+
+__device__ __forceinline__ void atomic_addc(double* p, cpx<double> v) {
+  unsafeAtomicAdd(p, v.x);
+  unsafeAtomicAdd(p + 1, v.y);
+}
+
+__device__ __forceinline__ unsigned long long wsum_u64(unsigned long long v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ double wmax_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ __forceinline__ double wmin_d(double v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// One workgroup per leading pair (i, j); threads sweep the (k, l) triangle.
+__global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
+  const WitDev& w = g.w;
+  const long long p = blockIdx.x;
+  int lo = 0, hi = w.M - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (w.offP[mid] <= p) lo = mid; else hi = mid - 1;
+  }
+  const int i = lo, j = i + (int)(p - w.offP[i]);
+  const int n = w.M - j;  // k, l in [j, M)
+  const long long tri = (long long)n * (n + 1) / 2;
+  unsigned long long kept = 0, nearc = 0, zo = 0, zi = 0;
+  double maxres = 0.0, maxz = 0.0, mindet = INFINITY;
+  const size_t plane = (size_t)w.nx * w.ny * g.cases;
+  for (long long f = threadIdx.x; f < tri; f += WNT) {
+    // unrank f -> (k', l'), 0 <= k' <= l' < n: row k' starts at k' n - k'(k'-1)/2
+    double disc = (2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * (double)f;
+    int kk = (int)floor(((2.0 * n + 1.0) - sqrt(fmax(disc, 0.0))) / 2.0);
+    if (kk < 0) kk = 0;
+    if (kk > n - 1) kk = n - 1;
+    while (kk > 0 && (long long)kk * n - (long long)kk * (kk - 1) / 2 > f) --kk;
+    while (kk + 1 < n && (long long)(kk + 1) * n - (long long)(kk + 1) * kk / 2 <= f) ++kk;
+    const long long start = (long long)kk * n - (long long)kk * (kk - 1) / 2;
+    const int k = j + kk, l = k + (int)(f - start);
+    const int tx = w.modes[2 * i] + w.modes[2 * j] + w.modes[2 * k] + w.modes[2 * l];
+    const int ty = w.modes[2 * i + 1] + w.modes[2 * j + 1] + w.modes[2 * k + 1] + w.modes[2 * l + 1];
+    if (tx == 0 && ty == 0) {
+      ++zo;
+      continue;
+    }
+    if (abs(tx) >= w.nx / 2 || abs(ty) >= w.ny / 2) continue;
+    Acc4 a;
+    a.r[1] = &w.single[i];
+    a.r[2] = &w.single[j];
+    a.r[4] = &w.single[k];
+    a.r[8] = &w.single[l];
+    a.r[3] = &w.pair[pidx(w, i, j)];
+    a.r[5] = &w.pair[pidx(w, i, k)];
+    a.r[9] = &w.pair[pidx(w, i, l)];
+    a.r[6] = &w.pair[pidx(w, j, k)];
+    a.r[10] = &w.pair[pidx(w, j, l)];
+    a.r[12] = &w.pair[pidx(w, k, l)];
+    a.r[7] = &w.triple[tidx(w, i, j, k)];
+    a.r[11] = &w.triple[tidx(w, i, j, l)];
+    a.r[13] = &w.triple[tidx(w, i, k, l)];
+    a.r[14] = &w.triple[tidx(w, j, k, l)];
+    const Rec* sub[4] = {a.r[1], a.r[2], a.r[4], a.r[8]};
+    Rec S;
+    base_record(w, sub, 4, S);
+    TopOut o;
+    subset_state<4>(a, S, w.md, o);
+    // strict-zero intermediates of the ten proper subsets of degree >= 2
+    double zcount = 0.0, zsrc = 0.0;
+    constexpr int kInner[10] = {3, 5, 6, 9, 10, 12, 7, 11, 13, 14};
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      zcount += a.r[kInner[q]]->zero;
+      zsrc = fmax(zsrc, a.r[kInner[q]]->zero_src);
+    }
+    // paired residual (:413-429)
+    cpx<double> er = glw::csub(glw::csub(ci(-S.om, o.eta), cs(o.psi, S.q)), o.src0);
+    cpx<double> pr = glw::csub(ca(ci(-S.om, o.psi), cs(o.eta, w.md.g)), o.src1);
+    if (o.near) {
+      const double ww = sqrt(w.md.g * S.q), ms = sqrt(ww / (2.0 * w.md.g)), ns = sqrt(w.md.g / (2.0 * ww));
+      const double sg = S.om >= 0.0 ? 1.0 : -1.0;
+      er = ca(er, cs(o.retained, ms));
+      pr = ca(pr, ci(-sg * ns, o.retained));
+    }
+    const double res = hypot(cabs2(er), cabs2(pr));
+    // repeated-index factor (:486-499)
+    const int id[4] = {i, j, k, l};
+    int fac = 1, run = 1;
+#pragma unroll
+    for (int s = 1; s < 4; ++s) {
+      if (id[s] == id[s - 1]) {
+        ++run;
+      } else {
+        for (int v = 2; v <= run; ++v) fac *= v;
+        run = 1;
+      }
+    }
+    for (int v = 2; v <= run; ++v) fac *= v;
+    const size_t ix = (size_t)(((tx % w.nx) + w.nx) % w.nx), iy = (size_t)(((ty % w.ny) + w.ny) % w.ny);
+    for (int c = 0; c < g.cases; ++c) {
+      const cpx<double>* ac = g.amps + (size_t)c * w.M;
+      cpx<double> wt = cm(cm(cm(cs(ac[i], 2.0), ac[j]), ac[k]), ac[l]);
+      wt = cs(wt, 1.0 / (double)fac);
+      const size_t loc = (size_t)c * w.nx * w.ny + ix + (size_t)w.nx * iy;
+      atomic_addc(g.out + 2 * loc, cm(wt, o.eta));
+      atomic_addc(g.out + 2 * (plane + loc), cm(wt, o.psi));
+      if (o.near) atomic_addc(g.out + 2 * (4 * plane + loc), cm(wt, o.retained));
+    }
+    ++kept;
+    nearc += o.near ? 1 : 0;
+    zi += (unsigned long long)zcount;
+    maxres = fmax(maxres, res);
+    mindet = fmin(mindet, o.rho);
+    maxz = fmax(maxz, zsrc);
+  }
+  kept = wsum_u64(kept);
+  nearc = wsum_u64(nearc);
+  zo = wsum_u64(zo);
+  zi = wsum_u64(zi);
+  maxres = wmax_d(maxres);
+  maxz = wmax_d(maxz);
+  mindet = wmin_d(mindet);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(g.cnt + 0, kept);
+    atomicAdd(g.cnt + 1, nearc);
+    atomicAdd(g.cnt + 2, zo);
+    atomicAdd(g.cnt + 3, zi);
+    atomicMax(g.dmax + 0, (unsigned long long)__double_as_longlong(maxres));
+    atomicMax(g.dmax + 1, (unsigned long long)__double_as_longlong(maxz));
+    if (mindet < INFINITY) atomicMin(g.dmax + 2, (unsigned long long)__double_as_longlong(mindet));
+  }
+}
+
+__global__ void k_wit_clear(double* out, size_t n, unsigned long long* cnt, unsigned long long* dmax) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) out[i] = 0.0;
+  if (i < 4) cnt[i] = 0;
+  if (i < 2) dmax[i] = 0;
+  if (i == 2) dmax[2] = (unsigned long long)__double_as_longlong(INFINITY);
+}
+
+__global__ void k_wit_counters(const unsigned long long* cnt, const unsigned long long* dmax, double total,
+                               double* counters) {
+  if (threadIdx.x != 0) return;
+  counters[0] = total;
+  counters[1] = (double)cnt[0];
+  counters[2] = (double)cnt[1];
+  counters[3] = (double)cnt[2];
+  counters[4] = (double)cnt[3];
+  counters[5] = __longlong_as_double((long long)dmax[0]);
+  counters[6] = __longlong_as_double((long long)dmax[2]);
+  counters[7] = __longlong_as_double((long long)dmax[1]);
+}
+
+}  // namespace
+
+struct glw_wit_s {
+  glw_wit_desc d{};
+  int device = 0;
+  WitDev w{};
+  std::vector<void*> allocs;
+  unsigned long long* cnt = nullptr;
+  unsigned long long* dmax = nullptr;
+  ~glw_wit_s() {
+    for (void* p : allocs) (void)hipFree(p);
+  }
+  int alloc(void** p, size_t bytes) {
+    W_HIP(hipMalloc(p, bytes ? bytes : 64));
+    allocs.push_back(*p);
+    return GLW_OK;
+  }
+};
+
+extern "C" {
+
+int glw_wit_create(const glw_wit_desc* d, glw_wit** out) {
+  if (!out || !d) return wfail(GLW_EINVAL, "null argument");
+  *out = nullptr;
+  if (d->modes_count < 1 || !d->modes || d->nx < 2 || d->ny < 2 || d->gravity <= 0 || d->depth <= 0)
+    return wfail(GLW_EINVAL, "invalid WIT descriptor");
+  glw_wit* W = new glw_wit_s();
+  W->d = *d;
+  int rc = GLW_OK;
+  const int M = d->modes_count;
+  if (hipGetDevice(&W->device) != hipSuccess) {
+    delete W;
+    return wfail(GLW_EDEVICE, "no HIP device");
+  }
+  const long long np = (long long)M * (M + 1) / 2;
+  std::vector<long long> offP(M + 1), offT(M + 1);
+  offP[0] = offT[0] = 0;
+  for (int i = 0; i < M; ++i) {
+    offP[i + 1] = offP[i] + (M - i);
+    const long long n = M - i;
+    offT[i + 1] = offT[i] + n * (n + 1) / 2;
+  }
+  const long long nt = offT[M];
+  void *modes, *single, *pair, *triple, *op, *ot, *cnt, *dmax;
+  rc = W->alloc(&modes, sizeof(int) * 2 * M);
+  if (!rc) rc = W->alloc(&single, sizeof(Rec) * M);
+  if (!rc) rc = W->alloc(&pair, sizeof(Rec) * np);
+  if (!rc) rc = W->alloc(&triple, sizeof(Rec) * nt);
+  if (!rc) rc = W->alloc(&op, sizeof(long long) * (M + 1));
+  if (!rc) rc = W->alloc(&ot, sizeof(long long) * (M + 1));
+  if (!rc) rc = W->alloc(&cnt, sizeof(unsigned long long) * 4);
+  if (!rc) rc = W->alloc(&dmax, sizeof(unsigned long long) * 4);
+  if (rc) {
+    delete W;
+    return rc;
+  }
+  auto copy = [&](void* dst, const void* src, size_t n) -> int {
+    W_HIP(hipMemcpy(dst, src, n, hipMemcpyHostToDevice));
+    return GLW_OK;
+  };
+  if ((rc = copy(modes, d->modes, sizeof(int) * 2 * M)) || (rc = copy(op, offP.data(), sizeof(long long) * (M + 1))) ||
+      (rc = copy(ot, offT.data(), sizeof(long long) * (M + 1)))) {
+    delete W;
+    return rc;
+  }
+  WitDev& w = W->w;
+  w.M = M;
+  w.nx = d->nx;
+  w.ny = d->ny;
+  w.md = Model{d->gravity, d->depth, d->threshold};
+  w.modes = (const int*)modes;
+  w.single = (Rec*)single;
+  w.pair = (Rec*)pair;
+  w.triple = (Rec*)triple;
+  w.offP = (const long long*)op;
+  w.offT = (const long long*)ot;
+  W->cnt = (unsigned long long*)cnt;
+  W->dmax = (unsigned long long*)dmax;
+  hipLaunchKernelGGL(k_wit_single_fill, dim3((M + 255) / 256), dim3(256), 0, 0, w, d->dkx, d->dky);
+  hipLaunchKernelGGL(k_wit_pairs, dim3((np + 255) / 256), dim3(256), 0, 0, w);
+  hipLaunchKernelGGL(k_wit_triples, dim3((nt + 255) / 256), dim3(256), 0, 0, w, nt);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    delete W;
+    return wfail(GLW_EDEVICE, "WIT subset tables failed");
+  }
+  *out = W;
+  return GLW_OK;
+}
+
+int glw_wit_destroy(glw_wit* w) {
+  delete w;
+  return GLW_OK;
+}
+
+int glw_wit_sector44(glw_wit* W, const double* amps, int32_t cases, double* out, double* counters, void* stream) {
+  if (!W || !amps || !out || !counters || cases < 1) return wfail(GLW_EINVAL, "null argument");
+  W_HIP(hipSetDevice(W->device));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t n = 5ull * cases * W->d.nx * W->d.ny * 2;
+  hipLaunchKernelGGL(k_wit_clear, dim3((n + 255) / 256), dim3(256), 0, st, out, n, W->cnt, W->dmax);
+  QuartetArgs g;
+  g.w = W->w;
+  g.amps = (const cpx<double>*)amps;
+  g.cases = cases;
+  g.out = out;
+  g.cnt = W->cnt;
+  g.dmax = W->dmax;
+  const long long M = W->d.modes_count;
+  hipLaunchKernelGGL(k_wit_quartets, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), 0, st, g);
+  const double total = (double)M * (M + 1) * (M + 2) * (M + 3) / 24.0;
+  hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, total, counters);
+  W_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+}  // extern "C"

@@ -119,6 +119,33 @@ int glw_order2_execute(glw_plan* plan, const double* s0, const double* times, in
 int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt,
                        double* eta33, double* psi33, void* stream);
 
+/* WIT sector-44 quartet sum: replaces the per-quartet DirectionalKernel::evaluate
+ * and the sector-44 loop of paper/order4/cpp/directional_eta4_openmp.cpp
+ * (:157-453, :690-734, accumulate :595-602, output arrays :747-773).  Parent
+ * subset states (singles, pairs, triples) are tabulated at creation. */
+typedef struct glw_wit_desc {
+  int32_t modes_count;          /* M */
+  const int32_t* modes;         /* host [M][2] integer modes (mx, my) */
+  int32_t nx, ny;               /* output grid */
+  double gravity, depth, dkx, dky;
+  double threshold;             /* near-resonance threshold on rho */
+} glw_wit_desc;
+
+typedef struct glw_wit_s glw_wit;
+
+int glw_wit_create(const glw_wit_desc* desc, glw_wit** wit);
+int glw_wit_destroy(glw_wit* wit);
+
+/* amplitudes: device complex128 [cases][M].  out: device complex128
+ * [5][cases][ny][nx] (x fastest), overwritten with the DIR44O01 arrays eta,
+ * psi, eta coordinate, psi coordinate (zero in sector 44), retained.
+ * counters: device double[8] = total, kept, near, strict-zero outputs,
+ * strict-zero intermediates, max residual, min detuning, max strict-zero
+ * source.  Accumulation uses FP64 atomics: bins are summed in a
+ * schedule-dependent order (relative differences at the 1e-16 level). */
+int glw_wit_sector44(glw_wit* wit, const double* amplitudes, int32_t cases, double* out, double* counters,
+                     void* stream);
+
 /* Stage timing (instrumentation; no reference counterpart).  enable != 0
  * creates a pool of HIP events; every kernel launch of later execute calls is
  * bracketed by two events on the execute stream.  glw_plan_stage_times waits

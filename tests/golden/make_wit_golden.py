@@ -1,0 +1,52 @@
+"""WIT golden vectors from the reference (run in the build container only).
+
+* copies the reference's DIR4IN01 inputs paper/order4/inputs/runtime/n0050,
+  n0100, n0200, n0400, n0800 and paper/order4/inputs/field_n0382 (data files)
+  to tests/golden/ (n0200..n0800 are timing inputs: no golden output);
+* runs the reference executable (oracle/_ref/wit_eta4, built from the
+  reference source by oracle/build_oracle.py) on n0050 and n0100, sector 44;
+* converts those outputs and the reference's shipped n0382 output
+  (paper/order4/data/wit_field_n0382.bin) to npz: eta and psi on their
+  nonzero bins plus the header counters (the other three arrays are zero in
+  these runs: near = 0).
+"""
+
+import shutil
+import subprocess
+import sys
+import tempfile
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+sys.path.insert(0, str(ROOT))
+from green_laplace_wave_reconstruction_amd import formats  # noqa: E402
+
+REF = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+EXE = ROOT / "oracle" / "_ref" / "wit_eta4"
+
+
+def save(name, out):
+    for k in ("retained", "eta_coordinate", "psi_coordinate"):
+        assert not np.any(out[k]), k
+    idx = np.flatnonzero((out["eta"] != 0) | (out["psi"] != 0))
+    np.savez_compressed(HERE / f"wit_{name}_sector44.npz", index=idx, eta=out["eta"].ravel()[idx],
+                        psi=out["psi"].ravel()[idx], shape=np.array(out["eta"].shape),
+                        counters=np.array([out[k] for k in ("total", "kept", "near", "strict_zero_output",
+                                                            "strict_zero_intermediate", "max_residual",
+                                                            "min_detuning", "max_strict_zero_source")], float))
+    print(name, "nonzero bins", idx.size, "kept", out["kept"])
+
+
+for name, rel in (("n0050", "inputs/runtime/n0050"), ("n0100", "inputs/runtime/n0100"),
+                  ("n0200", "inputs/runtime/n0200"), ("n0400", "inputs/runtime/n0400"),
+                  ("n0800", "inputs/runtime/n0800"), ("n0382", "inputs/field_n0382")):
+    shutil.copy(REF / "paper" / "order4" / rel / "directional_eta44_input.bin", HERE / f"wit_{name}_input.bin")
+for name in ("n0050", "n0100"):
+    with tempfile.TemporaryDirectory() as td:
+        out_path = Path(td) / "out.bin"
+        subprocess.run([str(EXE), str(HERE / f"wit_{name}_input.bin"), str(out_path), "--sector", "44"], check=True)
+        save(name, formats.read_dir4out(out_path))
+save("n0382", formats.read_dir4out(REF / "paper" / "order4" / "data" / "wit_field_n0382.bin"))
