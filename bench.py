@@ -90,8 +90,100 @@ def cpu_baseline(s, c, kxg, kyg, times_sample, gpu_fields=None):
     return len(times_sample) / elapsed, errs
 
 
+def bench_eta44(args):
+    """eta44 GL rank J on the 256^2 dimensionless order-4 sea state (one field per step)."""
+    import torch
+
+    from green_laplace_wave_reconstruction_amd import api, plan as PL, seastate
+    from green_laplace_wave_reconstruction_amd._native import DevicePlan
+
+    s = seastate.order4_dimensionless_256()
+    c = api.gl_spectral_coefficients(1, s.gravity, s.depth, s.a, s.b, s.kx, s.ky, 0, 0)
+    kxg, kyg = api._wavenumber_grid(s.lx, s.ly, s.nx, s.ny)
+    spec, _ = api.deposit_analytic_input(c, s.lx, s.ly, s.nx, s.ny)
+    host = PL.build_order4_plan(spec, s.depth * kxg, s.depth * kyg, peak_q=c["peak_depth"], rank=args.rank)
+    dp = DevicePlan(host, host.box1.take(spec), max_instants=1)
+    t = np.array([0.0])
+    for _ in range(args.warmup):
+        dp.run("order4", t)
+    torch.cuda.synchronize()
+    dp.profile(True)
+    dp.stage_times()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dp.run("order4", t)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    stages = dp.stage_times()
+    cpu = None
+    if not args.no_cpu and args.rank <= 6:
+        from oracle import gl_oracle as O
+
+        t1 = time.perf_counter()
+        O.eta44(spec, s.depth * kxg, s.depth * kyg, c["peak_depth"], args.rank)
+        cpu = {"value": 1.0 / (time.perf_counter() - t1), "unit": "fields/s", "cores": 1, "kind": "port",
+               "sample": f"one GL{args.rank} eta44 field, FP64 NumPy oracle restatement, numpy.fft single-threaded"}
+    return {
+        "metric": f"eta44 GL{args.rank} fields/s at 256^2", "value": args.steps / wall, "unit": "fields/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic seeded JONSWAP sea state (numpy default_rng seed 5)",
+        "config": {"workload": f"eta44 GL{args.rank}, 256x256, g=h=1, kp h=1.5, kp Hs/2=0.12, L=5 peak wavelengths, "
+                               f"{s.count} parents", "working_grids": host.audits["eta44"]["working_grids"],
+                   "reference_fft_ifft_count": host.audits["eta44"]["reference_fft_ifft_count"]},
+        "stages_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
+        "cpu_baseline": cpu,
+    }
+
+
+def bench_wit(args):
+    """WIT sector-44 quartet sum on the reference's frozen DIR4IN01 inputs (512^2)."""
+    import torch
+
+    from green_laplace_wave_reconstruction_amd import formats, wit
+
+    name = f"n{args.modes:04d}"
+    inp = formats.read_dir4in(ROOT / "tests" / "golden" / f"wit_{name}_input.bin")
+    plan = wit.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                       dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
+    for _ in range(args.warmup):
+        out, cnt = plan.sector44_device(inp["amplitudes"])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, cnt = plan.sector44_device(inp["amplitudes"], out)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c = cnt.cpu().numpy()
+    cpu = None
+    if not args.no_cpu:
+        from oracle import wit as ow
+
+        m = inp["modes"].shape[0]
+        first = max(1, m // 40)
+        t1 = time.perf_counter()
+        r = ow.sector44(inp, (m - first, m)) if m > 100 else ow.sector44(inp)
+        el = time.perf_counter() - t1
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": r["kept"] / el, "unit": "quartets/s", "cores": cores, "kind": "port",
+               "sample": (f"quartets with first index in [{m - first}, {m}) ({int(r['kept'])} quartets), C++/OpenMP "
+                          "restatement of DirectionalKernel::evaluate" if m > 100 else "full sum")}
+    return {
+        "metric": f"WIT sector-44 quartets/s, M={inp['modes'].shape[0]}", "value": c[1] * args.steps / wall,
+        "unit": "quartets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": f"reference frozen input paper/order4/inputs/runtime/{name} (copied as a fixture)",
+        "config": {"workload": f"WIT sector 44, M={inp['modes'].shape[0]}, {inp['nx']}x{inp['ny']}",
+                   "quartets": int(c[0])},
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="order3", choices=("order3", "eta44", "wit"))
+    ap.add_argument("--rank", type=int, default=6, help="eta44 GL rank")
+    ap.add_argument("--modes", type=int, default=200, help="WIT mode count (200, 400, 800)")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
@@ -125,6 +217,12 @@ def main():
     if not torch.cuda.is_available():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+    if args.workload == "eta44":
+        print(json.dumps(bench_eta44(args)))
+        return
+    if args.workload == "wit":
+        print(json.dumps(bench_wit(args)))
+        return
     s, c, kxg, kyg, spec, omega = workload()
     prec = PL.PREC_FP64 if args.precision == "fp64" else PL.PREC_FP32
     host = PL.build_plan(spec, s.depth * kxg, s.depth * kyg, depth=s.depth, gravity=s.gravity, order=3,

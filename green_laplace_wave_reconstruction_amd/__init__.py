@@ -9,6 +9,7 @@ from .api import (
     DirectionalFieldResult,
     GreenLaplaceError,
     gl_spectral_coefficients,
+    gl_pure_sum_order4,
     gl_spectral_surface,
     reconstruct_directional_field,
 )
@@ -18,6 +19,7 @@ __all__ = [
     "DirectionalFieldResult",
     "GreenLaplaceError",
     "gl_spectral_coefficients",
+    "gl_pure_sum_order4",
     "gl_spectral_surface",
     "reconstruct_directional_field",
     "build_plan",

@@ -39,6 +39,14 @@ class COrder2Desc(ctypes.Structure):
                 ("psi_coef", _DP), ("psi_sign", _DP)]
 
 
+class COrder4Desc(ctypes.Structure):
+    _fields_ = [("rank", c_int32), ("p4", CBox), ("g4x", c_int32), ("g4y", c_int32), ("t4", CBoxTables),
+                ("mask4", _U8P), ("damp_quartic", _DP), ("damp_third", _DP), ("damp_pair3", _DP),
+                ("damp_pair4", _DP), ("pair_cosh", _DP), ("pair_sinhr", _DP), ("pair_coef", _DP),
+                ("third_cosh", _DP), ("third_sinhr", _DP), ("third_coef", _DP), ("quartic_cosh", _DP),
+                ("quartic_sinhr", _DP), ("quartic_coef", _DP)]
+
+
 class CPlanDesc(ctypes.Structure):
     _fields_ = [("nx", c_int32), ("ny", c_int32), ("depth", c_double), ("gravity", c_double),
                 ("floor_level", c_double), ("precision", c_int32), ("max_instants", c_int32),
@@ -47,12 +55,12 @@ class CPlanDesc(ctypes.Structure):
                 ("t3", CBoxTables), ("omega", _DP), ("mask2", _U8P), ("mask3", _U8P), ("inner_rank", c_int32),
                 ("outer_rank", c_int32), ("damp_outer", _DP), ("damp_inner", _DP), ("inner_cosh", _DP),
                 ("inner_sinhr", _DP), ("inner_coef", _DP), ("outer_cosh", _DP), ("outer_sinhr", _DP),
-                ("outer_coef", _DP), ("o2", COrder2Desc)]
+                ("outer_coef", _DP), ("o2", COrder2Desc), ("with_order4", c_int32), ("o4", COrder4Desc)]
 
 
 #: every symbol include/glw.h declares
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
-           "glw_order3_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
+           "glw_order3_execute", "glw_order4_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
            "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
            "glw_fft_length_supported", "glw_last_error", "glw_version")
 
@@ -76,6 +84,8 @@ def load_library(path: str | os.PathLike | None = None):
         fn = getattr(lib, name)
         fn.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
         fn.restype = c_int
+    lib.glw_order4_execute.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]
+    lib.glw_order4_execute.restype = c_int
     lib.glw_plan_profile.argtypes = [c_void_p, c_int32]
     lib.glw_plan_profile.restype = c_int
     lib.glw_plan_stage_times.argtypes = [c_void_p, c_int32, c_char_p, c_size_t, _DP, POINTER(c_int32),
@@ -152,6 +162,14 @@ def describe(plan: HostPlan, max_instants: int) -> tuple[CPlanDesc, list]:
                            _dp(o2["eta_sinhr"], keep), _dp(o2["eta_coef"], keep), o2["channels"],
                            _dp(o2["psi_damp"], keep), _dp(o2["psi_factor"], keep), _dp(o2["psi_coef"], keep),
                            _dp(o2["psi_sign"], keep))
+    if plan.o4:
+        o = plan.o4
+        d.with_order4 = 1
+        d.o4 = COrder4Desc(o["rank"], _cbox(plan.box4), plan.g4[0], plan.g4[1], _ctab(plan.t4, keep),
+                           _u8p(plan.mask4, keep), *(_dp(o[k], keep) for k in (
+                               "damp_quartic", "damp_third", "damp_pair3", "damp_pair4", "pair_cosh", "pair_sinhr",
+                               "pair_coef", "third_cosh", "third_sinhr", "third_coef", "quartic_cosh",
+                               "quartic_sinhr", "quartic_coef")))
     return d, keep
 
 
@@ -214,9 +232,13 @@ class DevicePlan:
         for x in (a, b):
             if x.shape != shape or x.dtype != torch.complex128 or not x.is_contiguous() or x.device != self.device:
                 raise ValueError(f"outputs must be contiguous complex128 {shape} on {self.device}")
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        if kind == "order4":
+            _check(self.lib.glw_order4_execute(self._handle, c_void_p(self.s0.data_ptr()), c_void_p(t.data_ptr()), nt,
+                                               c_void_p(a.data_ptr()), c_void_p(stream)))
+            return a, None
         fn = {"linear": self.lib.glw_linear_execute, "order2": self.lib.glw_order2_execute,
               "order3": self.lib.glw_order3_execute}[kind]
-        stream = torch.cuda.current_stream(self.device).cuda_stream
         _check(fn(self._handle, c_void_p(self.s0.data_ptr()), c_void_p(t.data_ptr()), nt,
                   c_void_p(a.data_ptr()), c_void_p(b.data_ptr()), c_void_p(stream)))
         return a, b

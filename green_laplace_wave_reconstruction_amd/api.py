@@ -306,3 +306,28 @@ def reconstruct_directional_field(eta1, *, lx: float, ly: float, depth: float, g
         "stokes_correction": False, "component_audits": host.audits, "precision": precision,
     }
     return DirectionalFieldResult(x=x, y=y, eta=out["eta"], psi=out["psi"], components=out, audit=audit)
+
+
+def gl_pure_sum_order4(s, qx, qy, qp: float, rank: int = 4, *, precision: str = "fp64"):
+    """Experimental dimensionless positive pure-sum order-four field (src/gl_pure_sum_order4.m:1-15).
+
+    ``s`` is ``fft2(eta11_plus/h)`` on the full grid, ``qx = kx h``, ``qy = ky h``,
+    ``qp = kp h``.  Returns ``(eta44, psi44, audit)`` with ``eta44`` the analytic
+    eta44/h (take the real part for the physical field).  The reference's
+    C++ runtime (paper/order4/cpp/gl_eta44_fftw.cpp) computes eta44 only; so
+    does this build: ``psi44`` is ``None``.
+    """
+    import torch
+
+    host = _plan.build_order4_plan(np.asarray(s), np.asarray(qx, float), np.asarray(qy, float), peak_q=float(qp),
+                                   rank=int(rank), precision=_precision(precision))
+    dp = DevicePlan(host, host.box1.take(np.asarray(s, dtype=np.complex128)), max_instants=1)
+    try:
+        e, _ = dp.run("order4", np.array([0.0]))
+        torch.cuda.synchronize()
+        eta = e[0].cpu().numpy()
+    finally:
+        dp.close()
+    audit = dict(host.audits["eta44"])
+    audit["release_status"] = "experimental no-Stokes; eta44 only"
+    return eta, None, audit

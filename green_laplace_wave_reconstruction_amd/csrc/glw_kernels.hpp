@@ -22,7 +22,7 @@
 
 namespace glw {
 
-constexpr int kMaxFields = 24;
+constexpr int kMaxFields = 48;
 constexpr int NT = 256;
 
 // Spectral multiplier codes (functions of qx, qy, nu, 1/nu, q^2, A on a bin).
@@ -48,6 +48,20 @@ enum MulCode : int {
   M_Q2 = 18,        // q^2
   M_A = 19,         // A
   M_MI_INU = 20,    // -i / nu  (0 where nu == 0)
+  M_NU4 = 21,       // nu^4
+  M_Q2NU = 22,      // q^2 nu
+  M_NU2_QX = 23,    // nu^2 qx
+  M_NU2_QY = 24,    // nu^2 qy
+  M_MI_NU2_QX = 25, // -i nu^2 qx
+  M_MI_NU2_QY = 26, // -i nu^2 qy
+  M_MQ2A = 27,      // -q^2 A
+  M_I_ANU = 28,     // i A nu
+  M_QXQ2_NU = 29,   // qx q^2 / nu   (i qx q^2 applied to phi = -i s / nu)
+  M_QYQ2_NU = 30,   // qy q^2 / nu
+  M_MI_Q4_NU = 31,  // -i q^4 / nu   (q^4 applied to phi = -i s / nu)
+  M_I_QXA = 32,     // i qx A
+  M_I_QYA = 33,     // i qy A
+  M_Q2A = 34,       // q^2 A
 };
 
 struct BoxTables {
@@ -83,6 +97,20 @@ __device__ __forceinline__ cpx<double> apply_mul(int code, cpx<double> s, const 
     case M_Q2: re = t.q2[b]; break;
     case M_A: re = t.a[b]; break;
     case M_MI_INU: im = t.nu[b] > 0.0 ? -1.0 / t.nu[b] : 0.0; break;
+    case M_NU4: re = t.nu[b] * t.nu[b] * t.nu[b] * t.nu[b]; break;
+    case M_Q2NU: re = t.q2[b] * t.nu[b]; break;
+    case M_NU2_QX: re = t.nu[b] * t.nu[b] * t.qx[b]; break;
+    case M_NU2_QY: re = t.nu[b] * t.nu[b] * t.qy[b]; break;
+    case M_MI_NU2_QX: im = -t.nu[b] * t.nu[b] * t.qx[b]; break;
+    case M_MI_NU2_QY: im = -t.nu[b] * t.nu[b] * t.qy[b]; break;
+    case M_MQ2A: re = -t.q2[b] * t.a[b]; break;
+    case M_I_ANU: im = t.a[b] * t.nu[b]; break;
+    case M_QXQ2_NU: re = t.nu[b] > 0.0 ? t.qx[b] * t.q2[b] / t.nu[b] : 0.0; break;
+    case M_QYQ2_NU: re = t.nu[b] > 0.0 ? t.qy[b] * t.q2[b] / t.nu[b] : 0.0; break;
+    case M_MI_Q4_NU: im = t.nu[b] > 0.0 ? -t.q2[b] * t.q2[b] / t.nu[b] : 0.0; break;
+    case M_I_QXA: im = t.qx[b] * t.a[b]; break;
+    case M_I_QYA: im = t.qy[b] * t.a[b]; break;
+    case M_Q2A: re = t.q2[b] * t.a[b]; break;
     default: break;
   }
   return {re * s.x - im * s.y, re * s.y + im * s.x};

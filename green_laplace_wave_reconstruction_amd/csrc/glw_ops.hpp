@@ -105,3 +105,122 @@ struct ContactOp {
 };
 
 }  // namespace glw
+
+namespace glw {
+
+// ---------------------------------------------------------------------------
+// Order-4 graph (paper/order4/cpp/gl_eta44_fftw.cpp).  Products are evaluated
+// in the reference's association order where it is explicit.
+// ---------------------------------------------------------------------------
+
+// Pair state sources with two time derivatives (gl_eta44_fftw.cpp:394-403).
+// in: 0 v 1 vn 2 vn2 3 vn3 4 vn4 5 hx 6 hy 7 jx 8 jy 9 ro 10 q2 11 q2n
+//     12 njx 13 njy 14 n2jx 15 n2jy
+// out: sd, sk, sds, sks, sdss, skss
+struct Pair44Op {
+  template <typename T>
+  __device__ __forceinline__ static void apply(const cpx<T>* f, cpx<T>* o) {
+    const cpx<T> v = f[0], vn = f[1], vn2 = f[2], vn3 = f[3], vn4 = f[4], hx = f[5], hy = f[6];
+    const cpx<T> jx = f[7], jy = f[8], ro = f[9], q2 = f[10], q2n = f[11], njx = f[12], njy = f[13];
+    const cpx<T> n2jx = f[14], n2jy = f[15];
+    const cpx<T> hxjx = mul(hx, jx), hyjy = mul(hy, jy);
+    const cpx<T> grad = add(add(mul(jx, jx), mul(hx, njx)), add(mul(jy, jy), mul(hy, njy)));
+    o[0] = sub(sub(add(scl(mul(v, vn2), T(2)), mul(vn, vn)), mul(hx, hx)), mul(hy, hy));
+    o[1] = add(scl(mul(v, ro), T(2)), scl(add(hxjx, hyjy), T(2)));
+    o[2] = sub(sub(add(scl(mul(v, vn3), T(2)), scl(mul(vn, vn2), T(4))), scl(hxjx, T(2))), scl(hyjy, T(2)));
+    o[3] = add(scl(add(mul(vn, ro), mul(v, q2)), T(2)), scl(grad, T(2)));
+    o[4] = sub(add(add(scl(mul(v, vn4), T(2)), scl(mul(vn, vn3), T(6))), scl(mul(vn2, vn2), T(4))), scl(grad, T(2)));
+    o[5] = add(add(scl(add(add(mul(vn2, ro), scl(mul(vn, q2), T(2))), mul(v, q2n)), T(2)),
+                   scl(add(mul(jx, njx), mul(jy, njy)), T(6))),
+               scl(add(mul(hx, n2jx), mul(hy, n2jy)), T(2)));
+  }
+};
+
+// Third-order forcing with time derivatives (gl_eta44_fftw.cpp:569-593).
+// in: first 0 eta 1 ex 2 ey 3 px 4 py 5 pz 6 pxz 7 pyz 8 pzz 9 pzzz 10 ptz 11 ptzz
+//           12 pxz_t 13 pyz_t 14 pzzz_t 15 ptz_t
+//     pair  16 eta 17 ex 18 ey 19 eta_t 20 ex_t 21 ey_t 22 px 23 py 24 pz 25 pzz
+//           26 ptz 27 px_t 28 py_t 29 pzz_t 30 ptz_t
+// out: fk, fd, fkt, fdt
+struct Third44Op {
+  template <typename T>
+  __device__ __forceinline__ static void apply(const cpx<T>* f, cpx<T>* o) {
+    const cpx<T> e = f[0], ex = f[1], ey = f[2], px = f[3], py = f[4], pz = f[5], pxz = f[6], pyz = f[7];
+    const cpx<T> pzz = f[8], pzzz = f[9], ptz = f[10], ptzz = f[11], pxzt = f[12], pyzt = f[13], pzzzt = f[14];
+    const cpx<T> ptzt = f[15];
+    const cpx<T> Pe = f[16], Pex = f[17], Pey = f[18], Pet = f[19], Pext = f[20], Peyt = f[21], Ppx = f[22];
+    const cpx<T> Ppy = f[23], Ppz = f[24], Ppzz = f[25], Pptz = f[26], Ppxt = f[27], Ppyt = f[28], Ppzzt = f[29];
+    const cpx<T> Pptzt = f[30];
+    const cpx<T> e2 = mul(e, e);
+    const cpx<T> kp = sub(sub(add(add(add(mul(px, Pex), mul(py, Pey)), mul(Ppx, ex)), mul(Ppy, ey)), mul(e, Ppzz)),
+                          mul(Pe, pzz));
+    const cpx<T> kd = sub(mul(e, add(mul(pxz, ex), mul(pyz, ey))), scl(mul(e2, pzzz), T(0.5)));
+    const cpx<T> dp = add(add(add(add(mul(e, Pptz), mul(Pe, ptz)), mul(px, Ppx)), mul(py, Ppy)), mul(pz, Ppz));
+    const cpx<T> dd = add(scl(mul(e2, ptzz), T(0.5)), mul(e, add(add(mul(px, pxz), mul(py, pyz)), mul(pz, pzz))));
+    cpx<T> kpt = sub(mul(px, Pext), mul(ex, Pex));
+    kpt = add(kpt, sub(mul(py, Peyt), mul(ey, Pey)));
+    kpt = add(add(kpt, mul(Ppxt, ex)), mul(Ppx, pxz));
+    kpt = add(add(kpt, mul(Ppyt, ey)), mul(Ppy, pyz));
+    kpt = sub(sub(kpt, mul(pz, Ppzz)), mul(e, Ppzzt));
+    kpt = sub(sub(kpt, mul(Pet, pzz)), mul(Pe, ptzz));
+    const cpx<T> kdt = sub(sub(add(mul(pz, add(mul(pxz, ex), mul(pyz, ey))),
+                                   mul(e, add(add(add(mul(pxzt, ex), mul(pxz, pxz)), mul(pyzt, ey)), mul(pyz, pyz)))),
+                               mul(mul(e, pz), pzzz)),
+                           scl(mul(e2, pzzzt), T(0.5)));
+    cpx<T> dpt = add(add(add(mul(pz, Pptz), mul(e, Pptzt)), mul(Pet, ptz)), mul(Pe, ptzt));
+    dpt = add(sub(dpt, mul(ex, Ppx)), mul(px, Ppxt));
+    dpt = add(sub(dpt, mul(ey, Ppy)), mul(py, Ppyt));
+    dpt = add(add(dpt, mul(ptz, Ppz)), mul(pz, Pptz));
+    cpx<T> inner = add(sub(mul(px, pxzt), mul(ex, pxz)), sub(mul(py, pyzt), mul(ey, pyz)));
+    inner = add(add(inner, mul(ptz, pzz)), mul(pz, ptzz));
+    const cpx<T> ddt = add(add(sub(mul(mul(e, pz), ptzz), scl(mul(e2, pzzz), T(0.5))),
+                               mul(pz, add(add(mul(px, pxz), mul(py, pyz)), mul(pz, pzz)))),
+                           mul(e, inner));
+    o[0] = add(scl(kd, T(0.25)), scl(kp, T(0.5)));
+    o[1] = add(scl(dd, T(0.25)), scl(dp, T(0.5)));
+    o[2] = add(scl(kdt, T(0.25)), scl(kpt, T(0.5)));
+    o[3] = add(scl(ddt, T(0.25)), scl(dpt, T(0.5)));
+  }
+};
+
+// Order-4 forcing partitions 3+1, 2+2, 2+1+1, 1+1+1+1 (gl_eta44_fftw.cpp:711-735).
+// in: first 0..15 as Third44Op, 16 pxzz 17 pyzz 18 pzzzz
+//     pair  19 eta 20 ex 21 ey 22 eta_t 23 ex_t 24 ey_t 25 px 26 py 27 pz 28 pzz
+//           29 ptz 30 px_t 31 py_t 32 pzz_t 33 ptz_t 34 pxz 35 pyz 36 pzzz
+//     third 37 eta 38 ex 39 ey 40 px 41 py 42 pz 43 pzz 44 pzt
+// out: fk, fd
+struct Quartic44Op {
+  template <typename T>
+  __device__ __forceinline__ static void apply(const cpx<T>* f, cpx<T>* o) {
+    const cpx<T> e = f[0], ex = f[1], ey = f[2], px = f[3], py = f[4], pz = f[5], pxz = f[6], pyz = f[7];
+    const cpx<T> pzz = f[8], pzzz = f[9], ptz = f[10], ptzz = f[11], pzzzt = f[14];
+    const cpx<T> pxzz = f[16], pyzz = f[17], pzzzz = f[18];
+    const cpx<T> Pe = f[19], Pex = f[20], Pey = f[21], Ppx = f[25], Ppy = f[26], Ppz = f[27], Ppzz = f[28];
+    const cpx<T> Pptz = f[29], Ppzzt = f[32], Ppxz = f[34], Ppyz = f[35], Ppzzz = f[36];
+    const cpx<T> Te = f[37], Tex = f[38], Tey = f[39], Tpx = f[40], Tpy = f[41], Tpz = f[42], Tpzz = f[43];
+    const cpx<T> Tpzt = f[44];
+    const cpx<T> e2 = mul(e, e), e3 = mul(e2, e);
+    const cpx<T> k31 = sub(sub(add(add(add(mul(px, Tex), mul(py, Tey)), mul(Tpx, ex)), mul(Tpy, ey)), mul(e, Tpzz)),
+                           mul(Te, pzz));
+    const cpx<T> d31 = add(add(add(add(mul(e, Tpzt), mul(Te, ptz)), mul(px, Tpx)), mul(py, Tpy)), mul(pz, Tpz));
+    const cpx<T> k22 = sub(add(mul(Ppx, Pex), mul(Ppy, Pey)), mul(Pe, Ppzz));
+    const cpx<T> d22 = add(mul(Pe, Pptz), scl(add(add(mul(Ppx, Ppx), mul(Ppy, Ppy)), mul(Ppz, Ppz)), T(0.5)));
+    const cpx<T> ePxz = add(mul(e, Ppxz), mul(Pe, pxz));
+    const cpx<T> ePyz = add(mul(e, Ppyz), mul(Pe, pyz));
+    cpx<T> k211 = mul(e, add(mul(pxz, Pex), mul(pyz, Pey)));
+    k211 = add(add(k211, mul(ePxz, ex)), mul(ePyz, ey));
+    k211 = sub(sub(k211, scl(mul(e2, Ppzzz), T(0.5))), mul(mul(e, Pe), pzzz));
+    cpx<T> d211 = add(scl(mul(e2, Ppzzt), T(0.5)), mul(mul(e, Pe), ptzz));
+    d211 = add(add(add(d211, mul(px, ePxz)), mul(py, ePyz)), mul(pz, add(mul(e, Ppzz), mul(Pe, pzz))));
+    d211 = add(add(add(d211, mul(Ppx, mul(e, pxz))), mul(Ppy, mul(e, pyz))), mul(Ppz, mul(e, pzz)));
+    const cpx<T> k1111 = sub(scl(mul(e2, add(mul(pxzz, ex), mul(pyzz, ey))), T(0.5)),
+                             scl(mul(e3, pzzzz), T(1.0 / 6.0)));
+    const cpx<T> d1111 = add(add(scl(mul(e3, pzzzt), T(1.0 / 6.0)),
+                                 scl(mul(e2, add(add(mul(px, pxzz), mul(py, pyzz)), mul(pz, pzzz))), T(0.5))),
+                             scl(mul(e2, add(add(mul(pxz, pxz), mul(pyz, pyz)), mul(pzz, pzz))), T(0.5)));
+    o[0] = add(scl(k31, T(0.5)), add(add(scl(k1111, T(0.125)), scl(k211, T(0.25))), scl(k22, T(0.5))));
+    o[1] = add(scl(d31, T(0.5)), add(add(scl(d1111, T(0.125)), scl(d211, T(0.25))), scl(d22, T(0.5))));
+  }
+};
+
+}  // namespace glw

@@ -259,6 +259,125 @@ __global__ __launch_bounds__(NT) void k_acc_psi22(AccPsi22Args g) {
   g.out[((size_t)t * 2 + 1) * g.box + b] = v;
 }
 
+
+// ---------------------------------------------------------------------------
+// Order-4 accumulations (paper/order4/cpp/gl_eta44_fftw.cpp)
+// ---------------------------------------------------------------------------
+struct Acc4Args {
+  const cpx<double>* spec;  // [nset*J][nsrc][box]
+  const uint64_t* fmax;     // [nset*J][nsrc]
+  double floor_level;
+  int J, box, nsrc, nacc;
+  const double* a;
+  const uint8_t* mask;
+  const double* ch;         // [J][box]
+  const double* sh;
+  const double* coef;       // [J]
+  cpx<double>* out;         // [nset][nacc][box]
+  uint64_t* omax;           // [nset][nacc]
+};
+
+// Pair level (:404-428): 6 sources -> es, ets, ps, pts, ptts, then masked
+// (floors applied by the consumer with omax).
+__global__ __launch_bounds__(NT) void k_acc_pair44(Acc4Args g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int set = blockIdx.y;
+  cpx<double> acc[5] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  const bool on = b < g.box && g.mask[b];
+  if (on) {
+    const double A = g.a[b];
+    for (int i = 0; i < g.J; ++i) {
+      const size_t task = (size_t)set * g.J + i;
+      cpx<double> s[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        s[k] = floored(g.spec[(task * 6 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 6 + k]));
+      const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
+      // es += c (-A sh sd + ch sk)
+      acc[0].x += c * (-A * shv * s[0].x + chv * s[1].x);
+      acc[0].y += c * (-A * shv * s[0].y + chv * s[1].y);
+      // ets -= i c (-A sh sds + ch sks)
+      const cpx<double> z1 = {c * (-A * shv * s[2].x + chv * s[3].x), c * (-A * shv * s[2].y + chv * s[3].y)};
+      acc[1] = {acc[1].x + z1.y, acc[1].y - z1.x};
+      // ps += i c (ch sd - sh sk)
+      const cpx<double> z2 = {c * (chv * s[0].x - shv * s[1].x), c * (chv * s[0].y - shv * s[1].y)};
+      acc[2] = {acc[2].x - z2.y, acc[2].y + z2.x};
+      // pts += c (ch sds - sh sks)
+      acc[3].x += c * (chv * s[2].x - shv * s[3].x);
+      acc[3].y += c * (chv * s[2].y - shv * s[3].y);
+      // ptts -= i c (ch sdss - sh skss)
+      const cpx<double> z4 = {c * (chv * s[4].x - shv * s[5].x), c * (chv * s[4].y - shv * s[5].y)};
+      acc[4] = {acc[4].x + z4.y, acc[4].y - z4.x};
+    }
+  }
+  if (b < g.box)
+    for (int k = 0; k < 5; ++k) g.out[((size_t)set * 5 + k) * g.box + b] = acc[k];
+  for (int k = 0; k < 5; ++k) {
+    block_max_atomic(on ? hypot(acc[k].x, acc[k].y) : 0.0, g.omax + (size_t)set * 5 + k);
+    __syncthreads();
+  }
+}
+
+// Third level (:594-615): 4 sources -> es, ps, pts.
+__global__ __launch_bounds__(NT) void k_acc_third44(Acc4Args g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int set = blockIdx.y;
+  cpx<double> es = {0, 0}, ps = {0, 0}, pts = {0, 0};
+  const bool on = b < g.box && g.mask[b];
+  if (on) {
+    const double A = g.a[b];
+    for (int i = 0; i < g.J; ++i) {
+      const size_t task = (size_t)set * g.J + i;
+      cpx<double> s[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        s[k] = floored(g.spec[(task * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 4 + k]));
+      const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
+      // es += c (A sh fd - i ch fk)
+      es.x += c * (A * shv * s[1].x + chv * s[0].y);
+      es.y += c * (A * shv * s[1].y - chv * s[0].x);
+      // ps += c (-i ch fd - sh fk)
+      ps.x += c * (chv * s[1].y - shv * s[0].x);
+      ps.y += c * (-chv * s[1].x - shv * s[0].y);
+      // pts += c (-i ch fdt - sh fkt)
+      pts.x += c * (chv * s[3].y - shv * s[2].x);
+      pts.y += c * (-chv * s[3].x - shv * s[2].y);
+    }
+  }
+  if (b < g.box) {
+    g.out[((size_t)set * 3 + 0) * g.box + b] = es;
+    g.out[((size_t)set * 3 + 1) * g.box + b] = ps;
+    g.out[((size_t)set * 3 + 2) * g.box + b] = pts;
+  }
+  block_max_atomic(on ? hypot(es.x, es.y) : 0.0, g.omax + (size_t)set * 3);
+  __syncthreads();
+  block_max_atomic(on ? hypot(ps.x, ps.y) : 0.0, g.omax + (size_t)set * 3 + 1);
+  __syncthreads();
+  block_max_atomic(on ? hypot(pts.x, pts.y) : 0.0, g.omax + (size_t)set * 3 + 2);
+}
+
+// Quartic level (:736-750): 2 sources -> eta44 spectrum.
+__global__ __launch_bounds__(NT) void k_acc_quartic44(Acc4Args g) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int set = blockIdx.y;
+  cpx<double> o = {0, 0};
+  const bool on = b < g.box && g.mask[b];
+  if (on) {
+    const double A = g.a[b];
+    for (int i = 0; i < g.J; ++i) {
+      const size_t task = (size_t)set * g.J + i;
+      const cpx<double> fk = floored(g.spec[(task * 2) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2]));
+      const cpx<double> fd =
+          floored(g.spec[(task * 2 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2 + 1]));
+      const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
+      o.x += c * (A * shv * fd.x + chv * fk.y);
+      o.y += c * (A * shv * fd.y - chv * fk.x);
+    }
+  }
+  if (b < g.box) g.out[(size_t)set * g.box + b] = o;
+  block_max_atomic(on ? hypot(o.x, o.y) : 0.0, g.omax + set);
+}
+
 // ----------------------------------------------------------------------------
 // Dispatch over FFT lengths
 // ----------------------------------------------------------------------------
@@ -402,6 +521,13 @@ struct glw_plan_s {
   uint8_t *mask2 = nullptr, *mask3 = nullptr;
   double *in_ch = nullptr, *in_sh = nullptr, *in_coef = nullptr;
   double *out_ch = nullptr, *out_sh = nullptr, *out_coef = nullptr;
+  // order 4
+  DevTables t4;
+  uint8_t* mask4 = nullptr;
+  double *o4_dq = nullptr, *o4_dt = nullptr, *o4_dp3 = nullptr, *o4_dp4 = nullptr;
+  double *o4_pch = nullptr, *o4_psh = nullptr, *o4_pco = nullptr;
+  double *o4_tch = nullptr, *o4_tsh = nullptr, *o4_tco = nullptr;
+  double *o4_qch = nullptr, *o4_qsh = nullptr, *o4_qco = nullptr;
   // order 2
   double *e22_damp = nullptr, *e22_ch = nullptr, *e22_sh = nullptr, *e22_coef = nullptr;
   double *p22_damp = nullptr, *p22_fac = nullptr, *p22_coef = nullptr, *p22_sign = nullptr;
@@ -527,6 +653,24 @@ int validate(const glw_plan_desc* d) {
       return fail(GLW_EINVAL, "GL ranks must be 1..16");
     if (d->p2.wx > d->g3x || d->p2.wy > d->g3y) return fail(GLW_EINVAL, "pair box wider than the cubic grid");
   }
+  if (d->with_order4) {
+    const glw_order4_desc& o = d->o4;
+    if (o.rank < 1 || o.rank > 12) return fail(GLW_EINVAL, "order-4 GL rank must be 1..12");
+    if (!length_supported(d->g2x) || !length_supported(d->g2y) || !length_supported(d->g3x) ||
+        !length_supported(d->g3y) || !length_supported(o.g4x) || !length_supported(o.g4y))
+      return fail(GLW_EUNSUPPORTED, "order-4 working grid length not supported");
+    if (2 * d->p1.wx - 1 > d->g2x || 2 * d->p1.wy - 1 > d->g2y || 3 * d->p1.wx - 2 > d->g3x ||
+        3 * d->p1.wy - 2 > d->g3y || 4 * d->p1.wx - 3 > o.g4x || 4 * d->p1.wy - 3 > o.g4y)
+      return fail(GLW_EINVAL, "order-4 working grids narrower than the product supports");
+    if (!box_ok(d->p2, d->g2x, d->g2y) || !box_ok(d->p3, d->g3x, d->g3y) || !box_ok(o.p4, o.g4x, o.g4y) ||
+        !box_ok(o.p4, d->nx, d->ny))
+      return fail(GLW_EINVAL, "order-4 boxes do not fit the working grids");
+    if (d->p2.wx > o.g4x || d->p2.wy > o.g4y || d->p3.wx > o.g4x || d->p3.wy > o.g4y || d->p2.wx > d->g3x ||
+        d->p2.wy > d->g3y)
+      return fail(GLW_EINVAL, "lower-order boxes wider than the higher-order grids");
+    if ((long long)o.rank * o.rank * o.rank * d->max_instants > 65535)
+      return fail(GLW_EINVAL, "order-4 task count exceeds the launch grid (lower max_instants)");
+  }
   if (d->order >= 2 && (d->o2.eta_rank < 1 || d->o2.eta_rank > 16 || d->o2.channels != 2))
     return fail(GLW_EINVAL, "order-2 tables: eta rank 1..16 and two psi22 channels required");
   return GLW_OK;
@@ -604,6 +748,63 @@ int build(glw_plan* P) {
     if ((rc = P->reserve("max_combo", nt * O * 2 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("ep33", nt * 2 * b3 * zsz))) return rc;
   }
+  if (d.with_order4) {
+    const glw_order4_desc& o = d.o4;
+    const size_t J = o.rank, b4 = box_size(o.p4);
+    wfin = std::max({wfin, (int)d.p2.wx, (int)o.p4.wx});
+    if (d.order < 2) {  // order-4-only plan: pair tables and grids
+      if ((rc = P->upload_tables(P->t2, d.t2, b2))) return rc;
+      if ((rc = P->upload(&P->mask2, d.mask2, b2))) return rc;
+      if ((rc = P->twiddles(d.g2x))) return rc;
+      if ((rc = P->twiddles(d.g2y))) return rc;
+    }
+    if (d.order < 3) {
+      if ((rc = P->upload_tables(P->t3, d.t3, b3))) return rc;
+      if ((rc = P->upload(&P->mask3, d.mask3, b3))) return rc;
+      if ((rc = P->twiddles(d.g3x))) return rc;
+      if ((rc = P->twiddles(d.g3y))) return rc;
+    }
+    if ((rc = P->upload_tables(P->t4, o.t4, b4))) return rc;
+    if ((rc = P->upload(&P->mask4, o.mask4, b4))) return rc;
+    if ((rc = P->twiddles(o.g4x))) return rc;
+    if ((rc = P->twiddles(o.g4y))) return rc;
+    if ((rc = P->upload(&P->o4_dq, o.damp_quartic, J * b1))) return rc;
+    if ((rc = P->upload(&P->o4_dt, o.damp_third, J * J * b1))) return rc;
+    if ((rc = P->upload(&P->o4_dp3, o.damp_pair3, J * J * J * b1))) return rc;
+    if ((rc = P->upload(&P->o4_dp4, o.damp_pair4, J * J * b1))) return rc;
+    if ((rc = P->upload(&P->o4_pch, o.pair_cosh, J * b2))) return rc;
+    if ((rc = P->upload(&P->o4_psh, o.pair_sinhr, J * b2))) return rc;
+    if ((rc = P->upload(&P->o4_pco, o.pair_coef, J))) return rc;
+    if ((rc = P->upload(&P->o4_tch, o.third_cosh, J * b3))) return rc;
+    if ((rc = P->upload(&P->o4_tsh, o.third_sinhr, J * b3))) return rc;
+    if ((rc = P->upload(&P->o4_tco, o.third_coef, J))) return rc;
+    if ((rc = P->upload(&P->o4_qch, o.quartic_cosh, J * b4))) return rc;
+    if ((rc = P->upload(&P->o4_qsh, o.quartic_sinhr, J * b4))) return rc;
+    if ((rc = P->upload(&P->o4_qco, o.quartic_coef, J))) return rc;
+    const size_t g2 = d.g2y, g3 = d.g3y, g4 = o.g4y;
+    const size_t w12 = std::max(d.p1.wx, d.p2.wx), w123 = std::max<size_t>(w12, d.p3.wx);
+    const size_t np = nt * J * J * J;  // pair tasks of the third level (the larger pair batch)
+    if ((rc = P->reserve("o4_pt", np * 16 * g2 * d.p1.wx * csz))) return rc;
+    if ((rc = P->reserve("o4_pu", np * 6 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("o4_ps", np * 6 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("o4_pm", np * 6 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("o4_set3", nt * J * J * 5 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("o4_max3", nt * J * J * 5 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("o4_set4", nt * J * 5 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("o4_max4", nt * J * 5 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("o4_tt", nt * J * J * 31 * g3 * w12 * csz))) return rc;
+    if ((rc = P->reserve("o4_tu", nt * J * J * 4 * g3 * d.p3.wx * csz))) return rc;
+    if ((rc = P->reserve("o4_ts", nt * J * J * 4 * b3 * zsz))) return rc;
+    if ((rc = P->reserve("o4_tm", nt * J * J * 4 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("o4_th", nt * J * 3 * b3 * zsz))) return rc;
+    if ((rc = P->reserve("o4_thm", nt * J * 3 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("o4_qt", nt * J * 45 * g4 * w123 * csz))) return rc;
+    if ((rc = P->reserve("o4_qu", nt * J * 2 * g4 * o.p4.wx * csz))) return rc;
+    if ((rc = P->reserve("o4_qs", nt * J * 2 * b4 * zsz))) return rc;
+    if ((rc = P->reserve("o4_qm", nt * J * 2 * sizeof(uint64_t)))) return rc;
+    if ((rc = P->reserve("o4_out", nt * b4 * zsz))) return rc;
+    if ((rc = P->reserve("o4_outm", nt * sizeof(uint64_t)))) return rc;
+  }
   if ((rc = P->reserve("tfin", nt * 2 * (size_t)d.ny * wfin * zsz))) return rc;
   return GLW_OK;
 }
@@ -650,9 +851,13 @@ ColInvArgs colinv_base(const cpx<double>* spec, int nsrc, int sdiv, const glw_bo
 // (src, scale, multiplier) triple.
 int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& box, const DevTables& tab,
                int nt, const int* srcs, const int* codes, const double* scales, double** outs, int nout,
-               hipStream_t st) {
+               hipStream_t st, const uint64_t* fmax = nullptr, int fmax_stride = 1, int floor_mask = 0) {
   const glw_plan_desc& d = P->d;
   ColInvArgs a = colinv_base(spec, nsrc, 1, box, tab);
+  a.fmax = fmax;
+  a.fmax_stride = fmax_stride;
+  a.floor_src_mask = floor_mask;
+  a.floor_level = d.floor_level;
   a.nfield = nout;
   for (int f = 0; f < nout; ++f) {
     a.fsrc[f] = srcs[f];
@@ -1041,14 +1246,299 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
   return synthesize(P, P->buf<cpx<double>>("ep22"), 2, d.p2, P->t2, nt, srcs, codes, scales, outs, 2, st);
 }
 
+
+template <typename T>
+int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
+  const glw_plan_desc& d = P->d;
+  const glw_order4_desc& o = d.o4;
+  const bool f32 = std::is_same<T, float>::value;
+  const int J = o.rank;
+  const int b2 = box_size(d.p2), b3 = box_size(d.p3), b4 = box_size(o.p4);
+  const double nn = (double)d.nx * d.ny, sc = 1.0 / nn;
+  const cpx<double>* stp = P->buf<cpx<double>>("st");
+  int rc;
+  auto fields = [](ColInvArgs& a, int n, const int* srcs, const int* codes, double scale) {
+    a.nfield = n;
+    for (int f = 0; f < n; ++f) {
+      a.fsrc[f] = srcs ? srcs[f] : 0;
+      a.fcode[f] = codes[f];
+      a.fscale[f] = scale;
+    }
+  };
+  // ---- pair level (paper/order4/cpp/gl_eta44_fftw.cpp:299-529)
+  static const int pair_codes[16] = {M_ONE,   M_NU,    M_NU2, M_NU3, M_NU4, M_QX_NU,  M_QY_NU,  M_QX,
+                                     M_QY,    M_Q2_NU, M_Q2,  M_Q2NU, M_NU_QX, M_NU_QY, M_NU2_QX, M_NU2_QY};
+  auto pair_level = [&](int per_instant, const double* damp, const char* set, const char* setmax) -> int {
+    const int ntask = nt * per_instant, nset = ntask / J;
+    int rc2;
+    ColInvArgs a = colinv_base(stp, 1, per_instant, d.p1, P->t1);
+    a.damp = damp;
+    a.dmod = per_instant;
+    fields(a, 16, nullptr, pair_codes, sc);
+    a.out = P->ws.at("o4_pt").first;
+    a.out_nfield = 16;
+    a.ostride = d.p1.wx;
+    if ((rc2 = GLW_STAGE(P, "o4_pair_colinv", st, launch_colinv<T>(d.g2y, a, ntask, P->tw(d.g2y, f32), st))))
+      return rc2;
+    RowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.in = P->ws.at("o4_pt").first;
+    r.nin = 16;
+    r.istride = d.p1.wx;
+    for (int f = 0; f < 16; ++f) {
+      r.fmx0[f] = d.p1.mx0;
+      r.fwx[f] = d.p1.wx;
+    }
+    r.out = P->ws.at("o4_pu").first;
+    r.nout = 6;
+    r.ostride = d.p2.wx;
+    r.omx0 = d.p2.mx0;
+    r.owx = d.p2.wx;
+    r.gy = d.g2y;
+    if ((rc2 = GLW_STAGE(P, "o4_pair_row", st,
+                         (launch_row<T, 16, 6, Pair44Op>(d.g2x, r, ntask, P->tw(d.g2x, f32), st)))))
+      return rc2;
+    if ((rc2 = fill_zero(P->buf<uint64_t>("o4_pm"), (size_t)ntask * 6, st))) return rc2;
+    ColFwdArgs c;
+    c.in = P->ws.at("o4_pu").first;
+    c.nsrc = 6;
+    c.istride = d.p2.wx;
+    c.omx0 = d.p2.mx0;
+    c.owx = d.p2.wx;
+    c.omy0 = d.p2.my0;
+    c.owy = d.p2.wy;
+    c.mask = P->mask2;
+    c.scale = nn / ((double)d.g2x * d.g2y);
+    c.out = P->buf<cpx<double>>("o4_ps");
+    c.fmax = P->buf<uint64_t>("o4_pm");
+    if ((rc2 = GLW_STAGE(P, "o4_pair_colfwd", st, launch_colfwd<T>(d.g2y, c, ntask, P->tw(d.g2y, f32), st))))
+      return rc2;
+    if ((rc2 = fill_zero(P->buf<uint64_t>(setmax), (size_t)nset * 5, st))) return rc2;
+    Acc4Args g;
+    g.spec = P->buf<cpx<double>>("o4_ps");
+    g.fmax = P->buf<uint64_t>("o4_pm");
+    g.floor_level = d.floor_level;
+    g.J = J;
+    g.box = b2;
+    g.nsrc = 6;
+    g.nacc = 5;
+    g.a = P->t2.a;
+    g.mask = P->mask2;
+    g.ch = P->o4_pch;
+    g.sh = P->o4_psh;
+    g.coef = P->o4_pco;
+    g.out = P->buf<cpx<double>>(set);
+    g.omax = P->buf<uint64_t>(setmax);
+    P->mark_begin("o4_pair_acc", st);
+    hipLaunchKernelGGL(k_acc_pair44, dim3((b2 + NT - 1) / NT, nset), dim3(NT), 0, st, g);
+    P->mark_end(st);
+    GLW_HIP(hipGetLastError());
+    return GLW_OK;
+  };
+  if ((rc = pair_level(J * J * J, P->o4_dp3, "o4_set3", "o4_max3"))) return rc;  // pairs of the third level
+  if ((rc = pair_level(J * J, P->o4_dp4, "o4_set4", "o4_max4"))) return rc;      // pairs of the quartic level
+  static const int first_codes[19] = {M_ONE,     M_I_QX,    M_I_QY,     M_QX_NU,     M_QY_NU,     M_MI_NU,  M_NU_QX,
+                                      M_NU_QY,   M_MI_Q2_NU, M_MI_Q2NU, M_MA,        M_MQ2,       M_MI_NU2_QX,
+                                      M_MI_NU2_QY, M_MQ2A,  M_I_ANU,    M_QXQ2_NU,   M_QYQ2_NU,   M_MI_Q4_NU};
+  static const int pair_src[18] = {0, 0, 0, 1, 1, 1, 2, 2, 2, 2, 3, 3, 3, 3, 4, 2, 2, 2};
+  static const int pair_out_codes[18] = {M_ONE, M_I_QX, M_I_QY, M_ONE, M_I_QX, M_I_QY, M_I_QX,  M_I_QY,  M_A,
+                                         M_Q2,  M_A,    M_I_QX, M_I_QY, M_Q2,  M_A,    M_I_QXA, M_I_QYA, M_Q2A};
+  // ---- third level (:537-637): tasks (t, o4, o3)
+  const int w12 = std::max(d.p1.wx, d.p2.wx);
+  {
+    const int ntask = nt * J * J;
+    ColInvArgs a = colinv_base(stp, 1, J * J, d.p1, P->t1);
+    a.damp = P->o4_dt;
+    a.dmod = J * J;
+    fields(a, 16, nullptr, first_codes, sc);
+    a.out = P->ws.at("o4_tt").first;
+    a.out_nfield = 31;
+    a.out_foff = 0;
+    a.ostride = w12;
+    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, a, ntask, P->tw(d.g3y, f32), st))))
+      return rc;
+    ColInvArgs b = colinv_base(P->buf<cpx<double>>("o4_set3"), 5, 1, d.p2, P->t2);
+    b.fmax = P->buf<uint64_t>("o4_max3");
+    b.fmax_stride = 5;
+    b.floor_src_mask = 0x1f;
+    b.floor_level = d.floor_level;
+    fields(b, 15, pair_src, pair_out_codes, sc);
+    b.out = P->ws.at("o4_tt").first;
+    b.out_nfield = 31;
+    b.out_foff = 16;
+    b.ostride = w12;
+    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, b, ntask, P->tw(d.g3y, f32), st))))
+      return rc;
+    RowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.in = P->ws.at("o4_tt").first;
+    r.nin = 31;
+    r.istride = w12;
+    for (int f = 0; f < 31; ++f) {
+      r.fmx0[f] = f < 16 ? d.p1.mx0 : d.p2.mx0;
+      r.fwx[f] = f < 16 ? d.p1.wx : d.p2.wx;
+    }
+    r.out = P->ws.at("o4_tu").first;
+    r.nout = 4;
+    r.ostride = d.p3.wx;
+    r.omx0 = d.p3.mx0;
+    r.owx = d.p3.wx;
+    r.gy = d.g3y;
+    if ((rc = GLW_STAGE(P, "o4_third_row", st,
+                        (launch_row<T, 31, 4, Third44Op>(d.g3x, r, ntask, P->tw(d.g3x, f32), st)))))
+      return rc;
+    if ((rc = fill_zero(P->buf<uint64_t>("o4_tm"), (size_t)ntask * 4, st))) return rc;
+    ColFwdArgs c;
+    c.in = P->ws.at("o4_tu").first;
+    c.nsrc = 4;
+    c.istride = d.p3.wx;
+    c.omx0 = d.p3.mx0;
+    c.owx = d.p3.wx;
+    c.omy0 = d.p3.my0;
+    c.owy = d.p3.wy;
+    c.mask = P->mask3;
+    c.scale = nn / ((double)d.g3x * d.g3y);
+    c.out = P->buf<cpx<double>>("o4_ts");
+    c.fmax = P->buf<uint64_t>("o4_tm");
+    if ((rc = GLW_STAGE(P, "o4_third_colfwd", st, launch_colfwd<T>(d.g3y, c, ntask, P->tw(d.g3y, f32), st))))
+      return rc;
+    if ((rc = fill_zero(P->buf<uint64_t>("o4_thm"), (size_t)nt * J * 3, st))) return rc;
+    Acc4Args g;
+    g.spec = P->buf<cpx<double>>("o4_ts");
+    g.fmax = P->buf<uint64_t>("o4_tm");
+    g.floor_level = d.floor_level;
+    g.J = J;
+    g.box = b3;
+    g.nsrc = 4;
+    g.nacc = 3;
+    g.a = P->t3.a;
+    g.mask = P->mask3;
+    g.ch = P->o4_tch;
+    g.sh = P->o4_tsh;
+    g.coef = P->o4_tco;
+    g.out = P->buf<cpx<double>>("o4_th");
+    g.omax = P->buf<uint64_t>("o4_thm");
+    P->mark_begin("o4_third_acc", st);
+    hipLaunchKernelGGL(k_acc_third44, dim3((b3 + NT - 1) / NT, nt * J), dim3(NT), 0, st, g);
+    P->mark_end(st);
+    GLW_HIP(hipGetLastError());
+  }
+  // ---- quartic level (:696-750): tasks (t, o4)
+  const int w123 = std::max(w12, (int)d.p3.wx);
+  {
+    const int ntask = nt * J;
+    ColInvArgs a = colinv_base(stp, 1, J, d.p1, P->t1);
+    a.damp = P->o4_dq;
+    a.dmod = J;
+    fields(a, 19, nullptr, first_codes, sc);
+    a.out = P->ws.at("o4_qt").first;
+    a.out_nfield = 45;
+    a.out_foff = 0;
+    a.ostride = w123;
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, a, ntask, P->tw(o.g4y, f32), st))))
+      return rc;
+    ColInvArgs b = colinv_base(P->buf<cpx<double>>("o4_set4"), 5, 1, d.p2, P->t2);
+    b.fmax = P->buf<uint64_t>("o4_max4");
+    b.fmax_stride = 5;
+    b.floor_src_mask = 0x1f;
+    b.floor_level = d.floor_level;
+    fields(b, 18, pair_src, pair_out_codes, sc);
+    b.out = P->ws.at("o4_qt").first;
+    b.out_nfield = 45;
+    b.out_foff = 19;
+    b.ostride = w123;
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, b, ntask, P->tw(o.g4y, f32), st))))
+      return rc;
+    static const int third_src[8] = {0, 0, 0, 1, 1, 1, 1, 2};
+    static const int third_codes[8] = {M_ONE, M_I_QX, M_I_QY, M_I_QX, M_I_QY, M_A, M_Q2, M_A};
+    ColInvArgs c3 = colinv_base(P->buf<cpx<double>>("o4_th"), 3, 1, d.p3, P->t3);
+    c3.fmax = P->buf<uint64_t>("o4_thm");
+    c3.fmax_stride = 3;
+    c3.floor_src_mask = 0x7;
+    c3.floor_level = d.floor_level;
+    fields(c3, 8, third_src, third_codes, sc);
+    c3.out = P->ws.at("o4_qt").first;
+    c3.out_nfield = 45;
+    c3.out_foff = 37;
+    c3.ostride = w123;
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, c3, ntask, P->tw(o.g4y, f32), st))))
+      return rc;
+    RowArgs r;
+    memset(&r, 0, sizeof(r));
+    r.in = P->ws.at("o4_qt").first;
+    r.nin = 45;
+    r.istride = w123;
+    for (int f = 0; f < 45; ++f) {
+      const glw_box& bx = f < 19 ? d.p1 : (f < 37 ? d.p2 : d.p3);
+      r.fmx0[f] = bx.mx0;
+      r.fwx[f] = bx.wx;
+    }
+    r.out = P->ws.at("o4_qu").first;
+    r.nout = 2;
+    r.ostride = o.p4.wx;
+    r.omx0 = o.p4.mx0;
+    r.owx = o.p4.wx;
+    r.gy = o.g4y;
+    if ((rc = GLW_STAGE(P, "o4_quartic_row", st,
+                        (launch_row<T, 45, 2, Quartic44Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), st)))))
+      return rc;
+    if ((rc = fill_zero(P->buf<uint64_t>("o4_qm"), (size_t)ntask * 2, st))) return rc;
+    ColFwdArgs c;
+    c.in = P->ws.at("o4_qu").first;
+    c.nsrc = 2;
+    c.istride = o.p4.wx;
+    c.omx0 = o.p4.mx0;
+    c.owx = o.p4.wx;
+    c.omy0 = o.p4.my0;
+    c.owy = o.p4.wy;
+    c.mask = P->mask4;
+    c.scale = nn / ((double)o.g4x * o.g4y);
+    c.out = P->buf<cpx<double>>("o4_qs");
+    c.fmax = P->buf<uint64_t>("o4_qm");
+    if ((rc = GLW_STAGE(P, "o4_quartic_colfwd", st, launch_colfwd<T>(o.g4y, c, ntask, P->tw(o.g4y, f32), st))))
+      return rc;
+    if ((rc = fill_zero(P->buf<uint64_t>("o4_outm"), (size_t)nt, st))) return rc;
+    Acc4Args g;
+    g.spec = P->buf<cpx<double>>("o4_qs");
+    g.fmax = P->buf<uint64_t>("o4_qm");
+    g.floor_level = d.floor_level;
+    g.J = J;
+    g.box = b4;
+    g.nsrc = 2;
+    g.nacc = 1;
+    g.a = P->t4.a;
+    g.mask = P->mask4;
+    g.ch = P->o4_qch;
+    g.sh = P->o4_qsh;
+    g.coef = P->o4_qco;
+    g.out = P->buf<cpx<double>>("o4_out");
+    g.omax = P->buf<uint64_t>("o4_outm");
+    P->mark_begin("o4_quartic_acc", st);
+    hipLaunchKernelGGL(k_acc_quartic44, dim3((b4 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
+    P->mark_end(st);
+    GLW_HIP(hipGetLastError());
+  }
+  const int srcs[1] = {0}, codes[1] = {M_ONE};
+  const double scales[1] = {d.depth / nn};
+  double* outs[1] = {eta};
+  return synthesize(P, P->buf<cpx<double>>("o4_out"), 1, o.p4, P->t4, nt, srcs, codes, scales, outs, 1, st,
+                    P->buf<uint64_t>("o4_outm"), 1, 1);
+}
+
+int order4_dispatch(glw_plan* P, int nt, double* a, double* /*unused*/, hipStream_t st) {
+  return P->d.precision == GLW_PREC_FP32 ? order4_chunk<float>(P, nt, a, st) : order4_chunk<double>(P, nt, a, st);
+}
+
 typedef int (*chunk_fn)(glw_plan*, int, double*, double*, hipStream_t);
+
+int order4_dispatch(glw_plan* P, int nt, double* a, double* b, hipStream_t st);
 
 int run_chunks(glw_plan* P, const double* s0, const double* times, int nt, double* a, double* b, void* stream,
                chunk_fn fn) {
   if (!P) return fail(GLW_EINVAL, "null plan");
   if (nt < 0) return fail(GLW_EINVAL, "negative instant count");
   if (nt == 0) return GLW_OK;
-  if (!s0 || !times || !a || !b) return fail(GLW_EINVAL, "null array argument");
+  if (!s0 || !times || !a || (!b && fn != order4_dispatch)) return fail(GLW_EINVAL, "null array argument");
   GLW_HIP(hipSetDevice(P->device));
   hipStream_t st = (hipStream_t)stream;
   const size_t plane = (size_t)P->d.ny * P->d.nx * 2;  // doubles per output field
@@ -1056,7 +1546,7 @@ int run_chunks(glw_plan* P, const double* s0, const double* times, int nt, doubl
     const int n = std::min(P->d.max_instants, nt - t0);
     int rc = phase(P, s0, times + t0, n, st);
     if (rc) return rc;
-    rc = fn(P, n, a + (size_t)t0 * plane, b + (size_t)t0 * plane, st);
+    rc = fn(P, n, a + (size_t)t0 * plane, b ? b + (size_t)t0 * plane : nullptr, st);
     if (rc) return rc;
   }
   return GLW_OK;
@@ -1124,6 +1614,12 @@ int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, in
                        double* psi33, void* stream) {
   if (plan && plan->d.order < 3) return fail(GLW_EINVAL, "plan built without order-3 tables");
   return run_chunks(plan, s0, times, nt, eta33, psi33, stream, order3_dispatch);
+}
+
+int glw_order4_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta44,
+                       void* stream) {
+  if (plan && !plan->d.with_order4) return fail(GLW_EINVAL, "plan built without order-4 tables");
+  return run_chunks(plan, s0, times, nt, eta44, nullptr, stream, order4_dispatch);
 }
 
 int glw_plan_profile(glw_plan* plan, int32_t enable) {

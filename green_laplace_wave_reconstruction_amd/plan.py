@@ -172,6 +172,11 @@ class HostPlan:
     peak_q: float = 0.0
     o2: dict[str, Any] = field(default_factory=dict)
     o3: dict[str, Any] = field(default_factory=dict)
+    o4: dict[str, Any] = field(default_factory=dict)
+    box4: Box | None = None
+    g4: tuple[int, int] | None = None
+    t4: BoxTables | None = None
+    mask4: np.ndarray | None = None
     audits: dict[str, Any] = field(default_factory=dict)
 
 
@@ -381,3 +386,90 @@ def _order3_tables(plan: HostPlan, support, s2, s3, q, nu_full, inner_rank, oute
         "core_fft_ifft_count": int(outer_rank) * (12 + 16 * int(inner_rank) + 9 + 4) + 2,
         "working_grids": {"quadratic": list(plan.g2[::-1]), "cubic": list(plan.g3[::-1])},
     }
+
+
+def build_order4_plan(spectrum: np.ndarray, qx: np.ndarray, qy: np.ndarray, *, peak_q: float, rank: int,
+                      depth: float = 1.0, gravity: float = 1.0, precision: int = PREC_FP64) -> HostPlan:
+    """FP64 plan of the pure-sum GL eta44 graph (one GL rank for every nesting level).
+
+    Follows the setup of paper/order4/cpp/gl_eta44_fftw.cpp:76-155 and
+    src/internal/finite_depth_directional_green_laplace_eta44_three_kernels.m:33-56:
+    support, pair/triple/quartic supports, shifts c_n = (min nu + max sqrt(A)/n)/2,
+    scales lambda_n = n nu_p - sqrt(n q_p tanh(n q_p)).
+    """
+    if rank not in (4, 6, 8, 10, 12):
+        raise ValueError("Unsupported no-Stokes order-four Green--Laplace rank.")
+    spectrum = np.asarray(spectrum, dtype=np.complex128)
+    ny, nx = spectrum.shape
+    q = np.hypot(qx, qy)
+    mag = np.abs(spectrum)
+    support = mag > 1.0e-13 * max(1.0, float(np.max(mag)))
+    if not np.any(support) or np.any(qx[support] <= 0) or np.any(q[support] <= 0):
+        raise ValueError("Strict-forward eta11 support with every parent q>0 is required.")
+    mx, my = mode_grids(ny, nx)
+    if np.max(4 * np.abs(mx[support])) >= nx / 2 or np.max(4 * np.abs(my[support])) >= ny / 2:
+        raise ValueError("quartic output support reaches a Nyquist boundary")
+    nu_full = np.sqrt(q * np.tanh(q))
+    s2, s3, s4 = (sum_support(support, n) for n in (2, 3, 4))
+    plan = HostPlan(nx=nx, ny=ny, depth=float(depth), gravity=float(gravity), order=1, precision=int(precision),
+                    floor_level=16.0 * EPS * max(1.0, math.log2(nx * ny)), box1=box_of(support), peak_q=float(peak_q))
+    b1 = plan.box1
+    plan.box2, plan.box3, plan.box4 = box_of(s2), box_of(s3), box_of(s4)
+    plan.g2 = (fft_length_at_least(max(2 * b1.wx - 1, 16)), fft_length_at_least(max(2 * b1.wy - 1, 16)))
+    plan.g3 = (fft_length_at_least(max(3 * b1.wx - 2, 16)), fft_length_at_least(max(3 * b1.wy - 2, 16)))
+    plan.g4 = (fft_length_at_least(max(4 * b1.wx - 3, 16)), fft_length_at_least(max(4 * b1.wy - 3, 16)))
+    plan.t1 = _tables(b1, qx, qy)
+    plan.t2 = _tables(plan.box2, qx, qy)
+    plan.t3 = _tables(plan.box3, qx, qy)
+    plan.t4 = _tables(plan.box4, qx, qy)
+    plan.omega = math.sqrt(gravity / depth) * plan.t1.nu
+    plan.mask2 = plan.box2.take(s2).astype(np.uint8)
+    plan.mask3 = plan.box3.take(s3).astype(np.uint8)
+    plan.mask4 = plan.box4.take(s4).astype(np.uint8)
+    nu_min = float(np.min(nu_full[support]))
+    c2 = 0.5 * (nu_min + float(np.max(nu_full[s2])) / 2)
+    c3 = 0.5 * (nu_min + float(np.max(nu_full[s3])) / 3)
+    c4 = 0.5 * (nu_min + float(np.max(nu_full[s4])) / 4)
+    pn = math.sqrt(peak_q * math.tanh(peak_q))
+    lam = {n: n * pn - math.sqrt(n * peak_q * math.tanh(n * peak_q)) for n in (2, 3, 4)}
+    if not all(np.isfinite(v) and v > 0 for v in lam.values()):
+        raise ValueError("Frozen order-four Green--Laplace tail scale is invalid.")
+    x, w = gauss_laguerre_rule(rank)
+    nu1 = plan.t1.nu
+    d4 = np.stack([np.exp(-(xi / lam[4]) * (nu1 - c4)) for xi in x])
+    d3 = np.stack([np.exp(-(xi / lam[3]) * (nu1 - c3)) for xi in x])
+    d2 = np.stack([np.exp(-(xi / lam[2]) * (nu1 - c2)) for xi in x])
+    J = len(x)
+    dt = (d4[:, None] * d3[None, :]).reshape(J * J, *nu1.shape)
+    sa = {n: np.sqrt(t.a) for n, t in ((2, plan.t2), (3, plan.t3), (4, plan.t4))}
+
+    def factors(n):
+        taus = x / lam[n]
+        return (np.stack([np.cosh(sa[n] * t) for t in taus]), np.stack([_sinh_ratio(sa[n], t) for t in taus]))
+
+    pc, ps = factors(2)
+    tc, ts = factors(3)
+    qc, qs = factors(4)
+    cs = {2: c2, 3: c3, 4: c4}
+    coef = {n: np.array([wi * math.exp(xi) / lam[n] * math.exp(-n * cs[n] * (xi / lam[n])) for xi, wi in zip(x, w)])
+            for n in (2, 3, 4)}
+    plan.o4 = {
+        "rank": J,
+        "damp_quartic": d4,
+        "damp_third": dt,
+        "damp_pair3": (dt[:, None] * d2[None, :]).reshape(J * J * J, *nu1.shape),
+        "damp_pair4": (d4[:, None] * d2[None, :]).reshape(J * J, *nu1.shape),
+        "pair_cosh": pc, "pair_sinhr": ps, "pair_coef": coef[2] / 4.0,
+        "third_cosh": tc, "third_sinhr": ts, "third_coef": coef[3],
+        "quartic_cosh": qc, "quartic_sinhr": qs, "quartic_coef": coef[4],
+        "shifts": (c2, c3, c4), "scales": (lam[2], lam[3], lam[4]),
+    }
+    plan.audits["eta44"] = {
+        "implementation": "pure-sum-green-laplace-eta44-no-stokes",
+        "quadrature_rank": J,
+        "outer_scale": lam[4], "outer_balance_shift": c4,
+        "reference_fft_ifft_count": J * (J * (16 + J * 22 + 15 + 4) + 8 + 19 + J * 22 + 18 + 2) + 1,
+        "working_grids": {"quadratic": list(plan.g2[::-1]), "cubic": list(plan.g3[::-1]),
+                          "quartic": list(plan.g4[::-1])},
+    }
+    return plan

@@ -107,3 +107,13 @@ def order3_headline_512(seed: int = 2) -> SeaState:
 def headline_instants(tp: float = 12.0, count: int = 64) -> np.ndarray:
     """The 64 instants uniform over [0, 10 Tp]."""
     return np.linspace(0.0, 10.0 * tp, count)
+
+
+def order4_dimensionless_256(seed: int = 5) -> SeaState:
+    """Order-4 sea state: g = h = 1, kp h = 1.5, kp Hs / 2 = 0.12, 256^2 over 5 peak wavelengths
+    (dk = 0.3), JONSWAP gamma = 3.3, s = 10, |k| < 4 kp (quartic-alias safe)."""
+    kp = 1.5
+    tp = 2.0 * math.pi / math.sqrt(kp * math.tanh(kp))
+    length = 5.0 * 2.0 * math.pi / kp
+    return jonswap_sea_state(nx=256, ny=256, lx=length, ly=length, depth=1.0, hs=0.24 / kp, tp=tp, gamma=3.3,
+                             spread_s=10.0, gravity=1.0, seed=seed)

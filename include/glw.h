@@ -76,6 +76,23 @@ typedef struct glw_order2_desc {
   const double* psi_sign;       /* [2] sign of K/sqrt(A) per branch */
 } glw_order2_desc;
 
+/* Order-4 eta44 tables (optional; with_order4 != 0).  One GL rank J for every
+ * nesting level, as paper/order4/cpp/gl_eta44_fftw.cpp uses. */
+typedef struct glw_order4_desc {
+  int32_t rank;                 /* J */
+  glw_box p4;                   /* quartic box */
+  int32_t g4x, g4y;             /* working grid of quartic products */
+  glw_box_tables t4;            /* tables on p4 */
+  const uint8_t* mask4;         /* [p4] quartic support */
+  const double* damp_quartic;   /* [J][p1]     exp(-tau4 (nu - c4)) */
+  const double* damp_third;     /* [J*J][p1]   damp_quartic[o4] exp(-tau3[o3] (nu - c3)) */
+  const double* damp_pair3;     /* [J*J*J][p1] damp_third[o4,o3] exp(-tau2[i] (nu - c2)) */
+  const double* damp_pair4;     /* [J*J][p1]   damp_quartic[o4] exp(-tau2[i] (nu - c2)) */
+  const double *pair_cosh, *pair_sinhr, *pair_coef;          /* [J][p2], [J][p2], [J] */
+  const double *third_cosh, *third_sinhr, *third_coef;       /* [J][p3], [J][p3], [J] */
+  const double *quartic_cosh, *quartic_sinhr, *quartic_coef; /* [J][p4], [J][p4], [J] */
+} glw_order4_desc;
+
 typedef struct glw_plan_desc {
   int32_t nx, ny;               /* FFT grid */
   double depth, gravity;
@@ -101,6 +118,8 @@ typedef struct glw_plan_desc {
   const double* outer_sinhr;    /* [outer][p3] */
   const double* outer_coef;     /* [outer] w e^x / lambda3 * exp(-3 c3 tau) */
   glw_order2_desc o2;
+  int32_t with_order4;          /* build the eta44 graph (needs p2, p3, g2, g3 too) */
+  glw_order4_desc o4;
 } glw_plan_desc;
 
 int glw_plan_create(const glw_plan_desc* desc, glw_plan** plan);
@@ -118,6 +137,13 @@ int glw_order2_execute(glw_plan* plan, const double* s0, const double* times, in
 /* eta33+, psi33+ at nt instants (plan order 3). */
 int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt,
                        double* eta33, double* psi33, void* stream);
+
+/* eta44+ at nt instants (plan with_order4): the pure-sum GL eta44 of
+ * paper/order4/cpp/gl_eta44_fftw.cpp:639-752 and
+ * src/internal/finite_depth_directional_green_laplace_eta44_three_kernels.m:57-219
+ * (via gl_pure_sum_order4.m), scaled by depth. */
+int glw_order4_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta44,
+                       void* stream);
 
 /* WIT sector-44 quartet sum: replaces the per-quartet DirectionalKernel::evaluate
  * and the sector-44 loop of paper/order4/cpp/directional_eta4_openmp.cpp

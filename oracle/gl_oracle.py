@@ -592,3 +592,188 @@ def gl_spectral_surface(order, g, h, a, b, kx, ky, lx, ly, nx, ny, t, eta22_rank
     qx, qy = h * kxg, h * kyg
     return _assemble(spec, qx, qy, Geometry(qx, qy), h, g, order, eta22_rank, allow_below_model_parent_domain,
                      lx, ly, t, 0.0, peak_q)
+
+
+# --------------------------------------------------------------------------
+# order 4 (eta44, no Stokes corrections)
+# --------------------------------------------------------------------------
+def _suppress(x: np.ndarray, mask: np.ndarray) -> np.ndarray:
+    """Mask then roundoff floor (paper/order4/cpp/gl_eta44_fftw.cpp:157-171)."""
+    y = np.where(mask, x, 0.0)
+    top = float(np.max(np.abs(y[mask]), initial=0.0))
+    if top:
+        level = 16.0 * EPS * max(1.0, math.log2(x.size)) * top
+        y[mask & (np.abs(y) < level)] = 0.0
+    return y
+
+
+class _O4:
+    """Geometry, masks, shifts and scales of the order-4 graph (gl_eta44_fftw.cpp:76-155)."""
+
+    def __init__(self, spec, qx, qy, peak, nodes, weights):
+        self.g = Geometry(qx, qy)
+        self.nodes, self.weights = nodes, weights
+        sup = parent_support(spec)
+        g = self.g
+        if np.any(qx[sup] <= 0) or np.any(g.q[sup] <= 0):
+            raise ValueError("support")
+        self.m2, self.m3, self.m4 = (sum_support(sup, n) for n in (2, 3, 4))
+        mn = float(np.min(g.nu[sup]))
+        self.c2 = 0.5 * (mn + float(np.max(g.sa[self.m2])) / 2)
+        self.c3 = 0.5 * (mn + float(np.max(g.sa[self.m3])) / 3)
+        self.c4 = 0.5 * (mn + float(np.max(g.sa[self.m4])) / 4)
+        pn = math.sqrt(peak * math.tanh(peak))
+        self.l2 = 2 * pn - math.sqrt(2 * peak * math.tanh(2 * peak))
+        self.l3 = 3 * pn - math.sqrt(3 * peak * math.tanh(3 * peak))
+        self.l4 = 4 * pn - math.sqrt(4 * peak * math.tanh(4 * peak))
+
+    def factors(self, tau):
+        return sinh_ratio(self.g.sa, tau), np.cosh(self.g.sa * tau)
+
+
+def _first4(G: _O4, s, order4=False):
+    """First-order fields with time derivatives (gl_eta44_fftw.cpp:188-290)."""
+    g, ifft2 = G.g, np.fft.ifft2
+    q2 = g.q**2
+    f = {
+        "eta": ifft2(s), "ex": ifft2(1j * g.qx * s), "ey": ifft2(1j * g.qy * s),
+        "px": ifft2(g.qx / g.snu * s), "py": ifft2(g.qy / g.snu * s), "pz": ifft2(-1j * g.nu * s),
+        "pxz": ifft2(g.nu * g.qx * s), "pyz": ifft2(g.nu * g.qy * s), "pzz": ifft2(-1j * q2 / g.snu * s),
+        "pzzz": ifft2(-1j * q2 * g.nu * s), "ptz": ifft2(-g.a * s), "ptzz": ifft2(-q2 * s),
+        "pxz_t": ifft2(-1j * g.nu**2 * g.qx * s), "pyz_t": ifft2(-1j * g.nu**2 * g.qy * s),
+        "pzzz_t": ifft2(-q2 * g.a * s), "ptz_t": ifft2(1j * g.a * g.nu * s),
+    }
+    if order4:
+        phi = np.zeros_like(s)
+        nz = g.nu > 0
+        phi[nz] = -1j * s[nz] / g.nu[nz]
+        f["pxzz"] = ifft2(1j * g.qx * q2 * phi)
+        f["pyzz"] = ifft2(1j * g.qy * q2 * phi)
+        f["pzzzz"] = ifft2(q2 * q2 * phi)
+    return f
+
+
+def _pair4(G: _O4, s, order4=False):
+    """Nested pair state with first and second time derivatives (gl_eta44_fftw.cpp:299-529)."""
+    g, ifft2, fft2 = G.g, np.fft.ifft2, np.fft.fft2
+    q2 = g.q**2
+    es, ets, ps, pts, ptts = (np.zeros_like(s) for _ in range(5))
+    for x, w in zip(G.nodes, G.weights):
+        t = x / G.l2
+        d = s * np.exp(-t * (g.nu - G.c2))
+        v, vn, vn2, vn3, vn4 = (ifft2(d * g.nu**p) for p in range(5))
+        hx, hy = ifft2(d * g.qx / g.snu), ifft2(d * g.qy / g.snu)
+        jx, jy = ifft2(d * g.qx), ifft2(d * g.qy)
+        ro, q2f, q2n = ifft2(d * q2 / g.snu), ifft2(d * q2), ifft2(d * q2 * g.nu)
+        njx, njy = ifft2(d * g.nu * g.qx), ifft2(d * g.nu * g.qy)
+        n2jx, n2jy = ifft2(d * g.nu**2 * g.qx), ifft2(d * g.nu**2 * g.qy)
+        sd = 2 * v * vn2 + vn * vn - hx * hx - hy * hy
+        sk = 2 * v * ro + 2 * (hx * jx + hy * jy)
+        sds = 2 * v * vn3 + 4 * vn * vn2 - 2 * hx * jx - 2 * hy * jy
+        sks = 2 * (vn * ro + v * q2f) + 2 * (jx * jx + hx * njx + jy * jy + hy * njy)
+        sdss = 2 * v * vn4 + 6 * vn * vn3 + 4 * vn2 * vn2 - 2 * (jx * jx + hx * njx + jy * jy + hy * njy)
+        skss = 2 * (vn2 * ro + 2 * vn * q2f + v * q2n) + 6 * (jx * njx + jy * njy) + 2 * (hx * n2jx + hy * n2jy)
+        sd, sk, sds, sks, sdss, skss = (_suppress(fft2(a), G.m2) for a in (sd, sk, sds, sks, sdss, skss))
+        co = w * math.exp(x) / G.l2 * math.exp(-2 * G.c2 * t) / 4
+        sh, ch = G.factors(t)
+        es += co * (-g.a * sh * sd + ch * sk)
+        ets -= 1j * co * (-g.a * sh * sds + ch * sks)
+        ps += 1j * co * (ch * sd - sh * sk)
+        pts += co * (ch * sds - sh * sks)
+        ptts -= 1j * co * (ch * sdss - sh * skss)
+    es, ets, ps, pts, ptts = (_suppress(a, G.m2) for a in (es, ets, ps, pts, ptts))
+    p = {
+        "eta": ifft2(es), "ex": ifft2(1j * g.qx * es), "ey": ifft2(1j * g.qy * es),
+        "eta_t": ifft2(ets), "ex_t": ifft2(1j * g.qx * ets), "ey_t": ifft2(1j * g.qy * ets),
+        "px": ifft2(1j * g.qx * ps), "py": ifft2(1j * g.qy * ps), "pz": ifft2(g.a * ps), "pzz": ifft2(q2 * ps),
+        "ptz": ifft2(g.a * pts), "px_t": ifft2(1j * g.qx * pts), "py_t": ifft2(1j * g.qy * pts),
+        "pzz_t": ifft2(q2 * pts), "ptz_t": ifft2(g.a * ptts),
+    }
+    if order4:
+        p["pxz"] = ifft2(1j * g.qx * g.a * ps)
+        p["pyz"] = ifft2(1j * g.qy * g.a * ps)
+        p["pzzz"] = ifft2(q2 * g.a * ps)
+    return p
+
+
+def _third4(G: _O4, s):
+    """Nested third-order state (gl_eta44_fftw.cpp:537-637)."""
+    g, ifft2, fft2 = G.g, np.fft.ifft2, np.fft.fft2
+    q2 = g.q**2
+    es, ets, ps, pts = (np.zeros_like(s) for _ in range(4))
+    for x, w in zip(G.nodes, G.weights):
+        t = x / G.l3
+        d = s * np.exp(-t * (g.nu - G.c3))
+        f, p = _first4(G, d), _pair4(G, d)
+        kp = f["px"] * p["ex"] + f["py"] * p["ey"] + p["px"] * f["ex"] + p["py"] * f["ey"] - f["eta"] * p["pzz"] \
+            - p["eta"] * f["pzz"]
+        kd = f["eta"] * (f["pxz"] * f["ex"] + f["pyz"] * f["ey"]) - 0.5 * f["eta"] ** 2 * f["pzzz"]
+        dp = f["eta"] * p["ptz"] + p["eta"] * f["ptz"] + f["px"] * p["px"] + f["py"] * p["py"] + f["pz"] * p["pz"]
+        dd = 0.5 * f["eta"] ** 2 * f["ptzz"] + f["eta"] * (f["px"] * f["pxz"] + f["py"] * f["pyz"] + f["pz"] * f["pzz"])
+        kpt = (-f["ex"] * p["ex"] + f["px"] * p["ex_t"] - f["ey"] * p["ey"] + f["py"] * p["ey_t"]
+               + p["px_t"] * f["ex"] + p["px"] * f["pxz"] + p["py_t"] * f["ey"] + p["py"] * f["pyz"]
+               - f["pz"] * p["pzz"] - f["eta"] * p["pzz_t"] - p["eta_t"] * f["pzz"] - p["eta"] * f["ptzz"])
+        kdt = (f["pz"] * (f["pxz"] * f["ex"] + f["pyz"] * f["ey"])
+               + f["eta"] * (f["pxz_t"] * f["ex"] + f["pxz"] * f["pxz"] + f["pyz_t"] * f["ey"] + f["pyz"] * f["pyz"])
+               - f["eta"] * f["pz"] * f["pzzz"] - 0.5 * f["eta"] ** 2 * f["pzzz_t"])
+        dpt = (f["pz"] * p["ptz"] + f["eta"] * p["ptz_t"] + p["eta_t"] * f["ptz"] + p["eta"] * f["ptz_t"]
+               - f["ex"] * p["px"] + f["px"] * p["px_t"] - f["ey"] * p["py"] + f["py"] * p["py_t"]
+               + f["ptz"] * p["pz"] + f["pz"] * p["ptz"])
+        ddt = (f["eta"] * f["pz"] * f["ptzz"] - 0.5 * f["eta"] ** 2 * f["pzzz"]
+               + f["pz"] * (f["px"] * f["pxz"] + f["py"] * f["pyz"] + f["pz"] * f["pzz"])
+               + f["eta"] * (-f["ex"] * f["pxz"] + f["px"] * f["pxz_t"] - f["ey"] * f["pyz"] + f["py"] * f["pyz_t"]
+                             + f["ptz"] * f["pzz"] + f["pz"] * f["ptzz"]))
+        fk, fd, fkt, fdt = (_suppress(fft2(a), G.m3) for a in (kd / 4 + kp / 2, dd / 4 + dp / 2,
+                                                               kdt / 4 + kpt / 2, ddt / 4 + dpt / 2))
+        co = w * math.exp(x) / G.l3 * math.exp(-3 * G.c3 * t)
+        sh, ch = G.factors(t)
+        es += co * (g.a * sh * fd - 1j * ch * fk)
+        ets += co * (g.a * sh * fdt - 1j * ch * fkt)
+        ps += co * (-1j * ch * fd - sh * fk)
+        pts += co * (-1j * ch * fdt - sh * fkt)
+    es, ps, pts = (_suppress(a, G.m3) for a in (es, ps, pts))
+    return {"eta": ifft2(es), "ex": ifft2(1j * g.qx * es), "ey": ifft2(1j * g.qy * es),
+            "px": ifft2(1j * g.qx * ps), "py": ifft2(1j * g.qy * ps), "pz": ifft2(g.a * ps),
+            "pzz": ifft2(q2 * ps), "pzt": ifft2(g.a * pts)}
+
+
+def eta44(spec: np.ndarray, qx: np.ndarray, qy: np.ndarray, peak_q: float, rank: int):
+    """Pure-sum GL eta44+ (dimensionless), no Stokes corrections.
+
+    Restates paper/order4/cpp/gl_eta44_fftw.cpp:639-752 (itself a restatement
+    of src/internal/finite_depth_directional_green_laplace_eta44_three_kernels.m:57-219
+    with finite_depth_directional_nested_green_laplace_eta33.m); one GL rank for
+    every nesting level.
+    """
+    nodes, weights = gauss_laguerre_rule(rank)
+    G = _O4(spec, qx, qy, peak_q, nodes, weights)
+    g, fft2 = G.g, np.fft.fft2
+    out = np.zeros_like(spec)
+    for x, w in zip(nodes, weights):
+        tm = x / G.l4
+        d = spec * np.exp(-tm * (g.nu - G.c4))
+        t = _third4(G, d)
+        f = _first4(G, d, True)
+        p = _pair4(G, d, True)
+        k31 = f["px"] * t["ex"] + f["py"] * t["ey"] + t["px"] * f["ex"] + t["py"] * f["ey"] - f["eta"] * t["pzz"] \
+            - t["eta"] * f["pzz"]
+        d31 = f["eta"] * t["pzt"] + t["eta"] * f["ptz"] + f["px"] * t["px"] + f["py"] * t["py"] + f["pz"] * t["pz"]
+        k22 = p["px"] * p["ex"] + p["py"] * p["ey"] - p["eta"] * p["pzz"]
+        d22 = p["eta"] * p["ptz"] + 0.5 * (p["px"] ** 2 + p["py"] ** 2 + p["pz"] ** 2)
+        fe = f["eta"]
+        k211 = (fe * (f["pxz"] * p["ex"] + f["pyz"] * p["ey"]) + (fe * p["pxz"] + p["eta"] * f["pxz"]) * f["ex"]
+                + (fe * p["pyz"] + p["eta"] * f["pyz"]) * f["ey"] - 0.5 * fe * fe * p["pzzz"]
+                - fe * p["eta"] * f["pzzz"])
+        d211 = (0.5 * fe * fe * p["pzz_t"] + fe * p["eta"] * f["ptzz"] + f["px"] * (fe * p["pxz"] + p["eta"] * f["pxz"])
+                + f["py"] * (fe * p["pyz"] + p["eta"] * f["pyz"]) + f["pz"] * (fe * p["pzz"] + p["eta"] * f["pzz"])
+                + p["px"] * (fe * f["pxz"]) + p["py"] * (fe * f["pyz"]) + p["pz"] * (fe * f["pzz"]))
+        k1111 = 0.5 * fe * fe * (f["pxzz"] * f["ex"] + f["pyzz"] * f["ey"]) - fe**3 * f["pzzzz"] / 6
+        d1111 = (fe**3 * f["pzzz_t"] / 6 + 0.5 * fe * fe * (f["px"] * f["pxzz"] + f["py"] * f["pyzz"] + f["pz"] * f["pzzz"])
+                 + 0.5 * fe * fe * (f["pxz"] ** 2 + f["pyz"] ** 2 + f["pzz"] ** 2))
+        fk = _suppress(fft2(k31 / 2 + k1111 / 8 + k211 / 4 + k22 / 2), G.m4)
+        fd = _suppress(fft2(d31 / 2 + d1111 / 8 + d211 / 4 + d22 / 2), G.m4)
+        co = w * math.exp(x) / G.l4 * math.exp(-4 * G.c4 * tm)
+        sh, ch = G.factors(tm)
+        out += co * (g.a * sh * fd - 1j * ch * fk)
+    out = _suppress(out, G.m4)
+    return np.fft.ifft2(out)

@@ -1,0 +1,45 @@
+"""GPU eta44 (pure-sum GL, no Stokes) vs the reference's recorded GL-vs-WIT metrics and the oracle.
+
+The n0382 check reproduces paper/order4/validation/recomputed_field_metrics.csv
+(GL6/8/10 against the reference WIT field) to 1e-9 relative; the small case
+compares the field with the FP64 oracle at 1e-10 (conditioning: the GL node
+factors amplify roundoff as in order 3).
+"""
+
+import numpy as np
+import pytest
+
+from cases import rel
+from oracle import gl_oracle as O
+from order4_cases import N0382_METRICS, metrics, n0382, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def glw():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    import green_laplace_wave_reconstruction_amd as G
+
+    return G
+
+
+@pytest.mark.parametrize("rank", [6, 8, 10])
+def test_eta44_n0382_reproduces_reference_metrics(glw, rank):
+    s, qx, qy, h, ref = n0382()
+    print(f"eta44 n0382 GL{rank}", flush=True)
+    eta, psi, audit = glw.gl_pure_sum_order4(s, qx, qy, 1.0, rank)
+    l2, linf = metrics(h * np.real(eta), ref)
+    assert abs(l2 - N0382_METRICS[rank][0]) < 1e-9 * N0382_METRICS[rank][0]
+    assert abs(linf - N0382_METRICS[rank][1]) < 1e-8 * N0382_METRICS[rank][1]
+    assert psi is None and audit["quadrature_rank"] == rank
+
+
+@pytest.mark.parametrize("rank", [4, 6])
+def test_eta44_small_case_vs_oracle(glw, rank):
+    s, qx, qy = small_case()
+    eta, _, _ = glw.gl_pure_sum_order4(s, qx, qy, 1.0, rank)
+    assert rel(eta, O.eta44(s, qx, qy, 1.0, rank)) < 1e-10

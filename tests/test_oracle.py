@@ -154,3 +154,24 @@ def test_homogeneity(order, power):
     keys = ("eta22", "psi22") if order == 2 else ("eta33", "psi33")
     for k in keys:
         assert rel(b[k], 2.0**power * a[k]) < 1e-14, k
+
+
+def test_eta44_gl6_reproduces_reference_gl_wit_metric():
+    """Oracle eta44 (GL6) on the reference's n0382 input vs the reference WIT field:
+    rel-L2 and rel-Linf equal the recorded metrics to 1e-9 (the reference's own
+    C++-vs-archived-MATLAB parity is 1.8e-10, paper/order4/validation/cpp_paper_field.json)."""
+    from order4_cases import N0382_METRICS, metrics, n0382
+
+    s, qx, qy, h, ref = n0382()
+    l2, linf = metrics(h * np.real(O.eta44(s, qx, qy, 1.0, 6)), ref)
+    assert abs(l2 - N0382_METRICS[6][0]) < 1e-9 * N0382_METRICS[6][0]
+    assert abs(linf - N0382_METRICS[6][1]) < 1e-8 * N0382_METRICS[6][1]
+
+
+def test_eta44_homogeneity_degree_four():
+    from order4_cases import small_case
+
+    s, qx, qy = small_case()
+    a = O.eta44(s * 4.0, qx, qy, 1.0, 4)
+    b = O.eta44(s * 8.0, qx, qy, 1.0, 4)
+    assert rel(b, 16.0 * a) < 1e-14
