@@ -35,6 +35,20 @@ __device__ __forceinline__ cpx<T> cmul_i(cpx<T> a) {
   else return {a.y, -a.x};
 }
 
+// LDS layout of a batch of sequences: element e of sequence s lives at
+// s * seq_stride<L>() + e + e / 16.  One pad slot every 16 elements breaks the
+// power-of-two strides of the Stockham writes, and the odd sequence stride
+// keeps lanes that walk across sequences (column tiles) on distinct banks.
+__host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
+template <int L>
+__host__ __device__ constexpr int seq_stride() {
+  return (L + (L >> 4)) | 1;
+}
+template <int L>
+__device__ __forceinline__ int lds_at(int s, int e) {
+  return s * seq_stride<L>() + lds_pad(e);
+}
+
 // Greedy radix decomposition: 8s, then a 4, then a 2, then 3s.
 __host__ __device__ constexpr int radix_of(int L, int s) {
   int rem = L;
@@ -133,9 +147,9 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* __restrict
       if (TOTAL % NT == 0 || b < TOTAL) {
         const int seq = b / NB;
         const int j = b - seq * NB;
-        const cpx<T>* src = buf + seq * SSTR + j;
+        const cpx<T>* src = buf + seq * SSTR;
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[p][r] = src[r * NB];
+        for (int r = 0; r < R; ++r) v[p][r] = src[lds_pad(j + r * NB)];
         if constexpr (NS > 1) {
           const int k = j % NS;
 #pragma unroll
@@ -156,9 +170,10 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* __restrict
         const int j = b - seq * NB;
         const int k = j % NS;
         dft_reg<R, SIGN>(v[p]);
-        cpx<T>* dst = buf + seq * SSTR + (j / NS) * NS * R + k;
+        cpx<T>* dst = buf + seq * SSTR;
+        const int base = (j / NS) * NS * R + k;
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[r * NS] = v[p][r];
+        for (int r = 0; r < R; ++r) dst[lds_pad(base + r * NS)] = v[p][r];
       }
     }
     __syncthreads();
@@ -166,8 +181,9 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* __restrict
   }
 }
 
-// Transform NSEQ sequences in LDS.  Callers must __syncthreads() before the
-// call (data staged) and may read results right after it returns.
+// Transform NSEQ sequences in LDS, laid out as lds_at<L>(s, e) with
+// SSTR = seq_stride<L>().  Callers must __syncthreads() before the call (data
+// staged) and may read results right after it returns.
 template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN>
 __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* __restrict__ tw) {
   static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");

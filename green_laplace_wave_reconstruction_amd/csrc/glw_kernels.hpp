@@ -146,7 +146,7 @@ struct ColInvArgs {
 
 template <typename T, int GY, int CW>
 __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
-  __shared__ cpx<T> buf[CW * GY];
+  __shared__ cpx<T> buf[CW * seq_stride<GY>()];
   const int col0 = blockIdx.x * CW;
   const int f = blockIdx.y;
   const int task = blockIdx.z;
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __res
     const size_t k = (size_t)set * a.fmax_stride + (a.fmax_stride == 1 ? 0 : src);
     level = a.floor_level * __longlong_as_double((long long)a.fmax[k]);
   }
-  for (int i = threadIdx.x; i < CW * GY; i += NT) buf[i] = {T(0), T(0)};
+  for (int i = threadIdx.x; i < CW * seq_stride<GY>(); i += NT) buf[i] = {T(0), T(0)};
   __syncthreads();
   for (int i = threadIdx.x; i < CW * a.wy; i += NT) {
     const int r = i / CW;
@@ -174,17 +174,17 @@ __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __res
       if (level >= 0.0 && hypot(s.x, s.y) < level) s = {0.0, 0.0};
       if (damp) s = {s.x * damp[b], s.y * damp[b]};
       s = apply_mul(code, s, a.tab, b);
-      buf[c * GY + wrap(a.my0 + r, GY)] = {T(s.x * scale), T(s.y * scale)};
+      buf[lds_at<GY>(c, wrap(a.my0 + r, GY))] = {T(s.x * scale), T(s.y * scale)};
     }
   }
   __syncthreads();
-  fft_lds<T, GY, CW, GY, NT, +1>(buf, tw);
+  fft_lds<T, GY, CW, seq_stride<GY>(), NT, +1>(buf, tw);
   cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) +
                 ((size_t)task * a.out_nfield + a.out_foff + f) * GY * a.ostride;
   for (int i = threadIdx.x; i < CW * GY; i += NT) {
     const int y = i / CW;
     const int c = i - y * CW;
-    if (col0 + c < a.wx) out[(size_t)y * a.ostride + col0 + c] = buf[c * GY + y];
+    if (col0 + c < a.wx) out[(size_t)y * a.ostride + col0 + c] = buf[lds_at<GY>(c, y)];
   }
 }
 
@@ -217,7 +217,7 @@ __device__ __forceinline__ void block_max_atomic(double v, uint64_t* dst) {
 
 template <typename T, int GY, int CW>
 __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
-  __shared__ cpx<T> buf[CW * GY];
+  __shared__ cpx<T> buf[CW * seq_stride<GY>()];
   const int col0 = blockIdx.x * CW;
   const int s = blockIdx.y;
   const int task = blockIdx.z;
@@ -225,10 +225,10 @@ __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __res
   for (int i = threadIdx.x; i < CW * GY; i += NT) {
     const int y = i / CW;
     const int c = i - y * CW;
-    buf[c * GY + y] = (col0 + c < a.owx) ? in[(size_t)y * a.istride + col0 + c] : cpx<T>{T(0), T(0)};
+    buf[lds_at<GY>(c, y)] = (col0 + c < a.owx) ? in[(size_t)y * a.istride + col0 + c] : cpx<T>{T(0), T(0)};
   }
   __syncthreads();
-  fft_lds<T, GY, CW, GY, NT, -1>(buf, tw);
+  fft_lds<T, GY, CW, seq_stride<GY>(), NT, -1>(buf, tw);
   cpx<double>* out = a.out + ((size_t)task * a.nsrc + s) * a.owy * a.owx;
   double vmax = 0.0;
   for (int i = threadIdx.x; i < CW * a.owy; i += NT) {
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __res
     const int col = col0 + c;
     if (col < a.owx) {
       const int b = r * a.owx + col;
-      cpx<T> v = buf[c * GY + wrap(a.omy0 + r, GY)];
+      cpx<T> v = buf[lds_at<GY>(c, wrap(a.omy0 + r, GY))];
       cpx<double> d = {double(v.x) * a.scale, double(v.y) * a.scale};
       if (!a.mask[b]) d = {0.0, 0.0};
       else vmax = fmax(vmax, hypot(d.x, d.y));
@@ -264,11 +264,12 @@ struct RowArgs {
 
 template <typename T, int GX, int NIN, int NOUT, int RY, class Op>
 __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict__ tw) {
-  __shared__ cpx<T> buf[NIN * RY * GX];
+  constexpr int SS = seq_stride<GX>();
+  __shared__ cpx<T> buf[NIN * RY * SS];
   const int y0 = blockIdx.x * RY;
   const int task = blockIdx.y;
   const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)task * a.nin * a.gy * a.istride;
-  for (int i = threadIdx.x; i < NIN * RY * GX; i += NT) buf[i] = {T(0), T(0)};
+  for (int i = threadIdx.x; i < NIN * RY * SS; i += NT) buf[i] = {T(0), T(0)};
   __syncthreads();
 #pragma unroll
   for (int f = 0; f < NIN; ++f) {
@@ -276,30 +277,30 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
     for (int i = threadIdx.x; i < RY * wx; i += NT) {
       const int r = i / wx;
       const int j = i - r * wx;
-      buf[(f * RY + r) * GX + wrap(mx0 + j, GX)] = in[((size_t)f * a.gy + y0 + r) * a.istride + j];
+      buf[lds_at<GX>(f * RY + r, wrap(mx0 + j, GX))] = in[((size_t)f * a.gy + y0 + r) * a.istride + j];
     }
   }
   __syncthreads();
-  fft_lds<T, GX, NIN * RY, GX, NT, +1>(buf, tw);
+  fft_lds<T, GX, NIN * RY, SS, NT, +1>(buf, tw);
   for (int i = threadIdx.x; i < RY * GX; i += NT) {
     const int r = i / GX;
     const int x = i - r * GX;
     cpx<T> v[NIN];
 #pragma unroll
-    for (int f = 0; f < NIN; ++f) v[f] = buf[(f * RY + r) * GX + x];
+    for (int f = 0; f < NIN; ++f) v[f] = buf[lds_at<GX>(f * RY + r, x)];
     cpx<T> o[NOUT];
     Op::apply(v, o);
 #pragma unroll
-    for (int s = 0; s < NOUT; ++s) buf[(s * RY + r) * GX + x] = o[s];
+    for (int s = 0; s < NOUT; ++s) buf[lds_at<GX>(s * RY + r, x)] = o[s];
   }
   __syncthreads();
-  fft_lds<T, GX, NOUT * RY, GX, NT, -1>(buf, tw);
+  fft_lds<T, GX, NOUT * RY, SS, NT, -1>(buf, tw);
   cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) + (size_t)task * a.nout * a.gy * a.ostride;
   for (int i = threadIdx.x; i < NOUT * RY * a.owx; i += NT) {
     const int sr = i / a.owx;
     const int j = i - sr * a.owx;
     const int s = sr / RY, r = sr - (sr / RY) * RY;
-    out[((size_t)s * a.gy + y0 + r) * a.ostride + j] = buf[(s * RY + r) * GX + wrap(a.omx0 + j, GX)];
+    out[((size_t)s * a.gy + y0 + r) * a.ostride + j] = buf[lds_at<GX>(s * RY + r, wrap(a.omx0 + j, GX))];
   }
 }
 
@@ -315,24 +316,25 @@ struct SynthArgs {
 
 template <typename T, int GX, int RY>
 __global__ __launch_bounds__(NT) void k_synth(SynthArgs a, const cpx<T>* __restrict__ tw) {
-  __shared__ cpx<T> buf[RY * GX];
+  constexpr int SS = seq_stride<GX>();
+  __shared__ cpx<T> buf[RY * SS];
   const int y0 = blockIdx.x * RY;
   const int seq = blockIdx.y;
   const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)seq * a.in_seq;
-  for (int i = threadIdx.x; i < RY * GX; i += NT) buf[i] = {T(0), T(0)};
+  for (int i = threadIdx.x; i < RY * SS; i += NT) buf[i] = {T(0), T(0)};
   __syncthreads();
   for (int i = threadIdx.x; i < RY * a.wx; i += NT) {
     const int r = i / a.wx;
     const int j = i - r * a.wx;
-    buf[r * GX + wrap(a.mx0 + j, GX)] = in[(size_t)(y0 + r) * a.istride + j];
+    buf[lds_at<GX>(r, wrap(a.mx0 + j, GX))] = in[(size_t)(y0 + r) * a.istride + j];
   }
   __syncthreads();
-  fft_lds<T, GX, RY, GX, NT, +1>(buf, tw);
+  fft_lds<T, GX, RY, SS, NT, +1>(buf, tw);
   cpx<double>* out = a.out + (size_t)seq * a.gy * GX;
   for (int i = threadIdx.x; i < RY * GX; i += NT) {
     const int r = i / GX;
     const int x = i - r * GX;
-    cpx<T> v = buf[r * GX + x];
+    cpx<T> v = buf[lds_at<GX>(r, x)];
     out[(size_t)(y0 + r) * GX + x] = {double(v.x), double(v.y)};
   }
 }

@@ -389,7 +389,7 @@ constexpr int kLdsMax = 160 * 1024;    // gfx950 per-workgroup LDS
 
 template <typename T, int L>
 constexpr int col_width() {
-  int cw = kLdsBudget / (L * (int)sizeof(cpx<T>));
+  int cw = kLdsBudget / (seq_stride<L>() * (int)sizeof(cpx<T>));
   int p = 1;
   while (p * 2 <= cw && p < 32) p *= 2;
   return p;
@@ -447,7 +447,7 @@ int launch_colfwd(int gy, ColFwdArgs a, int ntask, const void* tw, hipStream_t s
 // Rows per workgroup: as many as fit a 64 KB tile (at most 4), at least 1.
 template <typename T, int NIN, int L>
 constexpr int row_ry() {
-  int per = NIN * L * (int)sizeof(cpx<T>);
+  int per = NIN * seq_stride<L>() * (int)sizeof(cpx<T>);
   int ry = (64 * 1024) / per;
   if (ry >= 4) return 4;
   if (ry >= 2) return 2;
@@ -460,7 +460,7 @@ int launch_row(int gx, RowArgs a, int ntask, const void* tw, hipStream_t st) {
 #define GLW_CASE(L)                                                                         \
   case L: {                                                                                 \
     constexpr int RY = row_ry<T, NIN, L>();                                                 \
-    if constexpr (NIN * RY * L * sizeof(cpx<T>) <= kLdsMax) {                               \
+    if constexpr (NIN * RY * seq_stride<L>() * sizeof(cpx<T>) <= kLdsMax) {                 \
       if (a.gy % RY) return fail(GLW_EINVAL, "row count not divisible by the row tile");    \
       hipLaunchKernelGGL((k_row<T, L, NIN, NOUT, RY, Op>), dim3(a.gy / RY, ntask), dim3(NT), \
                          0, st, a, (const cpx<T>*)tw);                                      \
