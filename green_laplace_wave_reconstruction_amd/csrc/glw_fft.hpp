@@ -49,13 +49,34 @@ __device__ __forceinline__ int lds_at(int s, int e) {
   return s * seq_stride<L>() + lds_pad(e);
 }
 
-// Greedy radix decomposition: 8s, then a 4, then a 2, then 3s.
+// Stage radices: two passes where an in-register radix <= 18 allows it
+// (fewer LDS round trips and barriers), else greedy 16/8/4/2/3 factors.
 __host__ __device__ constexpr int radix_of(int L, int s) {
+  switch (L) {  // explicit plans for the lengths the kernels use most
+    case 16: return s == 0 ? 16 : 0;
+    case 24: return s == 0 ? 6 : (s == 1 ? 4 : 0);
+    case 32: return s == 0 ? 8 : (s == 1 ? 4 : 0);
+    case 48: return s == 0 ? 12 : (s == 1 ? 4 : 0);
+    case 64: return s == 0 ? 8 : (s == 1 ? 8 : 0);
+    case 96: return s == 0 ? 12 : (s == 1 ? 8 : 0);
+    case 128: return s == 0 ? 16 : (s == 1 ? 8 : 0);
+    case 144: return s == 0 ? 12 : (s == 1 ? 12 : 0);
+    case 192: return s == 0 ? 16 : (s == 1 ? 12 : 0);
+    case 256: return s == 0 ? 16 : (s == 1 ? 16 : 0);
+    case 288: return s == 0 ? 18 : (s == 1 ? 16 : 0);
+    case 384: return s == 0 ? 16 : (s == 1 ? 12 : (s == 2 ? 2 : 0));
+    case 512: return s == 0 ? 8 : (s == 1 ? 8 : (s == 2 ? 8 : 0));
+    case 576: return s == 0 ? 16 : (s == 1 ? 12 : (s == 2 ? 3 : 0));
+    case 768: return s == 0 ? 16 : (s == 1 ? 16 : (s == 2 ? 3 : 0));
+    case 1024: return s == 0 ? 16 : (s == 1 ? 8 : (s == 2 ? 8 : 0));
+    default: break;
+  }
   int rem = L;
   for (int i = 0; i <= s; ++i) {
     int r = 0;
     if (rem <= 1) return 0;
-    if (rem % 8 == 0) r = 8;
+    if (rem % 16 == 0) r = 16;
+    else if (rem % 8 == 0) r = 8;
     else if (rem % 4 == 0) r = 4;
     else if (rem % 2 == 0) r = 2;
     else if (rem % 3 == 0) r = 3;
@@ -66,6 +87,57 @@ __host__ __device__ constexpr int radix_of(int L, int s) {
   return 0;
 }
 
+// Compile-time cos/sin of 2 pi m / n (Taylor series after reduction to
+// |x| <= pi/4), accurate to a few ulp of double.
+__host__ __device__ constexpr long double taylor_sin(long double x) {
+  long double term = x, sum = x;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2 * k) * (2 * k + 1));
+    sum += term;
+  }
+  return sum;
+}
+__host__ __device__ constexpr long double taylor_cos(long double x) {
+  long double term = 1, sum = 1;
+  for (int k = 1; k < 14; ++k) {
+    term *= -x * x / ((2 * k - 1) * (2 * k));
+    sum += term;
+  }
+  return sum;
+}
+// returns cos (which=0) or sin (which=1) of 2 pi m / n
+__host__ __device__ constexpr double unit_root(int m, int n, int which) {
+  const long double pi4 = 0.785398163397448309615660845819875721L;
+  m = ((m % n) + n) % n;
+  // angle = 2 pi m / n = (8 m / n) * pi/4 ; octant o and remainder r in [0, 1)
+  const long double t = 8.0L * m / n;
+  int o = (int)t;
+  long double r = t - o;
+  if (r > 0.5L) {
+    r -= 1.0L;
+    o += 1;
+  }
+  const long double c = taylor_cos(r * pi4), s = taylor_sin(r * pi4);
+  // rotate (c, s) by o * pi/4
+  long double cc = c, ss = s;
+  const long double h = 0.707106781186547524400844362104849039L;
+  switch (o & 7) {
+    case 0: cc = c; ss = s; break;
+    case 1: cc = h * (c - s); ss = h * (c + s); break;
+    case 2: cc = -s; ss = c; break;
+    case 3: cc = -h * (c + s); ss = h * (c - s); break;
+    case 4: cc = -c; ss = -s; break;
+    case 5: cc = h * (s - c); ss = -h * (c + s); break;
+    case 6: cc = s; ss = -c; break;
+    default: cc = h * (c + s); ss = h * (s - c); break;
+  }
+  return (double)(which ? ss : cc);
+}
+
+__host__ __device__ constexpr int small_factor(int R) {
+  return R % 4 == 0 ? 4 : (R % 3 == 0 ? 3 : 2);
+}
+
 __host__ __device__ constexpr bool fft_length_ok(int L) {
   int rem = L;
   while (rem % 2 == 0) rem /= 2;
@@ -73,10 +145,47 @@ __host__ __device__ constexpr bool fft_length_ok(int L) {
   return rem == 1 && L >= 8;
 }
 
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft_reg(cpx<T>* v);
+
+// Composite in-register DFT (Cooley-Tukey, R = R1 R2, n = R2 n1 + n2,
+// k = k1 + R1 k2) with compile-time twiddles exp(SIGN 2 pi i n2 k1 / R).
+template <int R, int SIGN, typename T>
+__device__ __forceinline__ void dft_composite(cpx<T>* v) {
+  constexpr int R1 = small_factor(R), R2 = R / R1;
+  cpx<T> y[R2][R1];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    cpx<T> a[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[R2 * n1 + n2];
+    dft_reg<R1, SIGN>(a);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) {
+      if (n2 * k1 == 0) {
+        y[n2][k1] = a[k1];
+      } else {
+        const T c = T(unit_root(n2 * k1, R, 0)), sn = T(SIGN * unit_root(n2 * k1, R, 1));
+        y[n2][k1] = {a[k1].x * c - a[k1].y * sn, a[k1].x * sn + a[k1].y * c};
+      }
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cpx<T> b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2][k1];
+    dft_reg<R2, SIGN>(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+  }
+}
+
 // In-register R-point DFT with exponent sign SIGN (forward = -1).
 template <int R, int SIGN, typename T>
 __device__ __forceinline__ void dft_reg(cpx<T>* v) {
-  if constexpr (R == 2) {
+  if constexpr (R == 1) {
+  } else if constexpr (R == 2) {
     cpx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -126,13 +235,15 @@ __device__ __forceinline__ void dft_reg(cpx<T>* v) {
     v[5] = csub(o0, o2);
     v[3] = cadd(o1, o3);
     v[7] = csub(o1, o3);
+  } else {
+    dft_composite<R, SIGN>(v);
   }
 }
 
 // One Stockham stage (radix R, NS = product of earlier radices), then the
 // remaining stages by compile-time recursion.
 template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS>
-__device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* __restrict__ tw) {
+__device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw) {
   constexpr int R = radix_of(L, S);
   if constexpr (R > 0) {
     constexpr int NB = L / R;           // butterflies per sequence
@@ -182,10 +293,16 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* __restrict
 }
 
 // Transform NSEQ sequences in LDS, laid out as lds_at<L>(s, e) with
-// SSTR = seq_stride<L>().  Callers must __syncthreads() before the call (data
-// staged) and may read results right after it returns.
+// SSTR = seq_stride<L>().  `tw` should point to an LDS copy of the twiddle
+// table (stage_twiddles).  Callers must __syncthreads() before the call (data
+// and twiddles staged) and may read results right after it returns.
+template <typename T, int L, int NT>
+__device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __restrict__ src) {
+  for (int i = threadIdx.x; i < L; i += NT) dst[i] = src[i];
+}
+
 template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN>
-__device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* __restrict__ tw) {
+__device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* tw) {
   static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
   fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1>(buf, tw);
 }

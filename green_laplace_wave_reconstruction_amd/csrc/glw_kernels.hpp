@@ -147,6 +147,8 @@ struct ColInvArgs {
 template <typename T, int GY, int CW>
 __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
+  __shared__ cpx<T> twl[GY];
+  stage_twiddles<T, GY, NT>(twl, tw);
   const int col0 = blockIdx.x * CW;
   const int f = blockIdx.y;
   const int task = blockIdx.z;
@@ -178,7 +180,7 @@ __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __res
     }
   }
   __syncthreads();
-  fft_lds<T, GY, CW, seq_stride<GY>(), NT, +1>(buf, tw);
+  fft_lds<T, GY, CW, seq_stride<GY>(), NT, +1>(buf, twl);
   cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) +
                 ((size_t)task * a.out_nfield + a.out_foff + f) * GY * a.ostride;
   for (int i = threadIdx.x; i < CW * GY; i += NT) {
@@ -218,6 +220,8 @@ __device__ __forceinline__ void block_max_atomic(double v, uint64_t* dst) {
 template <typename T, int GY, int CW>
 __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
+  __shared__ cpx<T> twl[GY];
+  stage_twiddles<T, GY, NT>(twl, tw);
   const int col0 = blockIdx.x * CW;
   const int s = blockIdx.y;
   const int task = blockIdx.z;
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __res
     buf[lds_at<GY>(c, y)] = (col0 + c < a.owx) ? in[(size_t)y * a.istride + col0 + c] : cpx<T>{T(0), T(0)};
   }
   __syncthreads();
-  fft_lds<T, GY, CW, seq_stride<GY>(), NT, -1>(buf, tw);
+  fft_lds<T, GY, CW, seq_stride<GY>(), NT, -1>(buf, twl);
   cpx<double>* out = a.out + ((size_t)task * a.nsrc + s) * a.owy * a.owx;
   double vmax = 0.0;
   for (int i = threadIdx.x; i < CW * a.owy; i += NT) {
@@ -266,6 +270,8 @@ template <typename T, int GX, int NIN, int NOUT, int RY, class Op>
 __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[NIN * RY * SS];
+  __shared__ cpx<T> twl[GX];
+  stage_twiddles<T, GX, NT>(twl, tw);
   const int y0 = blockIdx.x * RY;
   const int task = blockIdx.y;
   const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)task * a.nin * a.gy * a.istride;
@@ -281,7 +287,7 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
     }
   }
   __syncthreads();
-  fft_lds<T, GX, NIN * RY, SS, NT, +1>(buf, tw);
+  fft_lds<T, GX, NIN * RY, SS, NT, +1>(buf, twl);
   for (int i = threadIdx.x; i < RY * GX; i += NT) {
     const int r = i / GX;
     const int x = i - r * GX;
@@ -294,7 +300,7 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
     for (int s = 0; s < NOUT; ++s) buf[lds_at<GX>(s * RY + r, x)] = o[s];
   }
   __syncthreads();
-  fft_lds<T, GX, NOUT * RY, SS, NT, -1>(buf, tw);
+  fft_lds<T, GX, NOUT * RY, SS, NT, -1>(buf, twl);
   cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) + (size_t)task * a.nout * a.gy * a.ostride;
   for (int i = threadIdx.x; i < NOUT * RY * a.owx; i += NT) {
     const int sr = i / a.owx;
@@ -318,6 +324,8 @@ template <typename T, int GX, int RY>
 __global__ __launch_bounds__(NT) void k_synth(SynthArgs a, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[RY * SS];
+  __shared__ cpx<T> twl[GX];
+  stage_twiddles<T, GX, NT>(twl, tw);
   const int y0 = blockIdx.x * RY;
   const int seq = blockIdx.y;
   const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)seq * a.in_seq;
@@ -329,7 +337,7 @@ __global__ __launch_bounds__(NT) void k_synth(SynthArgs a, const cpx<T>* __restr
     buf[lds_at<GX>(r, wrap(a.mx0 + j, GX))] = in[(size_t)(y0 + r) * a.istride + j];
   }
   __syncthreads();
-  fft_lds<T, GX, RY, SS, NT, +1>(buf, tw);
+  fft_lds<T, GX, RY, SS, NT, +1>(buf, twl);
   cpx<double>* out = a.out + (size_t)seq * a.gy * GX;
   for (int i = threadIdx.x; i < RY * GX; i += NT) {
     const int r = i / GX;

@@ -460,7 +460,7 @@ int launch_row(int gx, RowArgs a, int ntask, const void* tw, hipStream_t st) {
 #define GLW_CASE(L)                                                                         \
   case L: {                                                                                 \
     constexpr int RY = row_ry<T, NIN, L>();                                                 \
-    if constexpr (NIN * RY * seq_stride<L>() * sizeof(cpx<T>) <= kLdsMax) {                 \
+    if constexpr ((NIN * RY * seq_stride<L>() + L) * sizeof(cpx<T>) <= kLdsMax) {           \
       if (a.gy % RY) return fail(GLW_EINVAL, "row count not divisible by the row tile");    \
       hipLaunchKernelGGL((k_row<T, L, NIN, NOUT, RY, Op>), dim3(a.gy / RY, ntask), dim3(NT), \
                          0, st, a, (const cpx<T>*)tw);                                      \
