@@ -1,8 +1,8 @@
 """GPU eta44 (pure-sum GL, no Stokes) vs the reference's recorded GL-vs-WIT metrics and the oracle.
 
 The n0382 check reproduces paper/order4/validation/recomputed_field_metrics.csv
-(GL6/8/10 against the reference WIT field): 1e-9 relative for GL6/GL8 and
-1e-7 for GL10.  GL10's largest node (x = 29.9) amplifies roundoff most: a
+(GL6/8/10 against the reference WIT field): 1e-9, 1e-8 and 1e-7 relative for
+GL6, GL8 and GL10.  GL10's largest node (x = 29.9) amplifies roundoff most: a
 1e-16 relative perturbation of the oracle's own input moves its GL10 n0382
 field by 1.6e-10, and the GPU field (reduced alias-free grids, different FFT
 sizes) differs from the oracle field by 2.3e-9 (GL8: 2.5e-10), which moves the
@@ -37,7 +37,7 @@ def test_eta44_n0382_reproduces_reference_metrics(glw, rank):
     print(f"eta44 n0382 GL{rank}", flush=True)
     eta, psi, audit = glw.gl_pure_sum_order4(s, qx, qy, 1.0, rank)
     l2, linf = metrics(h * np.real(eta), ref)
-    tol = 1e-7 if rank == 10 else 1e-9
+    tol = {6: 1e-9, 8: 1e-8, 10: 1e-7}[rank]
     assert abs(l2 - N0382_METRICS[rank][0]) < tol * N0382_METRICS[rank][0]
     assert abs(linf - N0382_METRICS[rank][1]) < 10 * tol * N0382_METRICS[rank][1]
     assert psi is None and audit["quadrature_rank"] == rank
