@@ -331,3 +331,36 @@ def gl_pure_sum_order4(s, qx, qy, qp: float, rank: int = 4, *, precision: str = 
     audit = dict(host.audits["eta44"])
     audit["release_status"] = "experimental no-Stokes; eta44 only"
     return eta, None, audit
+
+
+def run_glrti(input_path, output_path=None, rank: int = 6, *, precision: str = "fp64") -> dict:
+    """Drop-in for ``gl_eta44 INPUT OUTPUT RANK THREADS`` (paper/order4/cpp/gl_eta44_fftw.cpp:639-764).
+
+    Reads a GLRTI001 file (spectrum, dk, depth, peak kh, GL rules), evaluates
+    the pure-sum GL eta44 on the GPU and optionally writes GLE44O01 (the
+    dimensionless analytic eta44 on the n x n grid).  The GL rule of the
+    requested rank must be present in the file, as the reference requires;
+    this build regenerates it (Golub--Welsch) and checks it matches.
+    """
+    import time
+
+    from . import formats
+
+    inp = formats.read_glrti(input_path)
+    if int(rank) not in inp["rules"]:
+        raise ValueError("rank")  # reference message (gl_eta44_fftw.cpp:690)
+    x, w = _plan.gauss_laguerre_rule(int(rank))
+    fx, fw = inp["rules"][int(rank)]
+    if not (np.allclose(x, fx, rtol=1e-12, atol=0) and np.allclose(w, fw, rtol=1e-9, atol=0)):
+        raise ValueError("GL rule in the input differs from the Golub--Welsch rule of that rank")
+    n = int(inp["n"])
+    mode = np.fft.fftfreq(n) * n
+    mx, my = np.meshgrid(mode, mode)
+    h, dk = inp["depth"], inp["dk"]
+    t0 = time.perf_counter()
+    eta, _, audit = gl_pure_sum_order4(inp["spectrum"], h * dk * mx, h * dk * my, inp["peak_kh"], int(rank),
+                                       precision=precision)
+    elapsed = time.perf_counter() - t0
+    if output_path is not None:
+        formats.write_gle44(output_path, eta, rank=int(rank), threads=1, elapsed=elapsed)
+    return {"field": eta, "elapsed": elapsed, "audit": audit}

@@ -48,3 +48,16 @@ def test_eta44_small_case_vs_oracle(glw, rank):
     s, qx, qy = small_case()
     eta, _, _ = glw.gl_pure_sum_order4(s, qx, qy, 1.0, rank)
     assert rel(eta, O.eta44(s, qx, qy, 1.0, rank)) < 1e-10
+
+
+def test_glrti_gle44_drop_in(glw, tmp_path):
+    """GLRTI001 -> GPU eta44 -> GLE44O01, like ``gl_eta44 INPUT OUTPUT 8 THREADS``."""
+    from green_laplace_wave_reconstruction_amd import api, formats, plan as PL
+
+    s, qx, qy, h, ref = n0382()
+    rules = {r: PL.gauss_laguerre_rule(r) for r in (6, 8, 10)}
+    formats.write_glrti(tmp_path / "gl_input.bin", s, dk=0.2, depth=h, peak_kh=1.0, rules=rules)
+    api.run_glrti(tmp_path / "gl_input.bin", tmp_path / "gl_out.bin", 8)
+    out = formats.read_gle44(tmp_path / "gl_out.bin")
+    l2, _ = metrics(h * np.real(out["field"]), ref)
+    assert out["rank"] == 8 and abs(l2 - N0382_METRICS[8][0]) < 1e-8 * N0382_METRICS[8][0]
