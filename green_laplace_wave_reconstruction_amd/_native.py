@@ -61,8 +61,17 @@ class CPlanDesc(ctypes.Structure):
 #: every symbol include/glw.h declares
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
            "glw_order3_execute", "glw_order4_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
-           "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
+           "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_batch_create",
+           "glw_batch_launch", "glw_batch_destroy", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
            "glw_fft_length_supported", "glw_last_error", "glw_version")
+
+#: GLW_KIND_* of include/glw.h
+KINDS = {"linear": 1, "order2": 2, "order3": 3, "order4": 4}
+
+
+class CBatchItem(ctypes.Structure):
+    _fields_ = [("plan", c_void_p), ("kind", c_int32), ("nt", c_int32), ("s0", c_void_p), ("times", c_void_p),
+                ("a", c_void_p), ("b", c_void_p)]
 
 _lib = None
 
@@ -86,6 +95,19 @@ def load_library(path: str | os.PathLike | None = None):
         fn.restype = c_int
     lib.glw_order4_execute.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]
     lib.glw_order4_execute.restype = c_int
+    lib.glw_plan_autotune.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32,
+                                      c_void_p]
+    lib.glw_plan_autotune.restype = c_int
+    lib.glw_plan_get_schedule.argtypes = [c_void_p, c_char_p, c_size_t]
+    lib.glw_plan_get_schedule.restype = c_int
+    lib.glw_plan_set_schedule.argtypes = [c_void_p, c_char_p]
+    lib.glw_plan_set_schedule.restype = c_int
+    lib.glw_batch_create.argtypes = [POINTER(CBatchItem), c_int32, c_int32, POINTER(c_void_p)]
+    lib.glw_batch_create.restype = c_int
+    lib.glw_batch_launch.argtypes = [c_void_p, c_void_p]
+    lib.glw_batch_launch.restype = c_int
+    lib.glw_batch_destroy.argtypes = [c_void_p]
+    lib.glw_batch_destroy.restype = c_int
     lib.glw_plan_profile.argtypes = [c_void_p, c_int32]
     lib.glw_plan_profile.restype = c_int
     lib.glw_plan_stage_times.argtypes = [c_void_p, c_int32, c_char_p, c_size_t, _DP, POINTER(c_int32),
@@ -243,6 +265,42 @@ class DevicePlan:
                   c_void_p(a.data_ptr()), c_void_p(b.data_ptr()), c_void_p(stream)))
         return a, b
 
+    def _outputs(self, nt: int, out):
+        import torch
+
+        shape = (nt, self.host.ny, self.host.nx)
+        if out is None:
+            out = (torch.empty(shape, dtype=torch.complex128, device=self.device),
+                   torch.empty(shape, dtype=torch.complex128, device=self.device))
+        for x in out:
+            if x is not None and (x.shape != shape or x.dtype != torch.complex128 or not x.is_contiguous()
+                                  or x.device != self.device):
+                raise ValueError(f"outputs must be contiguous complex128 {shape} on {self.device}")
+        return out
+
+    def autotune(self, kind: str, times, out=None, reps: int = 3) -> str:
+        """Pick the fastest FFT schedule per stage for ``kind`` at these instants; returns it as text."""
+        import torch
+
+        t = self.times_tensor(times)
+        a, b = self._outputs(int(t.numel()), out)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _check(self.lib.glw_plan_autotune(self._handle, KINDS[kind], c_void_p(self.s0.data_ptr()),
+                                          c_void_p(t.data_ptr()), int(t.numel()), c_void_p(a.data_ptr()),
+                                          c_void_p(b.data_ptr() if b is not None else None), int(reps),
+                                          c_void_p(stream)))
+        return self.schedule
+
+    @property
+    def schedule(self) -> str:
+        buf = ctypes.create_string_buffer(8192)
+        _check(self.lib.glw_plan_get_schedule(self._handle, buf, 8192))
+        return buf.value.decode()
+
+    @schedule.setter
+    def schedule(self, text: str) -> None:
+        _check(self.lib.glw_plan_set_schedule(self._handle, text.encode()))
+
     def profile(self, enable: bool = True) -> None:
         """Bracket every kernel launch of later runs with HIP events on the run's stream."""
         _check(self.lib.glw_plan_profile(self._handle, int(enable)))
@@ -262,3 +320,40 @@ class DevicePlan:
         ptr, size = c_void_p(), c_size_t()
         _check(self.lib.glw_plan_buffer(self._handle, name.encode(), ctypes.byref(ptr), ctypes.byref(size)))
         return ptr.value, size.value
+
+
+class GraphBatch:
+    """Execute calls of many plans captured in one HIP graph (``glw_batch_*``).
+
+    ``items``: sequence of (DevicePlan, kind, times tensor, a, b) with device
+    tensors that stay alive as long as the batch; every ``launch`` re-reads
+    the inputs and rewrites the outputs.
+    """
+
+    def __init__(self, items, *, branches: int = 4):
+        self.lib = load_library()
+        self._keep = list(items)
+        arr = (CBatchItem * len(self._keep))()
+        for i, (plan, kind, times, a, b) in enumerate(self._keep):
+            arr[i] = CBatchItem(plan._handle, KINDS[kind], int(times.numel()), plan.s0.data_ptr(), times.data_ptr(),
+                                a.data_ptr(), b.data_ptr() if b is not None else None)
+        handle = c_void_p()
+        _check(self.lib.glw_batch_create(arr, len(self._keep), int(branches), ctypes.byref(handle)))
+        self._handle = handle
+
+    def launch(self, stream=None) -> None:
+        import torch
+
+        st = stream if stream is not None else torch.cuda.current_stream().cuda_stream
+        _check(self.lib.glw_batch_launch(self._handle, c_void_p(st)))
+
+    def close(self) -> None:
+        if getattr(self, "_handle", None) is not None and self._handle.value:
+            self.lib.glw_batch_destroy(self._handle)
+            self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -354,7 +354,7 @@ def run_glrti(input_path, output_path=None, rank: int = 6, *, precision: str = "
     if not (np.allclose(x, fx, rtol=1e-12, atol=0) and np.allclose(w, fw, rtol=1e-9, atol=0)):
         raise ValueError("GL rule in the input differs from the Golub--Welsch rule of that rank")
     n = int(inp["n"])
-    mode = np.fft.fftfreq(n) * n
+    mode = np.rint(np.fft.fftfreq(n) * n)
     mx, my = np.meshgrid(mode, mode)
     h, dk = inp["depth"], inp["dk"]
     t0 = time.perf_counter()

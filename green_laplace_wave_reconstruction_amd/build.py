@@ -10,11 +10,22 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = [PKG / "csrc" / "glw_plan.hip", PKG / "csrc" / "glw_wit.hip"]
+CSRC = PKG / "csrc"
+ROW_SETS = 7  # operator sets of glw_launch_rows.hip (see GLW_ROW_SET there)
+# (source, extra defines, object name): one hipcc process per unit, run in parallel
+UNITS = [
+    (CSRC / "glw_plan.hip", [], "glw_plan.o"),
+    (CSRC / "glw_wit.hip", [], "glw_wit.o"),
+    (CSRC / "glw_batch.hip", [], "glw_batch.o"),
+    (CSRC / "glw_launch_cols.hip", [], "glw_launch_cols.o"),
+    *[(CSRC / "glw_launch_rows.hip", [f"-DGLW_ROW_SET={i}"], f"glw_launch_rows{i}.o") for i in range(ROW_SETS)],
+]
+SOURCES = sorted({u[0] for u in UNITS})
 OUT = PKG / "_lib" / "libglw.so"
+OBJ = PKG / "_lib" / "obj"
 
 HIPCC_FLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-shared", "-fPIC",
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
     # packed FP32 instructions are disabled for every HIP translation unit
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-command-line-argument",
@@ -28,10 +39,26 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> Path:
-    OUT.parent.mkdir(parents=True, exist_ok=True)
+def _compile(unit, verbose: bool) -> Path:
+    src, defines, name = unit
+    obj = OBJ / name
+    cmd = [hipcc(), *HIPCC_FLAGS, *defines, "-I", str(ROOT / "include"), "-c", str(src), "-o", str(obj)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return obj
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every translation unit (in parallel) and link ``libglw.so``."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    OBJ.mkdir(parents=True, exist_ok=True)
+    jobs = jobs or min(len(UNITS), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        objs = list(pool.map(lambda u: _compile(u, verbose), UNITS))
     tmp = OUT.with_suffix(".so.tmp")
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", str(ROOT / "include"), *[str(x) for x in SOURCES], "-o", str(tmp)]
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", *[str(o) for o in objs], "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

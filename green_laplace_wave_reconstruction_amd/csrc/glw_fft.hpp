@@ -51,6 +51,26 @@ __device__ __forceinline__ int lds_at(int s, int e) {
 
 // Stage radices: two passes where an in-register radix <= 18 allows it
 // (fewer LDS round trips and barriers), else greedy 16/8/4/2/3 factors.
+__host__ __device__ constexpr int radix_greedy(int L, int s) {
+  int rem = L;
+  for (int i = 0; i <= s; ++i) {
+    int r = 0;
+    if (rem <= 1) return 0;
+    if (rem % 8 == 0) r = 8;
+    else if (rem % 4 == 0) r = 4;
+    else if (rem % 2 == 0) r = 2;
+    else if (rem % 3 == 0) r = 3;
+    else return -1;
+    if (i == s) return r;
+    rem /= r;
+  }
+  return 0;
+}
+
+// Schedule PLAN 0: two passes of composite radices; PLAN 1: greedy 8/4/2/3.
+template <int PLAN>
+__host__ __device__ constexpr int radix_plan(int L, int s);
+
 __host__ __device__ constexpr int radix_of(int L, int s) {
   switch (L) {  // explicit plans for the lengths the kernels use most
     case 16: return s == 0 ? 16 : 0;
@@ -84,6 +104,21 @@ __host__ __device__ constexpr int radix_of(int L, int s) {
     if (i == s) return r;
     rem /= r;
   }
+  return 0;
+}
+
+template <>
+__host__ __device__ constexpr int radix_plan<0>(int L, int s) { return radix_of(L, s); }
+template <>
+__host__ __device__ constexpr int radix_plan<1>(int L, int s) { return radix_greedy(L, s); }
+
+// Schedule actually compiled for PLAN at length L: PLAN 1 folds onto PLAN 0
+// where both give the same radix sequence, so no kernel is built twice.
+template <int L, int PLAN>
+constexpr int canonical_plan() {
+  if (PLAN == 0) return 0;
+  for (int s = 0; s < 16; ++s)
+    if (radix_plan<0>(L, s) != radix_plan<1>(L, s)) return PLAN;
   return 0;
 }
 
@@ -242,9 +277,9 @@ __device__ __forceinline__ void dft_reg(cpx<T>* v) {
 
 // One Stockham stage (radix R, NS = product of earlier radices), then the
 // remaining stages by compile-time recursion.
-template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS>
+template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS, int PLAN>
 __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw) {
-  constexpr int R = radix_of(L, S);
+  constexpr int R = radix_plan<PLAN>(L, S);
   if constexpr (R > 0) {
     constexpr int NB = L / R;           // butterflies per sequence
     constexpr int TOTAL = NSEQ * NB;    // butterflies in the workgroup
@@ -288,7 +323,7 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw) {
       }
     }
     __syncthreads();
-    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R>(buf, tw);
+    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R, PLAN>(buf, tw);
   }
 }
 
@@ -301,10 +336,10 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
   for (int i = threadIdx.x; i < L; i += NT) dst[i] = src[i];
 }
 
-template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN>
+template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN = 0>
 __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* tw) {
   static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
-  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1>(buf, tw);
+  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN>(buf, tw);
 }
 
 }  // namespace glw

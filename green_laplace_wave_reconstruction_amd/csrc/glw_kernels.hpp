@@ -144,7 +144,7 @@ struct ColInvArgs {
   int out_nfield, out_foff, ostride;
 };
 
-template <typename T, int GY, int CW>
+template <typename T, int GY, int CW, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
   __shared__ cpx<T> twl[GY];
@@ -180,7 +180,7 @@ __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __res
     }
   }
   __syncthreads();
-  fft_lds<T, GY, CW, seq_stride<GY>(), NT, +1>(buf, twl);
+  fft_lds<T, GY, CW, seq_stride<GY>(), NT, +1, PLAN>(buf, twl);
   cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) +
                 ((size_t)task * a.out_nfield + a.out_foff + f) * GY * a.ostride;
   for (int i = threadIdx.x; i < CW * GY; i += NT) {
@@ -217,7 +217,7 @@ __device__ __forceinline__ void block_max_atomic(double v, uint64_t* dst) {
   }
 }
 
-template <typename T, int GY, int CW>
+template <typename T, int GY, int CW, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
   __shared__ cpx<T> twl[GY];
@@ -232,7 +232,7 @@ __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __res
     buf[lds_at<GY>(c, y)] = (col0 + c < a.owx) ? in[(size_t)y * a.istride + col0 + c] : cpx<T>{T(0), T(0)};
   }
   __syncthreads();
-  fft_lds<T, GY, CW, seq_stride<GY>(), NT, -1>(buf, twl);
+  fft_lds<T, GY, CW, seq_stride<GY>(), NT, -1, PLAN>(buf, twl);
   cpx<double>* out = a.out + ((size_t)task * a.nsrc + s) * a.owy * a.owx;
   double vmax = 0.0;
   for (int i = threadIdx.x; i < CW * a.owy; i += NT) {
@@ -266,7 +266,7 @@ struct RowArgs {
   int gy;
 };
 
-template <typename T, int GX, int NIN, int NOUT, int RY, class Op>
+template <typename T, int GX, int NIN, int NOUT, int RY, class Op, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[NIN * RY * SS];
@@ -287,7 +287,7 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
     }
   }
   __syncthreads();
-  fft_lds<T, GX, NIN * RY, SS, NT, +1>(buf, twl);
+  fft_lds<T, GX, NIN * RY, SS, NT, +1, PLAN>(buf, twl);
   for (int i = threadIdx.x; i < RY * GX; i += NT) {
     const int r = i / GX;
     const int x = i - r * GX;
@@ -300,13 +300,90 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
     for (int s = 0; s < NOUT; ++s) buf[lds_at<GX>(s * RY + r, x)] = o[s];
   }
   __syncthreads();
-  fft_lds<T, GX, NOUT * RY, SS, NT, -1>(buf, twl);
+  fft_lds<T, GX, NOUT * RY, SS, NT, -1, PLAN>(buf, twl);
   cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) + (size_t)task * a.nout * a.gy * a.ostride;
   for (int i = threadIdx.x; i < NOUT * RY * a.owx; i += NT) {
     const int sr = i / a.owx;
     const int j = i - sr * a.owx;
     const int s = sr / RY, r = sr - (sr / RY) * RY;
     out[((size_t)s * a.gy + y0 + r) * a.ostride + j] = buf[lds_at<GX>(s * RY + r, wrap(a.omx0 + j, GX))];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Split row path, for row tiles that do not fit LDS (or when it is faster):
+// x-IFFT of every input field to physical rows in HBM (k_rowinv), pointwise
+// products per physical point (k_pointop), x-FFT of the products pruned to
+// the output box columns (k_rowfwd).  Same results as k_row, element for
+// element: the transforms and the products are the same operations.
+// ----------------------------------------------------------------------------
+template <typename T, int GX, int RY, int PLAN = 0>
+__global__ __launch_bounds__(NT) void k_rowinv(RowArgs a, cpx<T>* __restrict__ phys, const cpx<T>* __restrict__ tw) {
+  constexpr int SS = seq_stride<GX>();
+  __shared__ cpx<T> buf[RY * SS];
+  __shared__ cpx<T> twl[GX];
+  stage_twiddles<T, GX, NT>(twl, tw);
+  const int y0 = blockIdx.x * RY;
+  const int f = blockIdx.y;
+  const int task = blockIdx.z;
+  const int wx = a.fwx[f], mx0 = a.fmx0[f];
+  const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + ((size_t)task * a.nin + f) * a.gy * a.istride;
+  for (int i = threadIdx.x; i < RY * SS; i += NT) buf[i] = {T(0), T(0)};
+  __syncthreads();
+  for (int i = threadIdx.x; i < RY * wx; i += NT) {
+    const int r = i / wx;
+    const int j = i - r * wx;
+    buf[lds_at<GX>(r, wrap(mx0 + j, GX))] = in[(size_t)(y0 + r) * a.istride + j];
+  }
+  __syncthreads();
+  fft_lds<T, GX, RY, SS, NT, +1, PLAN>(buf, twl);
+  cpx<T>* out = phys + (((size_t)task * a.nin + f) * a.gy + y0) * GX;
+  for (int i = threadIdx.x; i < RY * GX; i += NT) {
+    const int r = i / GX;
+    const int x = i - r * GX;
+    out[(size_t)r * GX + x] = buf[lds_at<GX>(r, x)];
+  }
+}
+
+// in: [task][NIN][npts], out: [task][NOUT][npts]
+template <typename T, int NIN, int NOUT, class Op>
+__global__ __launch_bounds__(NT) void k_pointop(const cpx<T>* __restrict__ in, cpx<T>* __restrict__ out, int npts) {
+  const int p = blockIdx.x * NT + threadIdx.x;
+  const int task = blockIdx.y;
+  if (p >= npts) return;
+  const cpx<T>* src = in + (size_t)task * NIN * npts + p;
+  cpx<T> v[NIN];
+#pragma unroll
+  for (int f = 0; f < NIN; ++f) v[f] = src[(size_t)f * npts];
+  cpx<T> o[NOUT];
+  Op::apply(v, o);
+  cpx<T>* dst = out + (size_t)task * NOUT * npts + p;
+#pragma unroll
+  for (int s = 0; s < NOUT; ++s) dst[(size_t)s * npts] = o[s];
+}
+
+template <typename T, int GX, int RY, int PLAN = 0>
+__global__ __launch_bounds__(NT) void k_rowfwd(RowArgs a, const cpx<T>* __restrict__ phys, const cpx<T>* __restrict__ tw) {
+  constexpr int SS = seq_stride<GX>();
+  __shared__ cpx<T> buf[RY * SS];
+  __shared__ cpx<T> twl[GX];
+  stage_twiddles<T, GX, NT>(twl, tw);
+  const int y0 = blockIdx.x * RY;
+  const int s = blockIdx.y;
+  const int task = blockIdx.z;
+  const cpx<T>* src = phys + (((size_t)task * a.nout + s) * a.gy + y0) * GX;
+  for (int i = threadIdx.x; i < RY * GX; i += NT) {
+    const int r = i / GX;
+    const int x = i - r * GX;
+    buf[lds_at<GX>(r, x)] = src[(size_t)r * GX + x];
+  }
+  __syncthreads();
+  fft_lds<T, GX, RY, SS, NT, -1, PLAN>(buf, twl);
+  cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) + ((size_t)task * a.nout + s) * a.gy * a.ostride;
+  for (int i = threadIdx.x; i < RY * a.owx; i += NT) {
+    const int r = i / a.owx;
+    const int j = i - r * a.owx;
+    out[(size_t)(y0 + r) * a.ostride + j] = buf[lds_at<GX>(r, wrap(a.omx0 + j, GX))];
   }
 }
 

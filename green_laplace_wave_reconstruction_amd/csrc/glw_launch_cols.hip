@@ -1,0 +1,46 @@
+// Column passes, split-row transforms and row synthesis: launcher
+// instantiations for FP64 and FP32 tiles.
+#include "glw_launch_impl.hpp"
+
+namespace glw {
+
+bool length_supported(int n) {
+  switch (n) {
+#define GLW_CASE(L) case L:
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    return true;
+    default:
+      return false;
+  }
+}
+
+int launch_synth(int gx, const SynthArgs& a, int nseq, const void* tw, hipStream_t st) {
+  switch (gx) {
+#define GLW_CASE(L)                                                                                            \
+  case L: {                                                                                                    \
+    constexpr int RY = (L <= 256) ? 4 : 2;                                                                     \
+    if (a.gy % RY) return glw_report_error(GLW_EINVAL, "row count not divisible by the synthesis tile");       \
+    hipLaunchKernelGGL((k_synth<double, L, RY>), dim3(a.gy / RY, nseq), dim3(NT), 0, st, a,                    \
+                       (const cpx<double>*)tw);                                                                \
+    break;                                                                                                     \
+  }
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return glw_report_error(GLW_EUNSUPPORTED, "synthesis length " + std::to_string(gx) + " not instantiated");
+  }
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+#define GLW_INST(T)                                                                                        \
+  template int launch_colinv<T>(int, const ColInvArgs&, int, const void*, const Exec&, hipStream_t);       \
+  template int launch_colfwd<T>(int, const ColFwdArgs&, int, const void*, const Exec&, hipStream_t);       \
+  template int launch_rowinv<T>(int, const RowArgs&, int, const void*, int, void*, hipStream_t);           \
+  template int launch_rowfwd<T>(int, const RowArgs&, int, const void*, int, const void*, hipStream_t);
+GLW_INST(double)
+GLW_INST(float)
+#undef GLW_INST
+
+}  // namespace glw

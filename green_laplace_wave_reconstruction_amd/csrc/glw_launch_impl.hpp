@@ -1,0 +1,229 @@
+// Definitions of the launchers declared in glw_launch.hpp.  Included only by
+// the glw_launch_*.hip translation units, which instantiate them.
+#pragma once
+
+#include <string>
+
+#include "glw_internal.hpp"
+#include "glw_launch.hpp"
+
+namespace glw {
+
+constexpr int kLdsBudget = 48 * 1024;  // column tiles: keep several workgroups per CU
+constexpr int kLdsMax = 160 * 1024;    // gfx950 LDS per workgroup
+
+// Widest power-of-two column tile (<= 32 columns) within kLdsBudget.
+template <typename T, int L>
+constexpr int col_width() {
+  int cw = kLdsBudget / (seq_stride<L>() * (int)sizeof(cpx<T>));
+  int p = 1;
+  while (p * 2 <= cw && p < 32) p *= 2;
+  return p;
+}
+
+// Rows per workgroup of the fused row kernel: as many as fit 64 KB (at most 4).
+template <typename T, int NIN, int L>
+constexpr int row_ry() {
+  int per = NIN * seq_stride<L>() * (int)sizeof(cpx<T>);
+  int ry = (64 * 1024) / per;
+  if (ry >= 4) return 4;
+  if (ry >= 2) return 2;
+  return 1;
+}
+
+template <typename T, int NIN, int L, int RY>
+constexpr bool row_fits() {
+  return (size_t)(NIN * RY * seq_stride<L>() + L) * sizeof(cpx<T>) <= (size_t)kLdsMax;
+}
+
+// Rows per workgroup of the split-path transforms.
+template <typename T, int L>
+constexpr int split_ry() {
+  int ry = kLdsBudget / (seq_stride<L>() * (int)sizeof(cpx<T>));
+  return ry >= 4 ? 4 : (ry >= 2 ? 2 : 1);
+}
+
+// ---------------------------------------------------------------- columns
+template <typename T, int L, int CW, int PLAN>
+void colinv_go(const ColInvArgs& a, int ntask, const void* tw, hipStream_t st) {
+  hipLaunchKernelGGL((k_colinv<T, L, CW, PLAN>), dim3((a.wx + CW - 1) / CW, a.nfield, ntask), dim3(NT), 0, st, a,
+                     (const cpx<T>*)tw);
+}
+
+template <typename T, int L, int CW, int PLAN>
+void colfwd_go(const ColFwdArgs& a, int ntask, const void* tw, hipStream_t st) {
+  hipLaunchKernelGGL((k_colfwd<T, L, CW, PLAN>), dim3((a.owx + CW - 1) / CW, a.nsrc, ntask), dim3(NT), 0, st, a,
+                     (const cpx<T>*)tw);
+}
+
+template <typename T, int L>
+void colinv_len(const ColInvArgs& a, int ntask, const void* tw, Sched s, hipStream_t st) {
+  constexpr int CW0 = col_width<T, L>();
+  constexpr int CW1 = CW0 > 1 ? CW0 / 2 : 1;
+  constexpr int P1 = canonical_plan<L, 1>();
+  if (s.plan == 0)
+    s.tile == 0 ? colinv_go<T, L, CW0, 0>(a, ntask, tw, st) : colinv_go<T, L, CW1, 0>(a, ntask, tw, st);
+  else
+    s.tile == 0 ? colinv_go<T, L, CW0, P1>(a, ntask, tw, st) : colinv_go<T, L, CW1, P1>(a, ntask, tw, st);
+}
+
+template <typename T, int L>
+void colfwd_len(const ColFwdArgs& a, int ntask, const void* tw, Sched s, hipStream_t st) {
+  constexpr int CW0 = col_width<T, L>();
+  constexpr int CW1 = CW0 > 1 ? CW0 / 2 : 1;
+  constexpr int P1 = canonical_plan<L, 1>();
+  if (s.plan == 0)
+    s.tile == 0 ? colfwd_go<T, L, CW0, 0>(a, ntask, tw, st) : colfwd_go<T, L, CW1, 0>(a, ntask, tw, st);
+  else
+    s.tile == 0 ? colfwd_go<T, L, CW0, P1>(a, ntask, tw, st) : colfwd_go<T, L, CW1, P1>(a, ntask, tw, st);
+}
+
+template <typename T>
+int launch_colinv(int gy, const ColInvArgs& a, int ntask, const void* tw, const Exec& x, hipStream_t st) {
+  switch (gy) {
+#define GLW_CASE(L)                                 \
+  case L:                                           \
+    colinv_len<T, L>(a, ntask, tw, x.s, st);        \
+    break;
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return glw_report_error(GLW_EUNSUPPORTED, "column length " + std::to_string(gy) + " not instantiated");
+  }
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+template <typename T>
+int launch_colfwd(int gy, const ColFwdArgs& a, int ntask, const void* tw, const Exec& x, hipStream_t st) {
+  switch (gy) {
+#define GLW_CASE(L)                                 \
+  case L:                                           \
+    colfwd_len<T, L>(a, ntask, tw, x.s, st);        \
+    break;
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return glw_report_error(GLW_EUNSUPPORTED, "column length " + std::to_string(gy) + " not instantiated");
+  }
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+// ---------------------------------------------------------------- split rows
+template <typename T, int L, int PLAN>
+void rowinv_go(const RowArgs& a, int ntask, const void* tw, void* phys, hipStream_t st) {
+  constexpr int RY = split_ry<T, L>();
+  hipLaunchKernelGGL((k_rowinv<T, L, RY, PLAN>), dim3(a.gy / RY, a.nin, ntask), dim3(NT), 0, st, a, (cpx<T>*)phys,
+                     (const cpx<T>*)tw);
+}
+
+template <typename T, int L, int PLAN>
+void rowfwd_go(const RowArgs& a, int ntask, const void* tw, const void* phys, hipStream_t st) {
+  constexpr int RY = split_ry<T, L>();
+  hipLaunchKernelGGL((k_rowfwd<T, L, RY, PLAN>), dim3(a.gy / RY, a.nout, ntask), dim3(NT), 0, st, a,
+                     (const cpx<T>*)phys, (const cpx<T>*)tw);
+}
+
+template <typename T>
+int launch_rowinv(int gx, const RowArgs& a, int ntask, const void* tw, int plan, void* phys, hipStream_t st) {
+  switch (gx) {
+#define GLW_CASE(L)                                                                                      \
+  case L:                                                                                                \
+    if (a.gy % split_ry<T, L>()) return glw_report_error(GLW_EINVAL, "row count not divisible by tile"); \
+    plan == 0 ? rowinv_go<T, L, 0>(a, ntask, tw, phys, st)                                              \
+              : rowinv_go<T, L, canonical_plan<L, 1>()>(a, ntask, tw, phys, st);                        \
+    break;
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return glw_report_error(GLW_EUNSUPPORTED, "row length " + std::to_string(gx) + " not instantiated");
+  }
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+template <typename T>
+int launch_rowfwd(int gx, const RowArgs& a, int ntask, const void* tw, int plan, const void* phys, hipStream_t st) {
+  switch (gx) {
+#define GLW_CASE(L)                                                                                      \
+  case L:                                                                                                \
+    if (a.gy % split_ry<T, L>()) return glw_report_error(GLW_EINVAL, "row count not divisible by tile"); \
+    plan == 0 ? rowfwd_go<T, L, 0>(a, ntask, tw, phys, st)                                              \
+              : rowfwd_go<T, L, canonical_plan<L, 1>()>(a, ntask, tw, phys, st);                        \
+    break;
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return glw_report_error(GLW_EUNSUPPORTED, "row length " + std::to_string(gx) + " not instantiated");
+  }
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+// ---------------------------------------------------------------- fused rows
+template <typename T, int NIN, int NOUT, class Op, int L, int RY, int PLAN>
+int row_go(const RowArgs& a, int ntask, const void* tw, hipStream_t st) {
+  if (a.gy % RY) return glw_report_error(GLW_EINVAL, "row count not divisible by the row tile");
+  hipLaunchKernelGGL((k_row<T, L, NIN, NOUT, RY, Op, PLAN>), dim3(a.gy / RY, ntask), dim3(NT), 0, st, a,
+                     (const cpx<T>*)tw);
+  return GLW_OK;
+}
+
+template <typename T, int NIN, int NOUT, class Op>
+int row_split(int gx, const RowArgs& a, int ntask, const void* tw, const Exec& x, hipStream_t st) {
+  const size_t npts = (size_t)a.gy * gx;
+  const size_t bin = (size_t)ntask * NIN * npts * sizeof(cpx<T>);
+  const size_t bout = (size_t)ntask * NOUT * npts * sizeof(cpx<T>);
+  if (!x.scratch) return glw_report_error(GLW_EINVAL, "split row path needs a scratch provider");
+  void* ws = nullptr;
+  int rc = x.scratch(x.ctx, bin + bout, &ws);
+  if (rc) return rc;
+  char* pin = (char*)ws;
+  char* pout = pin + bin;
+  if ((rc = launch_rowinv<T>(gx, a, ntask, tw, x.s.plan, pin, st))) return rc;
+  hipLaunchKernelGGL((k_pointop<T, NIN, NOUT, Op>), dim3((unsigned)((npts + NT - 1) / NT), ntask), dim3(NT), 0, st,
+                     (const cpx<T>*)pin, (cpx<T>*)pout, (int)npts);
+  GLW_HIP_CHECK(hipGetLastError());
+  return launch_rowfwd<T>(gx, a, ntask, tw, x.s.plan, pout, st);
+}
+
+template <typename T, int NIN, int NOUT, class Op, int L>
+int row_len(const RowArgs& a, int ntask, const void* tw, const Exec& x, hipStream_t st) {
+  static_assert(NOUT <= NIN, "the fused row kernel reuses the input tile for the outputs");
+  constexpr int RY0 = row_ry<T, NIN, L>();
+  constexpr int P1 = canonical_plan<L, 1>();
+  int tile = x.s.tile;
+  if (tile == 0 && !row_fits<T, NIN, L, RY0>()) tile = 1;
+  if (tile == 1 && !row_fits<T, NIN, L, 1>()) tile = 2;
+  if (tile == 0) {
+    if constexpr (row_fits<T, NIN, L, RY0>())
+      return x.s.plan == 0 ? row_go<T, NIN, NOUT, Op, L, RY0, 0>(a, ntask, tw, st)
+                           : row_go<T, NIN, NOUT, Op, L, RY0, P1>(a, ntask, tw, st);
+  } else if (tile == 1) {
+    if constexpr (row_fits<T, NIN, L, 1>())
+      return x.s.plan == 0 ? row_go<T, NIN, NOUT, Op, L, 1, 0>(a, ntask, tw, st)
+                           : row_go<T, NIN, NOUT, Op, L, 1, P1>(a, ntask, tw, st);
+  }
+  return row_split<T, NIN, NOUT, Op>(L, a, ntask, tw, x, st);
+}
+
+template <typename T, int NIN, int NOUT, class Op>
+int launch_row(int gx, const RowArgs& a, int ntask, const void* tw, const Exec& x, hipStream_t st) {
+  int rc;
+  switch (gx) {
+#define GLW_CASE(L)                                               \
+  case L:                                                         \
+    rc = row_len<T, NIN, NOUT, Op, L>(a, ntask, tw, x, st);       \
+    break;
+    GLW_LENGTHS(GLW_CASE)
+#undef GLW_CASE
+    default:
+      return glw_report_error(GLW_EUNSUPPORTED, "row length " + std::to_string(gx) + " not instantiated");
+  }
+  if (rc) return rc;
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+}  // namespace glw

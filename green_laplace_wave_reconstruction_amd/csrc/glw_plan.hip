@@ -16,6 +16,7 @@
 #include "glw.h"
 #include "glw_internal.hpp"
 #include "glw_kernels.hpp"
+#include "glw_launch.hpp"
 #include "glw_ops.hpp"
 
 using namespace glw;
@@ -379,125 +380,6 @@ __global__ __launch_bounds__(NT) void k_acc_quartic44(Acc4Args g) {
 }
 
 // ----------------------------------------------------------------------------
-// Dispatch over FFT lengths
-// ----------------------------------------------------------------------------
-#define GLW_LENGTHS(X) \
-  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768) X(1024)
-
-constexpr int kLdsBudget = 48 * 1024;  // column kernels: keep several workgroups per CU
-constexpr int kLdsMax = 160 * 1024;    // gfx950 per-workgroup LDS
-
-template <typename T, int L>
-constexpr int col_width() {
-  int cw = kLdsBudget / (seq_stride<L>() * (int)sizeof(cpx<T>));
-  int p = 1;
-  while (p * 2 <= cw && p < 32) p *= 2;
-  return p;
-}
-
-bool length_supported(int n) {
-  switch (n) {
-#define GLW_CASE(L) case L:
-    GLW_LENGTHS(GLW_CASE)
-#undef GLW_CASE
-    return true;
-    default:
-      return false;
-  }
-}
-
-template <typename T>
-int launch_colinv(int gy, ColInvArgs a, int ntask, const void* tw, hipStream_t st) {
-  switch (gy) {
-#define GLW_CASE(L)                                                                       \
-  case L: {                                                                               \
-    constexpr int CW = col_width<T, L>();                                                 \
-    hipLaunchKernelGGL((k_colinv<T, L, CW>), dim3((a.wx + CW - 1) / CW, a.nfield, ntask), \
-                       dim3(NT), 0, st, a, (const cpx<T>*)tw);                            \
-    break;                                                                                \
-  }
-    GLW_LENGTHS(GLW_CASE)
-#undef GLW_CASE
-    default:
-      return fail(GLW_EUNSUPPORTED, "column length " + std::to_string(gy) + " not instantiated");
-  }
-  GLW_HIP(hipGetLastError());
-  return GLW_OK;
-}
-
-template <typename T>
-int launch_colfwd(int gy, ColFwdArgs a, int ntask, const void* tw, hipStream_t st) {
-  switch (gy) {
-#define GLW_CASE(L)                                                                        \
-  case L: {                                                                                \
-    constexpr int CW = col_width<T, L>();                                                  \
-    hipLaunchKernelGGL((k_colfwd<T, L, CW>), dim3((a.owx + CW - 1) / CW, a.nsrc, ntask),  \
-                       dim3(NT), 0, st, a, (const cpx<T>*)tw);                             \
-    break;                                                                                 \
-  }
-    GLW_LENGTHS(GLW_CASE)
-#undef GLW_CASE
-    default:
-      return fail(GLW_EUNSUPPORTED, "column length " + std::to_string(gy) + " not instantiated");
-  }
-  GLW_HIP(hipGetLastError());
-  return GLW_OK;
-}
-
-// Rows per workgroup: as many as fit a 64 KB tile (at most 4), at least 1.
-template <typename T, int NIN, int L>
-constexpr int row_ry() {
-  int per = NIN * seq_stride<L>() * (int)sizeof(cpx<T>);
-  int ry = (64 * 1024) / per;
-  if (ry >= 4) return 4;
-  if (ry >= 2) return 2;
-  return 1;
-}
-
-template <typename T, int NIN, int NOUT, class Op>
-int launch_row(int gx, RowArgs a, int ntask, const void* tw, hipStream_t st) {
-  switch (gx) {
-#define GLW_CASE(L)                                                                         \
-  case L: {                                                                                 \
-    constexpr int RY = row_ry<T, NIN, L>();                                                 \
-    if constexpr ((NIN * RY * seq_stride<L>() + L) * sizeof(cpx<T>) <= kLdsMax) {           \
-      if (a.gy % RY) return fail(GLW_EINVAL, "row count not divisible by the row tile");    \
-      hipLaunchKernelGGL((k_row<T, L, NIN, NOUT, RY, Op>), dim3(a.gy / RY, ntask), dim3(NT), \
-                         0, st, a, (const cpx<T>*)tw);                                      \
-    } else {                                                                                \
-      return fail(GLW_EUNSUPPORTED, "row tile exceeds LDS at length " + std::to_string(L)); \
-    }                                                                                       \
-    break;                                                                                  \
-  }
-    GLW_LENGTHS(GLW_CASE)
-#undef GLW_CASE
-    default:
-      return fail(GLW_EUNSUPPORTED, "row length " + std::to_string(gx) + " not instantiated");
-  }
-  GLW_HIP(hipGetLastError());
-  return GLW_OK;
-}
-
-int launch_synth(int gx, SynthArgs a, int nseq, const void* tw, hipStream_t st) {
-  switch (gx) {
-#define GLW_CASE(L)                                                                          \
-  case L: {                                                                                  \
-    constexpr int RY = (L <= 256) ? 4 : 2;                                                   \
-    if (a.gy % RY) return fail(GLW_EINVAL, "row count not divisible by the synthesis tile"); \
-    hipLaunchKernelGGL((k_synth<double, L, RY>), dim3(a.gy / RY, nseq), dim3(NT), 0, st, a,  \
-                       (const cpx<double>*)tw);                                              \
-    break;                                                                                   \
-  }
-    GLW_LENGTHS(GLW_CASE)
-#undef GLW_CASE
-    default:
-      return fail(GLW_EUNSUPPORTED, "synthesis length " + std::to_string(gx) + " not instantiated");
-  }
-  GLW_HIP(hipGetLastError());
-  return GLW_OK;
-}
-
-// ----------------------------------------------------------------------------
 // Plan
 // ----------------------------------------------------------------------------
 struct DevTables {
@@ -533,6 +415,14 @@ struct glw_plan_s {
   double *p22_damp = nullptr, *p22_fac = nullptr, *p22_coef = nullptr, *p22_sign = nullptr;
   // workspace: name -> (ptr, bytes)
   std::map<std::string, std::pair<void*, size_t>> ws;
+  // per-stage schedule (absent: plan 0, tile 0) and the kind of every stage seen
+  std::map<std::string, Sched> sched;
+  std::map<std::string, int> stage_kind;
+  const char* cur = "";
+  hipStream_t cur_stream = nullptr;
+  // scratch of the split row path, grown on demand outside graph capture
+  void* split_ptr = nullptr;
+  size_t split_bytes = 0;
   // stage timing
   std::vector<hipEvent_t> ev;
   std::vector<const char*> ev_name;
@@ -542,10 +432,41 @@ struct glw_plan_s {
   ~glw_plan_s() {
     for (hipEvent_t e : ev) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
+    if (split_ptr) (void)hipFree(split_ptr);
+  }
+
+  static int scratch(void* ctx, size_t bytes, void** ptr) {
+    glw_plan_s* P = static_cast<glw_plan_s*>(ctx);
+    if (bytes > P->split_bytes) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      GLW_HIP(hipStreamIsCapturing(P->cur_stream, &cs));
+      if (cs != hipStreamCaptureStatusNone)
+        return fail(GLW_EINVAL, "split row scratch must be sized before graph capture: execute the plan once first");
+      if (P->split_ptr) GLW_HIP(hipFree(P->split_ptr));  // hipFree waits for work in flight
+      P->split_ptr = nullptr;
+      P->split_bytes = 0;
+      const size_t rounded = (bytes + (size_t(2) << 20) - 1) & ~((size_t(2) << 20) - 1);
+      GLW_HIP(hipMalloc(&P->split_ptr, rounded));
+      P->split_bytes = rounded;
+    }
+    *ptr = P->split_ptr;
+    return GLW_OK;
+  }
+
+  // Schedule of the stage being launched (named by the last mark_begin).
+  Exec exec(int kind) {
+    stage_kind[cur] = kind;
+    Exec x;
+    auto it = sched.find(cur);
+    if (it != sched.end()) x.s = it->second;
+    x.scratch = &glw_plan_s::scratch;
+    x.ctx = this;
+    return x;
   }
 
   // Bracket one launch with two events (no-op unless profiling is enabled).
   void mark_begin(const char* name, hipStream_t st) {
+    cur = name;
     if (ev.empty()) return;
     if (ev_used + 2 > ev.size()) {
       ++ev_dropped;
@@ -869,7 +790,7 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
   a.out_nfield = nout;
   a.out_foff = 0;
   a.ostride = box.wx;
-  int rc = GLW_STAGE(P, "synth_colinv", st, launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), st));
+  int rc = GLW_STAGE(P, "synth_colinv", st, launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), P->exec(STAGE_COL), st));
   if (rc) return rc;
   for (int f = 0; f < nout; ++f) {
     // rows of field f of every instant: sequence t starts at (t*nout + f)*ny*wx
@@ -921,7 +842,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("tin").first;
     a.out_nfield = 12;
     a.ostride = d.p1.wx;
-    if ((rc = GLW_STAGE(P, "o3_inner_colinv", st, launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_colinv", st, launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     RowArgs r;
@@ -939,7 +860,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.omx0 = d.p2.mx0;
     r.owx = d.p2.wx;
     r.gy = d.g2y;
-    if ((rc = GLW_STAGE(P, "o3_inner_row", st, (launch_row<T, 12, 4, InnerOp>(d.g2x, r, n_in, P->tw(d.g2x, f32), st))))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_row", st, (launch_row<T, 12, 4, InnerOp>(d.g2x, r, n_in, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st))))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_in"), (size_t)n_in * 4, st))) return rc;
   {
@@ -955,7 +876,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g2x * d.g2y);
     c.out = P->buf<cpx<double>>("sin");
     c.fmax = P->buf<uint64_t>("max_in");
-    if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_et"), (size_t)n_to, st))) return rc;
   {
@@ -996,7 +917,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out_nfield = 21;
     a.out_foff = 0;
     a.ostride = wmax;
-    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     ColInvArgs a = colinv_base(P->buf<cpx<double>>("pair"), 4, 1, d.p2, P->t2);
@@ -1017,7 +938,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out_nfield = 21;
     a.out_foff = 12;
     a.ostride = wmax;
-    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     RowArgs r;
@@ -1035,7 +956,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.omx0 = d.p3.mx0;
     r.owx = d.p3.wx;
     r.gy = d.g3y;
-    if ((rc = GLW_STAGE(P, "o3_outer_row", st, (launch_row<T, 21, 4, OuterOp>(d.g3x, r, n_to, P->tw(d.g3x, f32), st))))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_row", st, (launch_row<T, 21, 4, OuterOp>(d.g3x, r, n_to, P->tw(d.g3x, f32), P->exec(STAGE_ROW), st))))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_out"), (size_t)n_to * 4, st))) return rc;
   if ((rc = fill_zero(P->buf<uint64_t>("max_combo"), (size_t)n_to * 2, st))) return rc;
@@ -1052,7 +973,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g3x * d.g3y);
     c.out = P->buf<cpx<double>>("sout");
     c.fmax = P->buf<uint64_t>("max_out");
-    if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     ComboArgs g;
@@ -1152,12 +1073,12 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("e22_t").first;
     a.out_nfield = 8;
     a.ostride = d.p1.wx;
-    if ((rc = GLW_STAGE(P, "o2_eta_colinv", st, launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), st)))) return rc;
-    if ((rc = GLW_STAGE(P, "o2_eta_row", st, (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), st)))
+    if ((rc = GLW_STAGE(P, "o2_eta_colinv", st, launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o2_eta_row", st, (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("e22_max"), (size_t)nt * J * 2, st))) return rc;
     if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
-                               P->tw(d.g2y, f32), st)))
+                               P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
     AccEta22Args g;
     g.spec = P->buf<cpx<double>>("e22_s");
@@ -1197,11 +1118,11 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
       a.out_nfield = 16;
       a.out_foff = 8 * ch;
       a.ostride = d.p1.wx;
-      if ((rc = GLW_STAGE(P, "o2_psi_colinv", st, launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), st)))) return rc;
+      if ((rc = GLW_STAGE(P, "o2_psi_colinv", st, launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
     }
-    if ((rc = GLW_STAGE(P, "o2_psi_row", st, (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), st)))
+    if ((rc = GLW_STAGE(P, "o2_psi_row", st, (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), st)))
+    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   // ---- psi22 Taylor contact: tasks t
@@ -1217,10 +1138,10 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("c22_t").first;
     a.out_nfield = 2;
     a.ostride = d.p1.wx;
-    if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), st)))) return rc;
-    if ((rc = GLW_STAGE(P, "o2_contact_row", st, (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), st)))
+    if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o2_contact_row", st, (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), st)))
+    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   {
@@ -1278,7 +1199,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     a.out = P->ws.at("o4_pt").first;
     a.out_nfield = 16;
     a.ostride = d.p1.wx;
-    if ((rc2 = GLW_STAGE(P, "o4_pair_colinv", st, launch_colinv<T>(d.g2y, a, ntask, P->tw(d.g2y, f32), st))))
+    if ((rc2 = GLW_STAGE(P, "o4_pair_colinv", st, launch_colinv<T>(d.g2y, a, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
       return rc2;
     RowArgs r;
     memset(&r, 0, sizeof(r));
@@ -1296,7 +1217,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     r.owx = d.p2.wx;
     r.gy = d.g2y;
     if ((rc2 = GLW_STAGE(P, "o4_pair_row", st,
-                         (launch_row<T, 16, 6, Pair44Op>(d.g2x, r, ntask, P->tw(d.g2x, f32), st)))))
+                         (launch_row<T, 16, 6, Pair44Op>(d.g2x, r, ntask, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))))
       return rc2;
     if ((rc2 = fill_zero(P->buf<uint64_t>("o4_pm"), (size_t)ntask * 6, st))) return rc2;
     ColFwdArgs c;
@@ -1311,7 +1232,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.scale = nn / ((double)d.g2x * d.g2y);
     c.out = P->buf<cpx<double>>("o4_ps");
     c.fmax = P->buf<uint64_t>("o4_pm");
-    if ((rc2 = GLW_STAGE(P, "o4_pair_colfwd", st, launch_colfwd<T>(d.g2y, c, ntask, P->tw(d.g2y, f32), st))))
+    if ((rc2 = GLW_STAGE(P, "o4_pair_colfwd", st, launch_colfwd<T>(d.g2y, c, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
       return rc2;
     if ((rc2 = fill_zero(P->buf<uint64_t>(setmax), (size_t)nset * 5, st))) return rc2;
     Acc4Args g;
@@ -1355,7 +1276,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     a.out_nfield = 31;
     a.out_foff = 0;
     a.ostride = w12;
-    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, a, ntask, P->tw(d.g3y, f32), st))))
+    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, a, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     ColInvArgs b = colinv_base(P->buf<cpx<double>>("o4_set3"), 5, 1, d.p2, P->t2);
     b.fmax = P->buf<uint64_t>("o4_max3");
@@ -1367,7 +1288,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     b.out_nfield = 31;
     b.out_foff = 16;
     b.ostride = w12;
-    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, b, ntask, P->tw(d.g3y, f32), st))))
+    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, b, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     RowArgs r;
     memset(&r, 0, sizeof(r));
@@ -1385,7 +1306,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     r.owx = d.p3.wx;
     r.gy = d.g3y;
     if ((rc = GLW_STAGE(P, "o4_third_row", st,
-                        (launch_row<T, 31, 4, Third44Op>(d.g3x, r, ntask, P->tw(d.g3x, f32), st)))))
+                        (launch_row<T, 31, 4, Third44Op>(d.g3x, r, ntask, P->tw(d.g3x, f32), P->exec(STAGE_ROW), st)))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_tm"), (size_t)ntask * 4, st))) return rc;
     ColFwdArgs c;
@@ -1400,7 +1321,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.scale = nn / ((double)d.g3x * d.g3y);
     c.out = P->buf<cpx<double>>("o4_ts");
     c.fmax = P->buf<uint64_t>("o4_tm");
-    if ((rc = GLW_STAGE(P, "o4_third_colfwd", st, launch_colfwd<T>(d.g3y, c, ntask, P->tw(d.g3y, f32), st))))
+    if ((rc = GLW_STAGE(P, "o4_third_colfwd", st, launch_colfwd<T>(d.g3y, c, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_thm"), (size_t)nt * J * 3, st))) return rc;
     Acc4Args g;
@@ -1435,7 +1356,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     a.out_nfield = 45;
     a.out_foff = 0;
     a.ostride = w123;
-    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, a, ntask, P->tw(o.g4y, f32), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, a, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     ColInvArgs b = colinv_base(P->buf<cpx<double>>("o4_set4"), 5, 1, d.p2, P->t2);
     b.fmax = P->buf<uint64_t>("o4_max4");
@@ -1447,7 +1368,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     b.out_nfield = 45;
     b.out_foff = 19;
     b.ostride = w123;
-    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, b, ntask, P->tw(o.g4y, f32), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, b, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     static const int third_src[8] = {0, 0, 0, 1, 1, 1, 1, 2};
     static const int third_codes[8] = {M_ONE, M_I_QX, M_I_QY, M_I_QX, M_I_QY, M_A, M_Q2, M_A};
@@ -1461,7 +1382,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c3.out_nfield = 45;
     c3.out_foff = 37;
     c3.ostride = w123;
-    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, c3, ntask, P->tw(o.g4y, f32), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, c3, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     RowArgs r;
     memset(&r, 0, sizeof(r));
@@ -1480,7 +1401,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     r.owx = o.p4.wx;
     r.gy = o.g4y;
     if ((rc = GLW_STAGE(P, "o4_quartic_row", st,
-                        (launch_row<T, 45, 2, Quartic44Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), st)))))
+                        (launch_row<T, 45, 2, Quartic44Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st)))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_qm"), (size_t)ntask * 2, st))) return rc;
     ColFwdArgs c;
@@ -1495,7 +1416,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.scale = nn / ((double)o.g4x * o.g4y);
     c.out = P->buf<cpx<double>>("o4_qs");
     c.fmax = P->buf<uint64_t>("o4_qm");
-    if ((rc = GLW_STAGE(P, "o4_quartic_colfwd", st, launch_colfwd<T>(o.g4y, c, ntask, P->tw(o.g4y, f32), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colfwd", st, launch_colfwd<T>(o.g4y, c, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_outm"), (size_t)nt, st))) return rc;
     Acc4Args g;
@@ -1539,8 +1460,11 @@ int run_chunks(glw_plan* P, const double* s0, const double* times, int nt, doubl
   if (nt < 0) return fail(GLW_EINVAL, "negative instant count");
   if (nt == 0) return GLW_OK;
   if (!s0 || !times || !a || (!b && fn != order4_dispatch)) return fail(GLW_EINVAL, "null array argument");
-  GLW_HIP(hipSetDevice(P->device));
+  int dev = -1;
+  GLW_HIP(hipGetDevice(&dev));
+  if (dev != P->device) GLW_HIP(hipSetDevice(P->device));
   hipStream_t st = (hipStream_t)stream;
+  P->cur_stream = st;
   const size_t plane = (size_t)P->d.ny * P->d.nx * 2;  // doubles per output field
   for (int t0 = 0; t0 < nt; t0 += P->d.max_instants) {
     const int n = std::min(P->d.max_instants, nt - t0);
@@ -1568,7 +1492,114 @@ int order2_dispatch(glw_plan* P, int nt, double* a, double* b, hipStream_t st) {
   return P->d.precision == GLW_PREC_FP32 ? order2_chunk<float>(P, nt, a, b, st) : order2_chunk<double>(P, nt, a, b, st);
 }
 
+// Per-stage time (ms, summed over launches) of the events recorded since the
+// pool was last reset; resets the pool.
+int collect_stage_ms(glw_plan* P, std::map<std::string, double>& agg) {
+  agg.clear();
+  if (P->ev_dropped) {
+    P->ev_used = P->ev_dropped = 0;
+    return fail(GLW_EINVAL, "event pool exhausted while timing stages (fewer instants or repetitions)");
+  }
+  if (P->ev_used) GLW_HIP(hipEventSynchronize(P->ev[P->ev_used - 1]));
+  for (size_t i = 0; i + 1 < P->ev_used; i += 2) {
+    float t = 0.f;
+    GLW_HIP(hipEventElapsedTime(&t, P->ev[i], P->ev[i + 1]));
+    agg[P->ev_name[i / 2]] += t;
+  }
+  P->ev_used = 0;
+  return GLW_OK;
+}
+
+int enable_events(glw_plan* P, bool on) {
+  for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
+  P->ev.clear();
+  P->ev_name.clear();
+  P->ev_used = P->ev_dropped = 0;
+  if (!on) return GLW_OK;
+  const size_t n = 8192;
+  P->ev.resize(n);
+  P->ev_name.resize(n / 2);
+  for (size_t i = 0; i < n; ++i) GLW_HIP(hipEventCreate(&P->ev[i]));
+  return GLW_OK;
+}
+
+// Candidate schedules, tried all at once: pass k gives every row stage
+// kRowSched[k] and every column stage kColSched[k % 4].
+constexpr Sched kRowSched[6] = {{0, 0}, {0, 1}, {0, 2}, {1, 0}, {1, 1}, {1, 2}};
+constexpr Sched kColSched[4] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+
+int autotune(glw_plan* P, int kind, const double* s0, const double* times, int nt, double* a, double* b, int reps,
+             hipStream_t st) {
+  int rc;
+  P->sched.clear();
+  P->stage_kind.clear();
+  if ((rc = glw_execute_kind(P, kind, s0, times, nt, a, b, st))) return rc;  // discovers the stages
+  GLW_HIP(hipStreamSynchronize(st));
+  std::vector<std::string> names;
+  for (const auto& kv : P->stage_kind)
+    if (kv.second == STAGE_ROW || kv.second == STAGE_COL) names.push_back(kv.first);
+  std::map<std::string, std::pair<double, Sched>> best;
+  std::map<std::string, double> agg;
+  for (int k = 0; k < 6; ++k) {
+    for (const std::string& nm : names)
+      P->sched[nm] = P->stage_kind[nm] == STAGE_ROW ? kRowSched[k] : kColSched[k % 4];
+    if ((rc = glw_execute_kind(P, kind, s0, times, nt, a, b, st))) return rc;  // warm-up, sizes scratch
+    GLW_HIP(hipStreamSynchronize(st));
+    P->ev_used = 0;
+    for (int r = 0; r < reps; ++r)
+      if ((rc = glw_execute_kind(P, kind, s0, times, nt, a, b, st))) return rc;
+    if ((rc = collect_stage_ms(P, agg))) return rc;
+    for (const std::string& nm : names) {
+      const double t = agg.count(nm) ? agg[nm] / reps : 0.0;
+      auto it = best.find(nm);
+      if (it == best.end() || t < it->second.first) best[nm] = {t, P->sched[nm]};
+    }
+  }
+  P->sched.clear();
+  for (const auto& kv : best) P->sched[kv.first] = kv.second.second;
+  return GLW_OK;
+}
+
 }  // namespace
+
+int glw_execute_kind(glw_plan* P, int kind, const double* s0, const double* times, int nt, double* a, double* b,
+                     hipStream_t st) {
+  if (!P) return fail(GLW_EINVAL, "null plan");
+  switch (kind) {
+    case GLW_KIND_LINEAR:
+      return run_chunks(P, s0, times, nt, a, b, st, linear_chunk);
+    case GLW_KIND_ORDER2:
+      if (P->d.order < 2) return fail(GLW_EINVAL, "plan built without order-2 tables");
+      return run_chunks(P, s0, times, nt, a, b, st, order2_dispatch);
+    case GLW_KIND_ORDER3:
+      if (P->d.order < 3) return fail(GLW_EINVAL, "plan built without order-3 tables");
+      return run_chunks(P, s0, times, nt, a, b, st, order3_dispatch);
+    case GLW_KIND_ORDER4:
+      if (!P->d.with_order4) return fail(GLW_EINVAL, "plan built without order-4 tables");
+      return run_chunks(P, s0, times, nt, a, nullptr, st, order4_dispatch);
+    default:
+      return fail(GLW_EINVAL, "unknown evaluation kind " + std::to_string(kind));
+  }
+}
+
+bool glw_plan_profiling(const glw_plan* P) { return P && !P->ev.empty(); }
+
+int glw_plan_release_scratch(glw_plan* P) {
+  if (P->split_ptr) GLW_HIP(hipFree(P->split_ptr));
+  P->split_ptr = nullptr;
+  P->split_bytes = 0;
+  return GLW_OK;
+}
+
+double glw_plan_cost(const glw_plan* P, int kind, int nt) {
+  const glw_plan_desc& d = P->d;
+  const double g2 = (double)d.g2x * d.g2y, g3 = (double)d.g3x * d.g3y, n2 = (double)d.nx * d.ny;
+  double c = 4.0 * n2;
+  if (kind == GLW_KIND_ORDER2) c += (8.0 * d.o2.eta_rank + 64.0 + 2.0) * g2;
+  if (kind == GLW_KIND_ORDER3) c += 16.0 * d.inner_rank * d.outer_rank * g2 + 25.0 * d.outer_rank * g3;
+  if (kind == GLW_KIND_ORDER4) c += 64.0 * d.o4.rank * d.o4.rank * d.o4.rank * g2;
+  return c * nt;
+}
 
 extern "C" {
 
@@ -1601,40 +1632,82 @@ int glw_plan_destroy(glw_plan* plan) {
 
 int glw_linear_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta11,
                        double* psi11, void* stream) {
-  return run_chunks(plan, s0, times, nt, eta11, psi11, stream, linear_chunk);
+  return glw_execute_kind(plan, GLW_KIND_LINEAR, s0, times, nt, eta11, psi11, (hipStream_t)stream);
 }
 
 int glw_order2_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta22,
                        double* psi22, void* stream) {
-  if (plan && plan->d.order < 2) return fail(GLW_EINVAL, "plan built without order-2 tables");
-  return run_chunks(plan, s0, times, nt, eta22, psi22, stream, order2_dispatch);
+  return glw_execute_kind(plan, GLW_KIND_ORDER2, s0, times, nt, eta22, psi22, (hipStream_t)stream);
 }
 
 int glw_order3_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta33,
                        double* psi33, void* stream) {
-  if (plan && plan->d.order < 3) return fail(GLW_EINVAL, "plan built without order-3 tables");
-  return run_chunks(plan, s0, times, nt, eta33, psi33, stream, order3_dispatch);
+  return glw_execute_kind(plan, GLW_KIND_ORDER3, s0, times, nt, eta33, psi33, (hipStream_t)stream);
 }
 
 int glw_order4_execute(glw_plan* plan, const double* s0, const double* times, int32_t nt, double* eta44,
                        void* stream) {
-  if (plan && !plan->d.with_order4) return fail(GLW_EINVAL, "plan built without order-4 tables");
-  return run_chunks(plan, s0, times, nt, eta44, nullptr, stream, order4_dispatch);
+  return glw_execute_kind(plan, GLW_KIND_ORDER4, s0, times, nt, eta44, nullptr, (hipStream_t)stream);
+}
+
+int glw_plan_autotune(glw_plan* plan, int32_t kind, const double* s0, const double* times, int32_t nt, double* a,
+                      double* b, int32_t reps, void* stream) {
+  if (!plan) return fail(GLW_EINVAL, "null plan");
+  if (reps < 1 || reps > 64) return fail(GLW_EINVAL, "reps must be 1..64");
+  if (nt < 1) return fail(GLW_EINVAL, "autotuning needs at least one instant");
+  hipStream_t st = (hipStream_t)stream;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  GLW_HIP(hipStreamIsCapturing(st, &cs));
+  if (cs != hipStreamCaptureStatusNone) return fail(GLW_EINVAL, "autotuning cannot run under graph capture");
+  const bool profiling = !plan->ev.empty();
+  int rc = enable_events(plan, true);
+  if (!rc) rc = autotune(plan, kind, s0, times, nt, a, b, reps, st);
+  // size the split-row scratch for the chosen schedule only
+  if (!rc) rc = glw_plan_release_scratch(plan);
+  if (!rc) rc = glw_execute_kind(plan, kind, s0, times, nt, a, b, st);
+  if (!rc) rc = hipStreamSynchronize(st) == hipSuccess ? GLW_OK : fail(GLW_EDEVICE, "stream synchronize failed");
+  const int rc2 = enable_events(plan, profiling);
+  return rc ? rc : rc2;
+}
+
+int glw_plan_get_schedule(glw_plan* plan, char* text, size_t len) {
+  if (!plan || !text || len < 1) return fail(GLW_EINVAL, "null argument");
+  std::string out;
+  for (const auto& kv : plan->sched)
+    out += (out.empty() ? "" : ";") + kv.first + "=" + std::to_string(kv.second.plan) + "/" +
+           std::to_string(kv.second.tile);
+  if (out.size() + 1 > len) return fail(GLW_EINVAL, "schedule text buffer too small");
+  memcpy(text, out.c_str(), out.size() + 1);
+  return GLW_OK;
+}
+
+int glw_plan_set_schedule(glw_plan* plan, const char* text) {
+  if (!plan || !text) return fail(GLW_EINVAL, "null argument");
+  std::map<std::string, Sched> sched;
+  std::string all(text);
+  size_t pos = 0;
+  while (pos < all.size()) {
+    size_t end = all.find(';', pos);
+    if (end == std::string::npos) end = all.size();
+    const std::string item = all.substr(pos, end - pos);
+    pos = end + 1;
+    if (item.empty()) continue;
+    const size_t eq = item.find('='), sl = item.find('/');
+    if (eq == std::string::npos || sl == std::string::npos || sl < eq || eq == 0)
+      return fail(GLW_EINVAL, "bad schedule entry '" + item + "' (want stage=plan/tile)");
+    const std::string pl = item.substr(eq + 1, sl - eq - 1), tl = item.substr(sl + 1);
+    if (pl.size() != 1 || tl.size() != 1 || pl[0] < '0' || pl[0] > '1' || tl[0] < '0' || tl[0] > '2')
+      return fail(GLW_EINVAL, "bad schedule entry '" + item + "' (plan 0..1, tile 0..2)");
+    sched[item.substr(0, eq)] = Sched{pl[0] - '0', tl[0] - '0'};
+  }
+  plan->sched = sched;
+  return GLW_OK;
 }
 
 int glw_plan_profile(glw_plan* plan, int32_t enable) {
   if (!plan) return fail(GLW_EINVAL, "null plan");
   GLW_HIP(hipSetDevice(plan->device));
-  for (hipEvent_t e : plan->ev) (void)hipEventDestroy(e);
-  plan->ev.clear();
-  plan->ev_name.clear();
-  plan->ev_used = plan->ev_dropped = 0;
-  if (!enable) return GLW_OK;
-  const size_t n = 8192;
-  plan->ev.resize(n);
-  plan->ev_name.resize(n / 2);
-  for (size_t i = 0; i < n; ++i) GLW_HIP(hipEventCreate(&plan->ev[i]));
-  return GLW_OK;
+  return enable_events(plan, enable != 0);
 }
 
 int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_len, double* ms,

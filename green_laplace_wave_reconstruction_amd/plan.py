@@ -104,14 +104,21 @@ class Box:
 
 
 def mode_grids(ny: int, nx: int):
-    return np.meshgrid(np.fft.fftfreq(nx) * nx, np.fft.fftfreq(ny) * ny)
+    """Signed integer mode numbers (mx, my) of a [ny, nx] FFT grid.
+
+    ``fftfreq(n) * n`` is not exact for n that are not powers of two
+    (28/192*192 = 27.999999999999996), so the products are rounded.
+    """
+    mx = np.rint(np.fft.fftfreq(nx) * nx).astype(np.int64)
+    my = np.rint(np.fft.fftfreq(ny) * ny).astype(np.int64)
+    return np.meshgrid(mx, my)
 
 
 def box_of(mask: np.ndarray) -> Box:
     ny, nx = mask.shape
     mx, my = mode_grids(ny, nx)
-    mxs = mx[mask].astype(int)
-    mys = my[mask].astype(int)
+    mxs = mx[mask]
+    mys = my[mask]
     return Box(int(mxs.min()), int(mys.min()), int(mxs.max() - mxs.min() + 1), int(mys.max() - mys.min() + 1))
 
 

@@ -117,3 +117,74 @@ def order4_dimensionless_256(seed: int = 5) -> SeaState:
     length = 5.0 * 2.0 * math.pi / kp
     return jonswap_sea_state(nx=256, ny=256, lx=length, ly=length, depth=1.0, hs=0.24 / kp, tp=tp, gamma=3.3,
                              spread_s=10.0, gravity=1.0, seed=seed)
+
+
+BATCH_GRIDS = (64, 128, 192, 256, 384, 512, 1024)
+
+
+@dataclass(frozen=True)
+class BatchState:
+    """One draw of the uneven batch: the sea state, its GL order and the draw itself."""
+
+    state: SeaState
+    order: int
+    kp_h: float
+    hs: float
+    tp: float
+    gamma: float
+    spread_s: float
+    theta0_drawn: float
+    theta0: float
+    seed: int
+
+
+def fold_direction(theta0: float) -> float:
+    """Map a mean direction into (-pi/4, pi/4] by the symmetries of a square grid.
+
+    Rotations by quarter turns and the flips x -> -x, y -> -y, x <-> y map the
+    square FFT grid onto itself; the GL kernels depend only on |k| and on dot
+    products of wavenumbers, so a sea state and its image give image fields.
+    """
+    q = 0.5 * math.pi
+    return 0.25 * math.pi - math.fmod(math.fmod(0.25 * math.pi - theta0, q) + q, q)
+
+
+def batch_sea_states(count: int = 256, seed: int = 3, *, gravity: float = 9.81, kmax_factor: float = 4.0,
+                     fill: float = 0.9) -> list[BatchState]:
+    """Seeded uneven batch of directional sea states for GL orders 2 and 3.
+
+    Draws (``numpy.random.default_rng(seed)``, in this order per state): grid
+    N from ``BATCH_GRIDS``, order in {2, 3}, kp h ~ U[0.5, 5], Hs ~ U[1, 8] m,
+    Tp ~ U[6, 16] s, gamma ~ U[1, 5], s ~ U[2, 30], theta0 ~ U[0, 2 pi).
+    Then: depth h from the dispersion relation at (Tp, kp h); theta0 folded
+    into (-45, 45] degrees (``fold_direction``) so that the strict-forward
+    half plane kx > 0 holds the spectrum; square domain
+    L = fill * N pi / (order kmax) with kmax = kmax_factor kp, so that
+    order kmax / dk = fill N / 2 < N / 2 (no aliasing of the order-n
+    products onto the grid); parents with kh <= 0.5 (order 3) or kh < 0.3
+    (order 2) dropped; component phases from seed ``seed * 1000 + i``.
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.choice(BATCH_GRIDS))
+        order = int(rng.integers(2, 4))
+        kp_h = float(rng.uniform(0.5, 5.0))
+        hs = float(rng.uniform(1.0, 8.0))
+        tp = float(rng.uniform(6.0, 16.0))
+        gamma = float(rng.uniform(1.0, 5.0))
+        spread = float(rng.uniform(2.0, 30.0))
+        theta_drawn = float(rng.uniform(0.0, 2.0 * math.pi))
+        omega_p = 2.0 * math.pi / tp
+        depth = gravity * kp_h * math.tanh(kp_h) / omega_p**2
+        kp = kp_h / depth
+        length = fill * n * math.pi / (order * kmax_factor * kp)
+        theta = fold_direction(theta_drawn)
+        # order 3 keeps kh > 0.5; order 2 keeps kh >= 0.3 (strict test with the bound nudged down)
+        min_kh = 0.5 if order == 3 else 0.3 * (1.0 - 1e-12)
+        st = jonswap_sea_state(nx=n, ny=n, lx=length, ly=length, depth=depth, hs=hs, tp=tp, gamma=gamma,
+                               spread_s=spread, theta0=theta, gravity=gravity, kmax_factor=kmax_factor,
+                               min_kh=min_kh, seed=seed * 1000 + i)
+        out.append(BatchState(state=st, order=order, kp_h=kp_h, hs=hs, tp=tp, gamma=gamma, spread_s=spread,
+                              theta0_drawn=theta_drawn, theta0=theta, seed=seed * 1000 + i))
+    return out

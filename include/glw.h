@@ -49,6 +49,12 @@ extern "C" {
 #define GLW_PREC_FP64 0 /* FFTs and products in FP64 */
 #define GLW_PREC_FP32 1 /* FFTs and products in FP32, FP64 accumulation */
 
+/* Evaluation kinds (glw_plan_autotune, glw_batch_item). */
+#define GLW_KIND_LINEAR 1 /* eta11, psi11 */
+#define GLW_KIND_ORDER2 2 /* eta22, psi22 */
+#define GLW_KIND_ORDER3 3 /* eta33, psi33 */
+#define GLW_KIND_ORDER4 4 /* eta44 */
+
 typedef struct glw_plan_s glw_plan;
 
 /* Rectangle of signed mode numbers: mx0..mx0+wx-1, my0..my0+wy-1. */
@@ -182,6 +188,48 @@ int glw_wit_sector44(glw_wit* wit, const double* amplitudes, int32_t cases, doub
 int glw_plan_profile(glw_plan* plan, int32_t enable);
 int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_len, double* ms,
                          int32_t* launches, int32_t* count);
+
+/* FFT schedule autotuner (no reference counterpart: the reference runs
+ * FFTW/MATLAB/NumPy FFTs whose planning is internal).  Runs `kind` of the
+ * plan (reps timed repetitions per candidate, plus warm-ups) with every
+ * candidate schedule -- radix sequence of the LDS FFTs (two-pass composite
+ * radices or greedy radix 8/4/2/3) x tile (column tile width; fused row
+ * kernel with the widest or a one-row tile, or the split row path) -- times
+ * every stage with HIP events and keeps, per stage, the fastest.  The
+ * outputs a, b are overwritten.  Synchronizes the stream; not capturable. */
+int glw_plan_autotune(glw_plan* plan, int32_t kind, const double* s0, const double* times, int32_t nt, double* a,
+                      double* b, int32_t reps, void* stream);
+/* The plan's per-stage schedule as "stage=plan/tile;..." (stages absent from
+ * the text run plan 0 / tile 0), and the reverse.  Lets one tuning serve many
+ * plans of the same grids. */
+int glw_plan_get_schedule(glw_plan* plan, char* text, size_t len);
+int glw_plan_set_schedule(glw_plan* plan, const char* text);
+
+/* Batched evaluation of many plans in one HIP graph (no reference
+ * counterpart: the reference evaluates one sea state per call of
+ * src/gl_spectral_surface.m / python/src/green_laplace/spatial.py).  Every
+ * item is one execute call; glw_batch_create captures all of them into one
+ * graph with `branches` parallel chains (items of one plan share a chain,
+ * chains are balanced by estimated work), and glw_batch_launch replays the
+ * whole batch with a single hipGraphLaunch.  Pointers are captured by value:
+ * inputs are read and outputs written at every launch.  Each plan must have
+ * run once eagerly with its current schedule (sizes its scratch) and must not
+ * have stage profiling enabled. */
+typedef struct glw_batch_item {
+  glw_plan* plan;
+  int32_t kind; /* GLW_KIND_* */
+  int32_t nt;
+  const double* s0;    /* device, as for the execute calls */
+  const double* times; /* device double[nt] */
+  double* a;           /* device complex128 [nt][ny][nx] */
+  double* b;           /* device complex128 [nt][ny][nx] (unused for GLW_KIND_ORDER4) */
+} glw_batch_item;
+
+typedef struct glw_batch_s glw_batch;
+
+int glw_batch_create(const glw_batch_item* items, int32_t n, int32_t branches, glw_batch** batch);
+int glw_batch_launch(glw_batch* batch, void* stream);
+int glw_batch_destroy(glw_batch* batch);
 
 /* Device pointer and size of a named workspace buffer (for stage-level tests). */
 int glw_plan_buffer(glw_plan* plan, const char* name, void** ptr, size_t* bytes);

@@ -140,3 +140,15 @@ def test_coefficients_pack_like_reference():
     assert c["peak_wavenumber"] == pytest.approx(np.hypot(3, 1))
     assert c["peak_depth"] == pytest.approx(2 * np.hypot(3, 1))
     np.testing.assert_allclose(c["omega"], np.sqrt(9.81 * np.hypot([2, 3], [0, 1]) * np.tanh(2 * np.hypot([2, 3], [0, 1]))))
+
+
+def test_mode_numbers_exact_on_radix3_grids():
+    """fftfreq(n) * n is inexact for n = 3 * 2^k (28/192*192 = 27.999...); the plan's mode grids are integers
+    and a box built from a mask reaches its outermost mode."""
+    for n in PL.FFT_LENGTHS:
+        mx, my = PL.mode_grids(n, n)
+        ideal = np.where(np.arange(n) < n // 2, np.arange(n), np.arange(n) - n)
+        assert mx.dtype.kind == "i" and np.array_equal(mx[0], ideal) and np.array_equal(my[:, 0], ideal)
+    mask = np.zeros((192, 192), dtype=bool)
+    mask[[-28, 0, 28], [28, 1, 27]] = True
+    assert PL.box_of(mask) == PL.Box(1, -28, 28, 57)
