@@ -61,7 +61,7 @@ class CPlanDesc(ctypes.Structure):
 #: every symbol include/glw.h declares
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
            "glw_order3_execute", "glw_order4_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
-           "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_batch_create",
+           "glw_adjoint_execute", "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_batch_create",
            "glw_batch_launch", "glw_batch_destroy", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
            "glw_fft_length_supported", "glw_last_error", "glw_version")
 
@@ -95,6 +95,8 @@ def load_library(path: str | os.PathLike | None = None):
         fn.restype = c_int
     lib.glw_order4_execute.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p]
     lib.glw_order4_execute.restype = c_int
+    lib.glw_adjoint_execute.argtypes = [c_void_p, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p]
+    lib.glw_adjoint_execute.restype = c_int
     lib.glw_plan_autotune.argtypes = [c_void_p, c_int32, c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_int32,
                                       c_void_p]
     lib.glw_plan_autotune.restype = c_int
@@ -290,6 +292,32 @@ class DevicePlan:
                                           c_void_p(b.data_ptr() if b is not None else None), int(reps),
                                           c_void_p(stream)))
         return self.schedule
+
+    def adjoint(self, grad_field, times=(0.0,), orders: int = 7, recompute: bool = True, out=None):
+        """dL/dRe S0 + i dL/dIm S0 on the parent box for a field gradient dL/dRe eta+ (+ i dL/dIm eta+).
+
+        ``grad_field``: complex128 (or real float64) [ny, nx] device tensor; ``orders``: bit mask
+        1 eta11, 2 eta22, 4 eta33 of the components that make up eta+.
+        """
+        import torch
+
+        t = self.times_tensor(times)
+        if int(t.numel()) != 1:
+            raise ValueError("the adjoint evaluates one instant")
+        g = grad_field
+        if not torch.is_complex(g):
+            g = g.to(torch.float64).to(torch.complex128)
+        g = g.to(device=self.device, dtype=torch.complex128).contiguous()
+        if g.shape != (self.host.ny, self.host.nx):
+            raise ValueError(f"grad_field must be [{self.host.ny}, {self.host.nx}]")
+        shape = (self.host.box1.wy, self.host.box1.wx)
+        if out is None:
+            out = torch.empty(shape, dtype=torch.complex128, device=self.device)
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        _check(self.lib.glw_adjoint_execute(self._handle, c_void_p(self.s0.data_ptr()), c_void_p(t.data_ptr()),
+                                            int(orders), c_void_p(g.data_ptr()), c_void_p(out.data_ptr()),
+                                            int(bool(recompute)), c_void_p(stream)))
+        return out
 
     @property
     def schedule(self) -> str:

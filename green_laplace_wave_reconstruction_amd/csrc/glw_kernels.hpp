@@ -194,8 +194,10 @@ __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __res
 // Pruned forward y-pass, support mask, scale and support max |x|.
 // ----------------------------------------------------------------------------
 struct ColFwdArgs {
-  const void* in;           // cpx<T> [task][nsrc][gy][istride]
+  const void* in;           // cpx<T> [task][in_nfield][gy][istride], sources from field in_foff
   int nsrc, istride;
+  int in_nfield = 0;        // fields per task in `in` (0: nsrc)
+  int in_foff = 0;
   int omx0, owx, omy0, owy; // output box (columns = the row kernel's output box)
   const uint8_t* mask;      // [owy*owx]
   double scale;
@@ -225,7 +227,8 @@ __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __res
   const int col0 = blockIdx.x * CW;
   const int s = blockIdx.y;
   const int task = blockIdx.z;
-  const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + ((size_t)task * a.nsrc + s) * GY * a.istride;
+  const int nf = a.in_nfield ? a.in_nfield : a.nsrc;
+  const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + ((size_t)task * nf + a.in_foff + s) * GY * a.istride;
   for (int i = threadIdx.x; i < CW * GY; i += NT) {
     const int y = i / CW;
     const int c = i - y * CW;
@@ -264,6 +267,7 @@ struct RowArgs {
   int nout, ostride;
   int omx0, owx;
   int gy;
+  int out_per_field;        // k_rowfwd: output s is pruned to box (fmx0[s], fwx[s]) instead of (omx0, owx)
 };
 
 template <typename T, int GX, int NIN, int NOUT, int RY, class Op, int PLAN = 0>
@@ -380,11 +384,32 @@ __global__ __launch_bounds__(NT) void k_rowfwd(RowArgs a, const cpx<T>* __restri
   __syncthreads();
   fft_lds<T, GX, RY, SS, NT, -1, PLAN>(buf, twl);
   cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) + ((size_t)task * a.nout + s) * a.gy * a.ostride;
-  for (int i = threadIdx.x; i < RY * a.owx; i += NT) {
-    const int r = i / a.owx;
-    const int j = i - r * a.owx;
-    out[(size_t)(y0 + r) * a.ostride + j] = buf[lds_at<GX>(r, wrap(a.omx0 + j, GX))];
+  const int owx = a.out_per_field ? a.fwx[s] : a.owx;
+  const int omx0 = a.out_per_field ? a.fmx0[s] : a.omx0;
+  for (int i = threadIdx.x; i < RY * owx; i += NT) {
+    const int r = i / owx;
+    const int j = i - r * owx;
+    out[(size_t)(y0 + r) * a.ostride + j] = buf[lds_at<GX>(r, wrap(omx0 + j, GX))];
   }
+}
+
+// Adjoint of the pointwise products: v [task][NIN][npts] (forward inputs),
+// g [task][NG][npts] (output gradients) -> gv [task][NIN][npts].
+template <typename T, int NIN, int NG, class Op>
+__global__ __launch_bounds__(NT) void k_pointadj(const cpx<T>* __restrict__ v, const cpx<T>* __restrict__ g,
+                                                 cpx<T>* __restrict__ gv, int npts) {
+  static_assert(NG == Op::NADJ, "gradient count must match the operator's adjoint");
+  const int p = blockIdx.x * NT + threadIdx.x;
+  const int task = blockIdx.y;
+  if (p >= npts) return;
+  cpx<T> f[NIN], gg[NG], o[NIN];
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) f[i] = v[((size_t)task * NIN + i) * npts + p];
+#pragma unroll
+  for (int s = 0; s < NG; ++s) gg[s] = g[((size_t)task * NG + s) * npts + p];
+  Op::adjoint(f, gg, o);
+#pragma unroll
+  for (int i = 0; i < NIN; ++i) gv[((size_t)task * NIN + i) * npts + p] = o[i];
 }
 
 // ----------------------------------------------------------------------------

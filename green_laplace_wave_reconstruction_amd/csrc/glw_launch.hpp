@@ -58,4 +58,12 @@ int launch_rowfwd(int gx, const RowArgs& a, int ntask, const void* tw, int plan,
 template <typename T, int NIN, int NOUT, class Op>
 int launch_row(int gx, const RowArgs& a, int ntask, const void* tw, const Exec& x, hipStream_t st);
 
+// Adjoint of launch_row<T, NIN, *, Op> for the first NG outputs: `fwd` holds
+// the forward inputs (as given to launch_row), `gin` the output-box
+// gradients (nin = NG fields, boxes in fmx0/fwx), `gout` receives the input
+// gradients (nout = NIN, out_per_field boxes).  Runs the split path.
+template <typename T, int NIN, int NG, class Op>
+int launch_row_adjoint(int gx, const RowArgs& fwd, const RowArgs& gin, const RowArgs& gout, int ntask,
+                       const void* tw, const Exec& x, hipStream_t st);
+
 }  // namespace glw

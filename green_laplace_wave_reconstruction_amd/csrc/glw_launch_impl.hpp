@@ -226,4 +226,26 @@ int launch_row(int gx, const RowArgs& a, int ntask, const void* tw, const Exec& 
   return GLW_OK;
 }
 
+template <typename T, int NIN, int NG, class Op>
+int launch_row_adjoint(int gx, const RowArgs& fwd, const RowArgs& gin, const RowArgs& gout, int ntask,
+                       const void* tw, const Exec& x, hipStream_t st) {
+  if (fwd.nin != NIN || gin.nin != NG || gout.nout != NIN || !gout.out_per_field)
+    return glw_report_error(GLW_EINVAL, "row adjoint: field counts do not match the operator");
+  const size_t npts = (size_t)fwd.gy * gx;
+  const size_t fb = (size_t)ntask * npts * sizeof(cpx<T>);
+  if (!x.scratch) return glw_report_error(GLW_EINVAL, "row adjoint needs a scratch provider");
+  void* ws = nullptr;
+  int rc = x.scratch(x.ctx, fb * (2 * NIN + NG), &ws);
+  if (rc) return rc;
+  char* pv = (char*)ws;
+  char* pg = pv + fb * NIN;
+  char* pgv = pg + fb * NG;
+  if ((rc = launch_rowinv<T>(gx, fwd, ntask, tw, x.s.plan, pv, st))) return rc;
+  if ((rc = launch_rowinv<T>(gx, gin, ntask, tw, x.s.plan, pg, st))) return rc;
+  hipLaunchKernelGGL((k_pointadj<T, NIN, NG, Op>), dim3((unsigned)((npts + NT - 1) / NT), ntask), dim3(NT), 0, st,
+                     (const cpx<T>*)pv, (const cpx<T>*)pg, (cpx<T>*)pgv, (int)npts);
+  GLW_HIP_CHECK(hipGetLastError());
+  return launch_rowfwd<T>(gx, gout, ntask, tw, x.s.plan, pgv, st);
+}
+
 }  // namespace glw

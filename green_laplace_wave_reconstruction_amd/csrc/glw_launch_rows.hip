@@ -10,10 +10,13 @@
 
 #if GLW_ROW_SET == 0
 #define GLW_ROW_OPS(X) X(12, 4, InnerOp)
+#define GLW_ADJ_OPS(X) X(12, 4, InnerOp)
 #elif GLW_ROW_SET == 1
 #define GLW_ROW_OPS(X) X(21, 4, OuterOp)
+#define GLW_ADJ_OPS(X) X(21, 2, OuterOp)
 #elif GLW_ROW_SET == 2
 #define GLW_ROW_OPS(X) X(8, 2, Eta22Op) X(2, 1, ContactOp)
+#define GLW_ADJ_OPS(X) X(8, 2, Eta22Op)
 #elif GLW_ROW_SET == 3
 #define GLW_ROW_OPS(X) X(16, 8, Psi22Op)
 #elif GLW_ROW_SET == 4
@@ -41,5 +44,14 @@ GLW_EXTERN(float)
   template int launch_row<float, NIN, NOUT, OP>(int, const RowArgs&, int, const void*, const Exec&, hipStream_t);
 GLW_ROW_OPS(GLW_INST)
 #undef GLW_INST
+
+// adjoints (FP64 only)
+#ifdef GLW_ADJ_OPS
+#define GLW_INST(NIN, NG, OP)                                                                               \
+  template int launch_row_adjoint<double, NIN, NG, OP>(int, const RowArgs&, const RowArgs&, const RowArgs&, int, \
+                                                       const void*, const Exec&, hipStream_t);
+GLW_ADJ_OPS(GLW_INST)
+#undef GLW_INST
+#endif
 
 }  // namespace glw

@@ -13,6 +13,13 @@ template <typename T>
 __device__ __forceinline__ cpx<T> sub(cpx<T> a, cpx<T> b) { return csub(a, b); }
 template <typename T>
 __device__ __forceinline__ cpx<T> scl(cpx<T> a, T s) { return cscale(a, s); }
+// conj(d) * g: one term of a holomorphic Jacobian-transpose product
+template <typename T>
+__device__ __forceinline__ cpx<T> cjm(cpx<T> d, cpx<T> g) { return {d.x * g.x + d.y * g.y, d.x * g.y - d.y * g.x}; }
+
+// Adjoints: for outputs o_s = Op_s(f) (holomorphic polynomials) and output
+// gradients g_s (d/dRe + i d/dIm of a real loss), gf_i = sum_s conj(do_s/df_i) g_s.
+// NADJ is the number of outputs whose gradients the adjoint takes.
 
 // Order-3 inner pair sources (python/src/green_laplace/_order3.py:216-233).
 // in: v, v_nu, v_nu2, v_nu3, hx, hy, jx, jy, radial, q2_field, nu_jx, nu_jy
@@ -28,6 +35,26 @@ struct InnerOp {
     o[2] = sub(sub(add(scl(mul(v, vn3), T(2)), scl(mul(vn, vn2), T(4))), scl(hxjx, T(2))), scl(hyjy, T(2)));
     o[3] = add(scl(add(mul(vn, rad), mul(v, q2f)), T(2)),
                scl(add(add(mul(jx, jx), mul(hx, njx)), add(mul(jy, jy), mul(hy, njy))), T(2)));
+  }
+  static constexpr int NADJ = 4;
+  template <typename T>
+  __device__ __forceinline__ static void adjoint(const cpx<T>* f, const cpx<T>* g, cpx<T>* gf) {
+    const cpx<T> v = f[0], vn = f[1], vn2 = f[2], vn3 = f[3], hx = f[4], hy = f[5];
+    const cpx<T> jx = f[6], jy = f[7], rad = f[8], q2f = f[9], njx = f[10], njy = f[11];
+    const cpx<T> G0 = scl(g[0], T(2)), G1 = scl(g[1], T(2)), G2 = scl(g[2], T(2)), G3 = scl(g[3], T(2));
+    const cpx<T> G12 = sub(G1, G2);  // hx, hy, jx, jy enter o1 and o2 with opposite signs
+    gf[0] = add(add(cjm(vn2, G0), cjm(rad, G1)), add(cjm(vn3, G2), cjm(q2f, G3)));
+    gf[1] = add(add(cjm(vn, G0), cjm(scl(vn2, T(2)), G2)), cjm(rad, G3));
+    gf[2] = add(cjm(v, G0), cjm(scl(vn, T(2)), G2));
+    gf[3] = cjm(v, G2);
+    gf[4] = add(sub(cjm(jx, G12), cjm(hx, G0)), cjm(njx, G3));
+    gf[5] = add(sub(cjm(jy, G12), cjm(hy, G0)), cjm(njy, G3));
+    gf[6] = add(cjm(hx, G12), cjm(scl(jx, T(2)), G3));
+    gf[7] = add(cjm(hy, G12), cjm(scl(jy, T(2)), G3));
+    gf[8] = add(cjm(v, G1), cjm(vn, G3));
+    gf[9] = cjm(v, G3);
+    gf[10] = cjm(hx, G3);
+    gf[11] = cjm(hy, G3);
   }
 };
 
@@ -61,6 +88,43 @@ struct OuterOp {
                    scl(mul(mul(fe, fet), fpzz), T(0.25))),
                scl(mul(fe2, fpzzt), T(0.125)));
   }
+  // Gradients of forcing_k (g[0]) and forcing_d (g[1]) only: eta33 is built
+  // from those two; the contacts feed psi33 alone.
+  static constexpr int NADJ = 2;
+  template <typename T>
+  __device__ __forceinline__ static void adjoint(const cpx<T>* f, const cpx<T>* g, cpx<T>* gf) {
+    const cpx<T> fe = f[0], fex = f[1], fey = f[2], fpx = f[3], fpy = f[4], fpz = f[5];
+    const cpx<T> fpxz = f[6], fpyz = f[7], fpzz = f[8], fpzzz = f[9], fptz = f[10], fptzz = f[11];
+    const cpx<T> pe = f[12], pex = f[13], pey = f[14], ppx = f[15], ppy = f[16], ppz = f[17];
+    const cpx<T> ppzz = f[18], pptz = f[19];
+    const cpx<T> gk = g[0], gd = g[1];
+    const cpx<T> hk = scl(gk, T(0.5)), hd = scl(gd, T(0.5));      // pair terms carry 1/2
+    const cpx<T> qk = scl(gk, T(0.25)), qd = scl(gd, T(0.25));    // direct terms carry 1/4
+    const cpx<T> fe2 = mul(fe, fe);
+    const cpx<T> dfe_k = sub(add(mul(fpxz, fex), mul(fpyz, fey)), mul(fe, fpzzz));
+    const cpx<T> dfe_d = add(mul(fe, fptzz), add(add(mul(fpx, fpxz), mul(fpy, fpyz)), mul(fpz, fpzz)));
+    gf[0] = add(add(cjm(dfe_k, qk), cjm(dfe_d, qd)), sub(cjm(pptz, hd), cjm(ppzz, hk)));
+    gf[1] = add(cjm(mul(fe, fpxz), qk), cjm(ppx, hk));
+    gf[2] = add(cjm(mul(fe, fpyz), qk), cjm(ppy, hk));
+    gf[3] = add(cjm(pex, hk), add(cjm(mul(fe, fpxz), qd), cjm(ppx, hd)));
+    gf[4] = add(cjm(pey, hk), add(cjm(mul(fe, fpyz), qd), cjm(ppy, hd)));
+    gf[5] = add(cjm(mul(fe, fpzz), qd), cjm(ppz, hd));
+    gf[6] = add(cjm(mul(fe, fex), qk), cjm(mul(fe, fpx), qd));
+    gf[7] = add(cjm(mul(fe, fey), qk), cjm(mul(fe, fpy), qd));
+    gf[8] = sub(cjm(mul(fe, fpz), qd), cjm(pe, hk));
+    gf[9] = scl(cjm(fe2, gk), T(-0.125));
+    gf[10] = cjm(pe, hd);
+    gf[11] = scl(cjm(fe2, gd), T(0.125));
+    gf[12] = sub(cjm(fptz, hd), cjm(fpzz, hk));
+    gf[13] = cjm(fpx, hk);
+    gf[14] = cjm(fpy, hk);
+    gf[15] = add(cjm(fex, hk), cjm(fpx, hd));
+    gf[16] = add(cjm(fey, hk), cjm(fpy, hd));
+    gf[17] = cjm(fpz, hd);
+    gf[18] = scl(cjm(fe, gk), T(-0.5));
+    gf[19] = cjm(fe, hd);
+    gf[20] = {T(0), T(0)};
+  }
 };
 
 // eta22 sources (spatial.py:196-205).
@@ -71,6 +135,20 @@ struct Eta22Op {
     const cpx<T> v = f[0], vn = f[1], vn2 = f[2], hx = f[3], hy = f[4], rad = f[5], jx = f[6], jy = f[7];
     o[0] = sub(sub(add(scl(mul(v, vn2), T(2)), mul(vn, vn)), mul(hx, hx)), mul(hy, hy));
     o[1] = add(scl(mul(v, rad), T(2)), scl(add(mul(hx, jx), mul(hy, jy)), T(2)));
+  }
+  static constexpr int NADJ = 2;
+  template <typename T>
+  __device__ __forceinline__ static void adjoint(const cpx<T>* f, const cpx<T>* g, cpx<T>* gf) {
+    const cpx<T> v = f[0], vn = f[1], vn2 = f[2], hx = f[3], hy = f[4], rad = f[5], jx = f[6], jy = f[7];
+    const cpx<T> G0 = scl(g[0], T(2)), G1 = scl(g[1], T(2));
+    gf[0] = add(cjm(vn2, G0), cjm(rad, G1));
+    gf[1] = cjm(vn, G0);
+    gf[2] = cjm(v, G0);
+    gf[3] = sub(cjm(jx, G1), cjm(hx, G0));
+    gf[4] = sub(cjm(jy, G1), cjm(hy, G0));
+    gf[5] = cjm(v, G1);
+    gf[6] = cjm(hx, G1);
+    gf[7] = cjm(hy, G1);
   }
 };
 

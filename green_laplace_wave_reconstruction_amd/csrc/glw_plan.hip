@@ -380,6 +380,143 @@ __global__ __launch_bounds__(NT) void k_acc_quartic44(Acc4Args g) {
 }
 
 // ----------------------------------------------------------------------------
+// Adjoint kernels (gradients of a real loss of the eta fields; a gradient of
+// a complex value z is dL/dRe z + i dL/dIm z).  Supports, floors and masks of
+// the forward graph are held fixed (piecewise-constant in the amplitudes).
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ cpx<double> conj_mul(cpx<double> m, cpx<double> g) {
+  return {m.x * g.x + m.y * g.y, m.x * g.y - m.y * g.x};
+}
+__device__ __forceinline__ bool kept(cpx<double> x, double level) { return !(hypot(x.x, x.y) < level); }
+
+// dst (box d) = / += scale * src (box u), where box u contains box d
+__global__ void k_adj_gather(const cpx<double>* src, glw_box u, glw_box d, double scale, int add, cpx<double>* dst) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= d.wx * d.wy) return;
+  const int r = b / d.wx, c = b - r * d.wx;
+  const cpx<double> v = src[(size_t)(d.my0 + r - u.my0) * u.wx + (d.mx0 + c - u.mx0)];
+  const cpx<double> w = {scale * v.x, scale * v.y};
+  dst[b] = add ? cadd(dst[b], w) : w;
+}
+
+// eta22 accumulation adjoint: gE [box] -> gS [J][2][box] (sd, sk)
+__global__ __launch_bounds__(NT) void k_adj_acc_eta22(AccEta22Args g, const cpx<double>* ge, const uint8_t* mask,
+                                                      cpx<double>* gs) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int j = blockIdx.y;
+  if (b >= g.box) return;
+  const size_t task = j;
+  cpx<double> gd = {0, 0}, gk = {0, 0};
+  if (mask[b]) {
+    const cpx<double> e = ge[b];
+    const double A = g.a[b], c = g.coef[j], chv = g.ch[(size_t)j * g.box + b], shv = g.sh[(size_t)j * g.box + b];
+    if (kept(g.spec[(task * 2) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2])))
+      gd = {-c * A * shv * e.x, -c * A * shv * e.y};
+    if (kept(g.spec[(task * 2 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2 + 1])))
+      gk = {c * chv * e.x, c * chv * e.y};
+  }
+  gs[(task * 2) * g.box + b] = gd;
+  gs[(task * 2 + 1) * g.box + b] = gk;
+}
+
+// eta33 outer accumulation adjoint: gE [box] -> gS [O][2][box] (forcing_k, forcing_d)
+__global__ __launch_bounds__(NT) void k_adj_acc_outer(AccOuterArgs g, const cpx<double>* ge, cpx<double>* gs) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int o = blockIdx.y;
+  if (b >= g.box) return;
+  cpx<double> gk = {0, 0}, gd = {0, 0};
+  if (g.mask[b]) {
+    const cpx<double> e = ge[b];
+    const double A = g.a[b], c = g.coef[o], chv = g.ch[(size_t)o * g.box + b], shv = g.sh[(size_t)o * g.box + b];
+    // eta += c (A sh F_D - i ch F_K)
+    if (kept(g.spec[((size_t)o * 4) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)o * 4])))
+      gk = {-c * chv * e.y, c * chv * e.x};  // conj(-i c ch) e = i c ch e
+    if (kept(g.spec[((size_t)o * 4 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)o * 4 + 1])))
+      gd = {c * A * shv * e.x, c * A * shv * e.y};
+  }
+  gs[((size_t)o * 2) * g.box + b] = gk;
+  gs[((size_t)o * 2 + 1) * g.box + b] = gd;
+}
+
+// Inner accumulation adjoint: gpair [O][4][box] (eta, eta_t, phi, phi_t) -> gS [O*I][4][box]
+__global__ __launch_bounds__(NT) void k_adj_acc_inner(AccInnerArgs g, const cpx<double>* gp, cpx<double>* gs) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int task = blockIdx.y;  // o * I + i
+  if (b >= g.box) return;
+  const int o = task / g.I, i = task - o * g.I;
+  cpx<double> r[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  if (g.mask[b]) {
+    const cpx<double>* G = gp + (size_t)o * 4 * g.box + b;
+    const cpx<double> ge = G[0], get = G[g.box], gph = G[2 * (size_t)g.box], gpt = G[3 * (size_t)g.box];
+    const double A = g.a[b], c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
+    // e = c(-A sh s0 + ch s1), et = -i c(-A sh s2 + ch s3), p = i c(ch s0 - sh s1), pt = c(ch s2 - sh s3)
+    r[0] = {c * (-A * shv * ge.x + chv * gph.y), c * (-A * shv * ge.y - chv * gph.x)};
+    r[1] = {c * (chv * ge.x - shv * gph.y), c * (chv * ge.y + shv * gph.x)};
+    r[2] = {c * (A * shv * get.y + chv * gpt.x), c * (-A * shv * get.x + chv * gpt.y)};
+    r[3] = {c * (-chv * get.y - shv * gpt.x), c * (chv * get.x - shv * gpt.y)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (!kept(g.spec[((size_t)task * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)task * 4 + k])))
+        r[k] = {0, 0};
+  }
+#pragma unroll
+  for (int k = 0; k < 4; ++k) gs[((size_t)task * 4 + k) * g.box + b] = r[k];
+}
+
+// Adjoint of a column-inverse pass (k_colinv) after its y-DFT: x [ntask][nfield][box]
+// holds DFT_y of the column gradients; out[set][src] += sum over the set's tasks
+// and fields f of conj(multiplier_f) * damping * fscale_f * x (floored bins excluded).
+__global__ __launch_bounds__(NT) void k_adj_collect(ColInvArgs a, const cpx<double>* x, cpx<double>* out) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  const int set = blockIdx.y;
+  const int box = a.wx * a.wy;
+  if (b >= box) return;
+  cpx<double> acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+  bool keep[4] = {true, true, true, true};
+  for (int s = 0; s < a.nsrc && s < 4; ++s)
+    if (a.fmax && ((a.floor_src_mask >> s) & 1)) {
+      const size_t k = (size_t)set * a.fmax_stride + (a.fmax_stride == 1 ? 0 : s);
+      keep[s] = kept(a.spec[((size_t)set * a.nsrc + s) * box + b], a.floor_level * bits_to_d(a.fmax[k]));
+    }
+  for (int u = 0; u < a.sdiv; ++u) {
+    const int task = set * a.sdiv + u;
+    const double dmp = a.damp ? a.damp[(size_t)((task % a.dmod) * a.dmul + a.dadd) * box + b] : 1.0;
+    for (int f = 0; f < a.nfield; ++f) {
+      const cpx<double> m = apply_mul(a.fcode[f], cpx<double>{1.0, 0.0}, a.tab, b);
+      const cpx<double> v = x[((size_t)task * a.nfield + f) * box + b];
+      const cpx<double> w = conj_mul(m, v);
+      const double sc = dmp * a.fscale[f];
+      const int s = a.fsrc[f];
+      acc[s] = {acc[s].x + sc * w.x, acc[s].y + sc * w.y};
+    }
+  }
+  for (int s = 0; s < a.nsrc && s < 4; ++s) {
+    cpx<double>* o = out + ((size_t)set * a.nsrc + s) * box + b;
+    if (keep[s]) *o = cadd(*o, acc[s]);
+  }
+}
+
+// Phase adjoint: g0 = exp(+i omega t) g
+__global__ void k_adj_phase(const cpx<double>* g, const double* omega, const double* times, int box, cpx<double>* g0) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= box) return;
+  double sn, cs;
+  sincos(omega[b] * times[0], &sn, &cs);
+  const cpx<double> v = g[b];
+  g0[b] = {v.x * cs - v.y * sn, v.y * cs + v.x * sn};
+}
+
+__global__ void k_fill_one_u8(uint8_t* p, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 1;
+}
+
+__global__ void k_fill_zero_c(cpx<double>* p, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = {0.0, 0.0};
+}
+
+// ----------------------------------------------------------------------------
 // Plan
 // ----------------------------------------------------------------------------
 struct DevTables {
@@ -1492,6 +1629,358 @@ int order2_dispatch(glw_plan* P, int nt, double* a, double* b, hipStream_t st) {
   return P->d.precision == GLW_PREC_FP32 ? order2_chunk<float>(P, nt, a, b, st) : order2_chunk<double>(P, nt, a, b, st);
 }
 
+// ----------------------------------------------------------------------------
+// Adjoint of eta11 + eta22 + eta33 (one instant): reverse pass of the
+// forward graph above, stage by stage.  Forward intermediates (column
+// passes, forced spectra, maxima) are read from the plan's workspace, so the
+// forward stages must have run for the same spectrum and instant (recompute).
+// ----------------------------------------------------------------------------
+int ensure_ws(glw_plan* P, const char* name, size_t bytes) {
+  auto it = P->ws.find(name);
+  if (it != P->ws.end())
+    return it->second.second >= bytes ? GLW_OK : fail(GLW_EINVAL, std::string("workspace size changed: ") + name);
+  return P->reserve(name, bytes);
+}
+
+glw_box union_box(const glw_box& a, const glw_box& b) {
+  glw_box u;
+  u.mx0 = std::min(a.mx0, b.mx0);
+  u.my0 = std::min(a.my0, b.my0);
+  u.wx = std::max(a.mx0 + a.wx, b.mx0 + b.wx) - u.mx0;
+  u.wy = std::max(a.my0 + a.wy, b.my0 + b.wy) - u.my0;
+  return u;
+}
+
+int gather(const cpx<double>* u, const glw_box& ub, const glw_box& db, double scale, int add, cpx<double>* dst,
+           hipStream_t st) {
+  const int n = box_size(db);
+  hipLaunchKernelGGL(k_adj_gather, dim3((n + 255) / 256), dim3(256), 0, st, u, ub, db, scale, add, dst);
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+RowArgs row_desc(const void* in, int nin, int istride, const glw_box* boxes, int gy) {
+  RowArgs r;
+  memset(&r, 0, sizeof(r));
+  r.in = in;
+  r.nin = nin;
+  r.istride = istride;
+  for (int f = 0; f < nin; ++f) {
+    r.fmx0[f] = boxes[f].mx0;
+    r.fwx[f] = boxes[f].wx;
+  }
+  r.gy = gy;
+  return r;
+}
+
+RowArgs row_grad_out(void* out, int nout, int ostride, const glw_box* boxes, int gy) {
+  RowArgs r;
+  memset(&r, 0, sizeof(r));
+  r.out = out;
+  r.nout = nout;
+  r.ostride = ostride;
+  r.out_per_field = 1;
+  for (int f = 0; f < nout; ++f) {
+    r.fmx0[f] = boxes[f].mx0;
+    r.fwx[f] = boxes[f].wx;
+  }
+  r.gy = gy;
+  return r;
+}
+
+ColFwdArgs col_dft(const void* in, int in_nfield, int in_foff, int nsrc, int istride, const glw_box& box,
+                   const uint8_t* ones, cpx<double>* out) {
+  ColFwdArgs c;
+  c.in = in;
+  c.nsrc = nsrc;
+  c.in_nfield = in_nfield;
+  c.in_foff = in_foff;
+  c.istride = istride;
+  c.omx0 = box.mx0;
+  c.owx = box.wx;
+  c.omy0 = box.my0;
+  c.owy = box.wy;
+  c.mask = ones;
+  c.scale = 1.0;
+  c.out = out;
+  c.fmax = nullptr;
+  return c;
+}
+
+// Spectra [ntask][nsrc][box] -> unnormalized inverse y-pass columns, all sources with unit multipliers.
+ColInvArgs col_idft(const cpx<double>* spec, int nsrc, const glw_box& box, const DevTables& t, double scale,
+                    void* out) {
+  ColInvArgs a = colinv_base(spec, nsrc, 1, box, t);
+  a.nfield = nsrc;
+  for (int f = 0; f < nsrc; ++f) {
+    a.fsrc[f] = f;
+    a.fcode[f] = M_ONE;
+    a.fscale[f] = scale;
+  }
+  a.out = out;
+  a.out_nfield = nsrc;
+  a.ostride = box.wx;
+  return a;
+}
+
+int collect(const ColInvArgs& fwd, const cpx<double>* x, int nset, cpx<double>* out, hipStream_t st) {
+  const int n = box_size(glw_box{fwd.mx0, fwd.my0, fwd.wx, fwd.wy});
+  hipLaunchKernelGGL(k_adj_collect, dim3((n + NT - 1) / NT, nset), dim3(NT), 0, st, fwd, x, out);
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+int fill_ones(uint8_t* p, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill_one_u8, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, const cpx<double>* gfield,
+                cpx<double>* gs0, int recompute, hipStream_t st) {
+  const glw_plan_desc& d = P->d;
+  if (d.precision != GLW_PREC_FP64) return fail(GLW_EINVAL, "the adjoint runs on FP64 plans");
+  if (!orders || (orders & ~7)) return fail(GLW_EINVAL, "orders must be a mask of 1 (eta11), 2 (eta22), 4 (eta33)");
+  if ((orders & 2) && d.order < 2) return fail(GLW_EINVAL, "plan built without order-2 tables");
+  if ((orders & 4) && d.order < 3) return fail(GLW_EINVAL, "plan built without order-3 tables");
+  const size_t z = sizeof(cpx<double>);
+  const int b1 = box_size(d.p1), b2 = box_size(d.p2), b3 = box_size(d.p3);
+  const double nn = (double)d.nx * d.ny, sc = 1.0 / nn;
+  glw_box U = d.p1;  // analysis box: every box of the plan, so the workspace does not depend on `orders`
+  if (d.order >= 2) U = union_box(U, d.p2);
+  if (d.order >= 3) U = union_box(U, d.p3);
+  int rc;
+#define GLW_WS(name, bytes) \
+  if ((rc = ensure_ws(P, name, bytes))) return rc
+  GLW_WS("adj_fwd", 2 * (size_t)d.nx * d.ny * z);
+  GLW_WS("adj_rows", (size_t)d.ny * U.wx * z);
+  GLW_WS("adj_u", (size_t)box_size(U) * z);
+  GLW_WS("adj_onesU", (size_t)box_size(U));
+  GLW_WS("adj_ones1", (size_t)b1);
+  GLW_WS("adj_gs", (size_t)b1 * z);
+  if (d.order >= 2) GLW_WS("adj_ones2", (size_t)b2);
+  const int J = d.o2.eta_rank, O = d.outer_rank, I = d.inner_rank;
+  const int wmax = std::max(d.p1.wx, d.p2.wx);
+  if (orders & 2) {
+    GLW_WS("adj_e22", (size_t)b2 * z);
+    GLW_WS("adj_s22", (size_t)J * 2 * b2 * z);
+    GLW_WS("adj_u22", (size_t)J * 2 * d.g2y * d.p2.wx * z);
+    GLW_WS("adj_t22", (size_t)J * 8 * d.g2y * d.p1.wx * z);
+    GLW_WS("adj_x22", (size_t)J * 8 * b1 * z);
+  }
+  if (orders & 4) {
+    GLW_WS("adj_e33", (size_t)b3 * z);
+    GLW_WS("adj_s33", (size_t)O * 2 * b3 * z);
+    GLW_WS("adj_u33", (size_t)O * 2 * d.g3y * d.p3.wx * z);
+    GLW_WS("adj_t33", (size_t)O * 21 * d.g3y * wmax * z);
+    GLW_WS("adj_x33a", (size_t)O * 12 * b1 * z);
+    GLW_WS("adj_x33b", (size_t)O * 9 * b2 * z);
+    GLW_WS("adj_pair", (size_t)O * 4 * b2 * z);
+    GLW_WS("adj_sin", (size_t)O * I * 4 * b2 * z);
+    GLW_WS("adj_uin", (size_t)O * I * 4 * d.g2y * d.p2.wx * z);
+    GLW_WS("adj_tin", (size_t)O * I * 12 * d.g2y * d.p1.wx * z);
+    GLW_WS("adj_xin", (size_t)O * I * 12 * b1 * z);
+  }
+#undef GLW_WS
+  auto B = [&](const char* n) { return P->buf<cpx<double>>(n); };
+  if ((rc = fill_ones(P->buf<uint8_t>("adj_onesU"), box_size(U), st))) return rc;
+  if ((rc = fill_ones(P->buf<uint8_t>("adj_ones1"), b1, st))) return rc;
+  if (d.order >= 2 && (rc = fill_ones(P->buf<uint8_t>("adj_ones2"), b2, st))) return rc;
+  const uint8_t* ones1 = P->buf<uint8_t>("adj_ones1");
+  const uint8_t* ones2 = d.order >= 2 ? P->buf<uint8_t>("adj_ones2") : nullptr;
+
+  if (recompute) {  // forward stages for this spectrum and instant
+    double* fo = P->buf<double>("adj_fwd");
+    const size_t plane = (size_t)d.nx * d.ny * 2;
+    if ((rc = phase(P, s0, times, 1, st))) return rc;
+    if ((orders & 2) && (rc = order2_chunk<double>(P, 1, fo, fo + plane, st))) return rc;
+    if ((orders & 4) && (rc = order3_chunk<double>(P, 1, fo, fo + plane, st))) return rc;
+  }
+  // analysis: (h / N^2) DFT2 of the field gradient on the union box
+  {
+    RowArgs ra;
+    memset(&ra, 0, sizeof(ra));
+    ra.gy = d.ny;
+    ra.nout = 1;
+    ra.out = B("adj_rows");
+    ra.ostride = U.wx;
+    ra.omx0 = U.mx0;
+    ra.owx = U.wx;
+    if ((rc = GLW_STAGE(P, "adj_analysis_row", st, launch_rowfwd<double>(d.nx, ra, 1, P->tw(d.nx, false), 0, gfield, st))))
+      return rc;
+    ColFwdArgs ca = col_dft(B("adj_rows"), 1, 0, 1, U.wx, U, P->buf<uint8_t>("adj_onesU"), B("adj_u"));
+    ca.scale = d.depth / nn;
+    if ((rc = GLW_STAGE(P, "adj_analysis_col", st, launch_colfwd<double>(d.ny, ca, 1, P->tw(d.ny, false), P->exec(STAGE_COL), st))))
+      return rc;
+  }
+  cpx<double>* gs = B("adj_gs");
+  if (orders & 1) {
+    if ((rc = gather(B("adj_u"), U, d.p1, 1.0, 0, gs, st))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_fill_zero_c, dim3((b1 + 255) / 256), dim3(256), 0, st, gs, (size_t)b1);
+    GLW_HIP(hipGetLastError());
+  }
+  const cpx<double>* stp = P->buf<cpx<double>>("st");
+  if (orders & 2) {
+    static const int e_codes[8] = {M_ONE, M_NU, M_NU2, M_QX_NU, M_QY_NU, M_Q2_NU, M_QX, M_QY};
+    if ((rc = gather(B("adj_u"), U, d.p2, 1.0, 0, B("adj_e22"), st))) return rc;
+    AccEta22Args g;
+    g.spec = B("e22_s");
+    g.fmax = P->buf<uint64_t>("e22_max");
+    g.floor_level = d.floor_level;
+    g.J = J;
+    g.box = b2;
+    g.a = P->t2.a;
+    g.ch = P->e22_ch;
+    g.sh = P->e22_sh;
+    g.coef = P->e22_coef;
+    g.out = nullptr;
+    P->mark_begin("adj_o2_acc", st);
+    hipLaunchKernelGGL(k_adj_acc_eta22, dim3((b2 + NT - 1) / NT, J), dim3(NT), 0, st, g, B("adj_e22"), P->mask2,
+                       B("adj_s22"));
+    P->mark_end(st);
+    GLW_HIP(hipGetLastError());
+    const double up = nn / ((double)d.g2x * d.g2y);
+    ColInvArgs a = col_idft(B("adj_s22"), 2, d.p2, P->t2, up, B("adj_u22"));
+    if ((rc = GLW_STAGE(P, "adj_o2_colinv", st, launch_colinv<double>(d.g2y, a, J, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+      return rc;
+    glw_box in_boxes[8], g_boxes[2] = {d.p2, d.p2};
+    for (int f = 0; f < 8; ++f) in_boxes[f] = d.p1;
+    const RowArgs fwd = row_desc(P->ws.at("e22_t").first, 8, d.p1.wx, in_boxes, d.g2y);
+    const RowArgs gin = row_desc(B("adj_u22"), 2, d.p2.wx, g_boxes, d.g2y);
+    const RowArgs gout = row_grad_out(B("adj_t22"), 8, d.p1.wx, in_boxes, d.g2y);
+    if ((rc = GLW_STAGE(P, "adj_o2_row", st, (launch_row_adjoint<double, 8, 2, Eta22Op>(d.g2x, fwd, gin, gout, J, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
+      return rc;
+    const ColFwdArgs c = col_dft(B("adj_t22"), 8, 0, 8, d.p1.wx, d.p1, ones1, B("adj_x22"));
+    if ((rc = GLW_STAGE(P, "adj_o2_colfwd", st, launch_colfwd<double>(d.g2y, c, J, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+      return rc;
+    ColInvArgs fa = colinv_base(stp, 1, J, d.p1, P->t1);
+    fa.damp = P->e22_damp;
+    fa.dmod = J;
+    fa.nfield = 8;
+    for (int f = 0; f < 8; ++f) {
+      fa.fsrc[f] = 0;
+      fa.fcode[f] = e_codes[f];
+      fa.fscale[f] = sc;
+    }
+    if ((rc = collect(fa, B("adj_x22"), 1, gs, st))) return rc;
+  }
+  if (orders & 4) {
+    const double up2 = nn / ((double)d.g2x * d.g2y), up3 = nn / ((double)d.g3x * d.g3y);
+    if ((rc = gather(B("adj_u"), U, d.p3, 1.0, 0, B("adj_e33"), st))) return rc;
+    AccOuterArgs g;
+    g.spec = B("sout");
+    g.fmax = P->buf<uint64_t>("max_out");
+    g.cmax = P->buf<uint64_t>("max_combo");
+    g.floor_level = d.floor_level;
+    g.O = O;
+    g.box = b3;
+    g.a = P->t3.a;
+    g.mask = P->mask3;
+    g.ch = P->out_ch;
+    g.sh = P->out_sh;
+    g.coef = P->out_coef;
+    g.out = nullptr;
+    P->mark_begin("adj_o3_outer_acc", st);
+    hipLaunchKernelGGL(k_adj_acc_outer, dim3((b3 + NT - 1) / NT, O), dim3(NT), 0, st, g, B("adj_e33"), B("adj_s33"));
+    P->mark_end(st);
+    GLW_HIP(hipGetLastError());
+    ColInvArgs a = col_idft(B("adj_s33"), 2, d.p3, P->t3, up3, B("adj_u33"));
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_colinv", st, launch_colinv<double>(d.g3y, a, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
+      return rc;
+    glw_box fb[21], g3b[2] = {d.p3, d.p3};
+    for (int f = 0; f < 21; ++f) fb[f] = f < 12 ? d.p1 : d.p2;
+    const RowArgs fwd = row_desc(P->ws.at("tout").first, 21, wmax, fb, d.g3y);
+    const RowArgs gin = row_desc(B("adj_u33"), 2, d.p3.wx, g3b, d.g3y);
+    const RowArgs gout = row_grad_out(B("adj_t33"), 21, wmax, fb, d.g3y);
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_row", st, (launch_row_adjoint<double, 21, 2, OuterOp>(d.g3x, fwd, gin, gout, O, P->tw(d.g3x, false), P->exec(STAGE_ROW), st)))))
+      return rc;
+    const ColFwdArgs ca = col_dft(B("adj_t33"), 21, 0, 12, wmax, d.p1, ones1, B("adj_x33a"));
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_colfwd", st, launch_colfwd<double>(d.g3y, ca, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
+      return rc;
+    const ColFwdArgs cb = col_dft(B("adj_t33"), 21, 12, 9, wmax, d.p2, ones2, B("adj_x33b"));
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_colfwd", st, launch_colfwd<double>(d.g3y, cb, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
+      return rc;
+    {  // first-order fields of the outer nodes -> parent spectrum
+      static const int codes[12] = {M_ONE,  M_I_QX, M_I_QY,     M_QX_NU,   M_QY_NU, M_MI_NU,
+                                    M_NU_QX, M_NU_QY, M_MI_Q2_NU, M_MI_Q2NU, M_MA,    M_MQ2};
+      ColInvArgs fa = colinv_base(stp, 1, O, d.p1, P->t1);
+      fa.damp = P->damp_o;
+      fa.dmod = O;
+      fa.nfield = 12;
+      for (int f = 0; f < 12; ++f) {
+        fa.fsrc[f] = 0;
+        fa.fcode[f] = codes[f];
+        fa.fscale[f] = sc;
+      }
+      if ((rc = collect(fa, B("adj_x33a"), 1, gs, st))) return rc;
+    }
+    {  // pair-state fields -> pair spectra (eta, eta_t, phi, phi_t)
+      hipLaunchKernelGGL(k_fill_zero_c, dim3((O * 4 * b2 + 255) / 256), dim3(256), 0, st, B("adj_pair"),
+                         (size_t)O * 4 * b2);
+      GLW_HIP(hipGetLastError());
+      static const int srcs[9] = {0, 0, 0, 2, 2, 2, 2, 3, 1};
+      static const int codes[9] = {M_ONE, M_I_QX, M_I_QY, M_I_QX, M_I_QY, M_A, M_Q2, M_A, M_ONE};
+      ColInvArgs fa = colinv_base(B("pair"), 4, 1, d.p2, P->t2);
+      fa.fmax = P->buf<uint64_t>("max_et");
+      fa.fmax_stride = 1;
+      fa.floor_src_mask = 1 << 1;
+      fa.floor_level = d.floor_level;
+      fa.nfield = 9;
+      for (int f = 0; f < 9; ++f) {
+        fa.fsrc[f] = srcs[f];
+        fa.fcode[f] = codes[f];
+        fa.fscale[f] = sc;
+      }
+      if ((rc = collect(fa, B("adj_x33b"), O, B("adj_pair"), st))) return rc;
+    }
+    AccInnerArgs gi;
+    gi.spec = B("sin");
+    gi.fmax = P->buf<uint64_t>("max_in");
+    gi.floor_level = d.floor_level;
+    gi.I = I;
+    gi.box = b2;
+    gi.a = P->t2.a;
+    gi.mask = P->mask2;
+    gi.ch = P->in_ch;
+    gi.sh = P->in_sh;
+    gi.coef = P->in_coef;
+    gi.out = nullptr;
+    gi.etmax = nullptr;
+    P->mark_begin("adj_o3_inner_acc", st);
+    hipLaunchKernelGGL(k_adj_acc_inner, dim3((b2 + NT - 1) / NT, O * I), dim3(NT), 0, st, gi, B("adj_pair"), B("adj_sin"));
+    P->mark_end(st);
+    GLW_HIP(hipGetLastError());
+    ColInvArgs ai = col_idft(B("adj_sin"), 4, d.p2, P->t2, up2, B("adj_uin"));
+    if ((rc = GLW_STAGE(P, "adj_o3_inner_colinv", st, launch_colinv<double>(d.g2y, ai, O * I, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+      return rc;
+    glw_box ib[12], g2b[4] = {d.p2, d.p2, d.p2, d.p2};
+    for (int f = 0; f < 12; ++f) ib[f] = d.p1;
+    const RowArgs ifwd = row_desc(P->ws.at("tin").first, 12, d.p1.wx, ib, d.g2y);
+    const RowArgs igin = row_desc(B("adj_uin"), 4, d.p2.wx, g2b, d.g2y);
+    const RowArgs igout = row_grad_out(B("adj_tin"), 12, d.p1.wx, ib, d.g2y);
+    if ((rc = GLW_STAGE(P, "adj_o3_inner_row", st, (launch_row_adjoint<double, 12, 4, InnerOp>(d.g2x, ifwd, igin, igout, O * I, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
+      return rc;
+    const ColFwdArgs ci = col_dft(B("adj_tin"), 12, 0, 12, d.p1.wx, d.p1, ones1, B("adj_xin"));
+    if ((rc = GLW_STAGE(P, "adj_o3_inner_colfwd", st, launch_colfwd<double>(d.g2y, ci, O * I, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+      return rc;
+    static const int icodes[12] = {M_ONE, M_NU, M_NU2, M_NU3, M_QX_NU, M_QY_NU, M_QX, M_QY, M_Q2_NU, M_Q2, M_NU_QX, M_NU_QY};
+    ColInvArgs fi = colinv_base(stp, 1, O * I, d.p1, P->t1);
+    fi.damp = P->damp_oi;
+    fi.dmod = O * I;
+    fi.nfield = 12;
+    for (int f = 0; f < 12; ++f) {
+      fi.fsrc[f] = 0;
+      fi.fcode[f] = icodes[f];
+      fi.fscale[f] = sc;
+    }
+    if ((rc = collect(fi, B("adj_xin"), 1, gs, st))) return rc;
+  }
+  hipLaunchKernelGGL(k_adj_phase, dim3((b1 + 255) / 256), dim3(256), 0, st, gs, P->omega, times, b1, gs0);
+  GLW_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
 // Per-stage time (ms, summed over launches) of the events recorded since the
 // pool was last reset; resets the pool.
 int collect_stage_ms(glw_plan* P, std::map<std::string, double>& agg) {
@@ -1668,6 +2157,18 @@ int glw_plan_autotune(glw_plan* plan, int32_t kind, const double* s0, const doub
   if (!rc) rc = hipStreamSynchronize(st) == hipSuccess ? GLW_OK : fail(GLW_EDEVICE, "stream synchronize failed");
   const int rc2 = enable_events(plan, profiling);
   return rc ? rc : rc2;
+}
+
+int glw_adjoint_execute(glw_plan* plan, const double* s0, const double* times, int32_t orders,
+                        const double* grad_field, double* grad_s0, int32_t recompute, void* stream) {
+  if (!plan) return fail(GLW_EINVAL, "null plan");
+  if (!s0 || !times || !grad_field || !grad_s0) return fail(GLW_EINVAL, "null array argument");
+  int dev = -1;
+  GLW_HIP(hipGetDevice(&dev));
+  if (dev != plan->device) GLW_HIP(hipSetDevice(plan->device));
+  hipStream_t st = (hipStream_t)stream;
+  plan->cur_stream = st;
+  return adjoint_run(plan, s0, times, orders, (const cpx<double>*)grad_field, (cpx<double>*)grad_s0, recompute, st);
 }
 
 int glw_plan_get_schedule(glw_plan* plan, char* text, size_t len) {

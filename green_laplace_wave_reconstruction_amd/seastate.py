@@ -188,3 +188,10 @@ def batch_sea_states(count: int = 256, seed: int = 3, *, gravity: float = 9.81, 
         out.append(BatchState(state=st, order=order, kp_h=kp_h, hs=hs, tp=tp, gamma=gamma, spread_s=spread,
                               theta0_drawn=theta_drawn, theta0=theta, seed=seed * 1000 + i))
     return out
+
+
+def inverse_256(seed: int = 4) -> SeaState:
+    """Hidden sea state of the inverse reconstruction: 256^2, h=40 m, Hs=5 m, Tp=11 s, gamma=3.3, s=10,
+    L=1700 m (|k| < 4 kp stays cubic-alias safe below L=1787 m), parents kh > 0.5."""
+    return jonswap_sea_state(nx=256, ny=256, lx=1700.0, ly=1700.0, depth=40.0, hs=5.0, tp=11.0, gamma=3.3,
+                             spread_s=10.0, min_kh=0.5, seed=seed)

@@ -189,6 +189,21 @@ int glw_plan_profile(glw_plan* plan, int32_t enable);
 int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_len, double* ms,
                          int32_t* launches, int32_t* count);
 
+/* Adjoint (no reference counterpart: the reference has no gradients).  For
+ * a real loss L of the elevation fields at one instant times[0], given
+ * grad_field = dL/dRe(eta+) + i dL/dIm(eta+) (device complex128 [ny][nx],
+ * the same for every included component), writes grad_s0 = dL/dRe S0 +
+ * i dL/dIm S0 on the parent box (device complex128 [wy][wx] of glw_plan_desc
+ * p1), where eta+ = sum of the components selected by `orders` (bit mask:
+ * 1 eta11, 2 eta22, 4 eta33).  Supports, GL nodes, shifts, masks and
+ * roundoff floors are those of the forward evaluation and are held fixed.
+ * recompute != 0 first runs the forward stages for (s0, times[0]); with 0
+ * the plan's workspace must still hold them (the last execute calls of
+ * the selected orders were for this s0 and instant, nt = 1).  FP64 plans
+ * only.  Allocates its workspace on the first call. */
+int glw_adjoint_execute(glw_plan* plan, const double* s0, const double* times, int32_t orders,
+                        const double* grad_field, double* grad_s0, int32_t recompute, void* stream);
+
 /* FFT schedule autotuner (no reference counterpart: the reference runs
  * FFTW/MATLAB/NumPy FFTs whose planning is internal).  Runs `kind` of the
  * plan (reps timed repetitions per candidate, plus warm-ups) with every
