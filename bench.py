@@ -235,6 +235,99 @@ def bench_batch(args):
     }
 
 
+def bench_inverse(args):
+    """Inverse reconstruction at 256^2: one step = misfit gradient (forward eta11+eta22+eta33 + adjoint)."""
+    import torch
+
+    from green_laplace_wave_reconstruction_amd import seastate
+    from green_laplace_wave_reconstruction_amd.inverse import GLElevationModel, gl_elevation
+
+    s = seastate.inverse_256()
+    m = GLElevationModel(s.kx, s.ky, s.a, s.b, lx=s.lx, ly=s.ly, nx=s.nx, ny=s.ny, depth=s.depth,
+                         gravity=s.gravity, order=3)
+    at = torch.tensor(s.a, device=m.device)
+    bt = torch.tensor(s.b, device=m.device)
+    target = m.elevation(at, bt)
+    rng = np.random.default_rng(14)
+    a = (at * (1 + 0.2 * torch.tensor(rng.standard_normal(s.count), device=m.device))).float().requires_grad_()
+    b = (bt * (1 + 0.2 * torch.tensor(rng.standard_normal(s.count), device=m.device))).float().requires_grad_()
+
+    def step():
+        if a.grad is not None:
+            a.grad = None
+            b.grad = None
+        loss = 0.5 * ((gl_elevation(a, b, m) - target) ** 2).sum()
+        loss.backward()
+        return loss
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the 200-step Adam fit (timed as a whole)
+    a.data.copy_((at * (1 + 0.2 * torch.tensor(np.random.default_rng(14).standard_normal(s.count),
+                                               device=m.device))).float())
+    b.data.copy_((bt * (1 + 0.2 * torch.tensor(np.random.default_rng(15).standard_normal(s.count),
+                                               device=m.device))).float())
+    amp = float(torch.sqrt((at**2 + bt**2).max()))
+    opt = torch.optim.Adam([a, b], lr=3e-2 * amp)
+    sched = torch.optim.lr_scheduler.ExponentialLR(opt, 0.97)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    first = None
+    for _ in range(200):
+        opt.zero_grad()
+        loss = step()
+        opt.step()
+        sched.step()
+        first = float(loss.detach()) if first is None else first
+    torch.cuda.synchronize()
+    adam_s = time.perf_counter() - t1
+    final = float(0.5 * ((m.elevation(a.detach().double(), b.detach().double()) - target) ** 2).sum())
+    err = float(torch.sqrt(((a.detach().double() - at) ** 2 + (b.detach().double() - bt) ** 2).sum()
+                           / (at**2 + bt**2).sum()))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import math as _m
+
+        from oracle import gl_oracle as O
+        from torch_gl import TorchGL, deposit
+
+        k = np.hypot(s.kx, s.ky)
+        om = np.sqrt(s.gravity * k * np.tanh(s.depth * k))
+        spec = O.deposit(s.a, s.b, s.kx, s.ky, om, s.depth, s.lx, s.ly, s.nx, s.ny, 0.0)
+        kxg, kyg = O.wavenumber_grid(s.lx, s.ly, s.nx, s.ny)
+        tg = TorchGL(spec, s.depth * kxg, s.depth * kyg, depth=s.depth, order=3, peak_q=m.host.peak_q)
+        mx = np.rint(s.kx * s.lx / (2 * _m.pi)).astype(int)
+        my = np.rint(s.ky * s.ly / (2 * _m.pi)).astype(int)
+        tgt = target.cpu()
+        ac = torch.tensor(s.a, requires_grad=True)
+        bc = torch.tensor(s.b, requires_grad=True)
+        t2 = time.perf_counter()
+        lc = 0.5 * ((tg.elevation(deposit(ac, bc, mx, my, s.nx, s.ny, s.depth)) - tgt) ** 2).sum()
+        lc.backward()
+        el = time.perf_counter() - t2
+        cpu = {"value": 1.0 / el, "unit": "gradients/s", "cores": torch.get_num_threads(), "kind": "port",
+               "sample": "one misfit gradient by torch-CPU autograd of the full-grid FP64 restatement "
+                         "(tests/torch_gl.py), torch.fft on the host threads"}
+    return {
+        "metric": "inverse reconstruction: misfit gradients/s at 256^2 (eta11+eta22+eta33, hand-written adjoint)",
+        "value": args.steps / wall, "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic seeded JONSWAP sea state (numpy default_rng seed 4), hidden amplitudes",
+        "config": {"workload": f"256x256, h=40 m, Hs=5 m, Tp=11 s, L=1700 m, {s.count} parents; amplitudes float32, "
+                               "forward and adjoint FP64"},
+        "adam_200_steps": {"seconds": adam_s, "misfit_first": first, "misfit_final": final,
+                           "amplitude_rel_error_final": err, "amplitude_rel_error_initial": 0.2},
+        "cpu_baseline": cpu,
+    }
+
+
 def rel_l2(a, b) -> float:
     nb = float(np.linalg.norm(b))
     return float(np.linalg.norm(a - b) / nb) if nb > 0 else float(np.linalg.norm(a))
@@ -290,7 +383,7 @@ def bench_wit(args):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="order3", choices=("order3", "eta44", "wit", "batch"))
+    ap.add_argument("--workload", default="order3", choices=("order3", "eta44", "wit", "batch", "inverse"))
     ap.add_argument("--states", type=int, default=256, help="batch workload: sea states per step")
     ap.add_argument("--branches", type=int, default=4, help="batch workload: parallel graph branches")
     ap.add_argument("--rank", type=int, default=6, help="eta44 GL rank")
@@ -337,6 +430,9 @@ def main():
         return
     if args.workload == "batch":
         print(json.dumps(bench_batch(args)))
+        return
+    if args.workload == "inverse":
+        print(json.dumps(bench_inverse(args)))
         return
     s, c, kxg, kyg, spec, omega = workload()
     prec = PL.PREC_FP64 if args.precision == "fp64" else PL.PREC_FP32
