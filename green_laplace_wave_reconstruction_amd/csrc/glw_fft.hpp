@@ -9,6 +9,8 @@
 // transforms are unnormalised in both directions.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 namespace glw {
@@ -275,27 +277,50 @@ __device__ __forceinline__ void dft_reg(cpx<T>* v) {
   }
 }
 
+// Optional global-memory ends of a transform: the first stage can read its
+// inputs through a loader `T ld(seq, e)` (element e of sequence seq) instead
+// of LDS, and the last stage can hand its outputs to a storer
+// `st(seq, e, v)` instead of LDS.  That removes the LDS staging pass (and its
+// zero fill) before the transform and the read-back pass after it.  Each
+// functor declares SEQ_FAST: whether consecutive sequences are contiguous in
+// its global layout, which picks the lane mapping of that stage.
+struct NoIO {
+  static constexpr bool SEQ_FAST = false;
+};
+
 // One Stockham stage (radix R, NS = product of earlier radices), then the
 // remaining stages by compile-time recursion.
-template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS, int PLAN>
-__device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw) {
+template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS, int PLAN, class LD, class ST>
+__device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st) {
   constexpr int R = radix_plan<PLAN>(L, S);
   if constexpr (R > 0) {
     constexpr int NB = L / R;           // butterflies per sequence
     constexpr int TOTAL = NSEQ * NB;    // butterflies in the workgroup
     constexpr int PER = (TOTAL + NT - 1) / NT;
     constexpr int TWS = L / (NS * R);   // twiddle index step
+    constexpr bool GLOAD = S == 0 && !std::is_same<LD, NoIO>::value;
+    constexpr bool GSTORE = radix_plan<PLAN>(L, S + 1) == 0 && !std::is_same<ST, NoIO>::value;
+    // Butterfly b -> (sequence, index).  With 8+ sequences in LDS the sequence
+    // runs fastest across lanes: neighbouring lanes then touch the same element
+    // of sequences an odd stride apart (no LDS bank conflicts) and read the
+    // same twiddle (broadcast).  A global end follows its functor's layout.
+    constexpr bool SEQ_FAST = GLOAD ? LD::SEQ_FAST : (GSTORE ? ST::SEQ_FAST : NSEQ >= 8);
     cpx<T> v[PER][R];
     const int tid = threadIdx.x;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int b = tid + p * NT;
       if (TOTAL % NT == 0 || b < TOTAL) {
-        const int seq = b / NB;
-        const int j = b - seq * NB;
-        const cpx<T>* src = buf + seq * SSTR;
+        const int seq = SEQ_FAST ? b % NSEQ : b / NB;
+        const int j = SEQ_FAST ? b / NSEQ : b - seq * NB;
+        if constexpr (GLOAD) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[p][r] = src[lds_pad(j + r * NB)];
+          for (int r = 0; r < R; ++r) v[p][r] = ld(seq, j + r * NB);
+        } else {
+          const cpx<T>* src = buf + seq * SSTR;
+#pragma unroll
+          for (int r = 0; r < R; ++r) v[p][r] = src[lds_pad(j + r * NB)];
+        }
         if constexpr (NS > 1) {
           const int k = j % NS;
 #pragma unroll
@@ -307,23 +332,28 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw) {
         }
       }
     }
-    __syncthreads();
+    if constexpr (!GLOAD) __syncthreads();  // in place: every read before any write
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int b = tid + p * NT;
       if (TOTAL % NT == 0 || b < TOTAL) {
-        const int seq = b / NB;
-        const int j = b - seq * NB;
+        const int seq = SEQ_FAST ? b % NSEQ : b / NB;
+        const int j = SEQ_FAST ? b / NSEQ : b - seq * NB;
         const int k = j % NS;
         dft_reg<R, SIGN>(v[p]);
-        cpx<T>* dst = buf + seq * SSTR;
         const int base = (j / NS) * NS * R + k;
+        if constexpr (GSTORE) {
 #pragma unroll
-        for (int r = 0; r < R; ++r) dst[lds_pad(base + r * NS)] = v[p][r];
+          for (int r = 0; r < R; ++r) st(seq, base + r * NS, v[p][r]);
+        } else {
+          cpx<T>* dst = buf + seq * SSTR;
+#pragma unroll
+          for (int r = 0; r < R; ++r) dst[lds_pad(base + r * NS)] = v[p][r];
+        }
       }
     }
-    __syncthreads();
-    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R, PLAN>(buf, tw);
+    if constexpr (!GSTORE) __syncthreads();
+    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R, PLAN>(buf, tw, ld, st);
   }
 }
 
@@ -339,7 +369,18 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
 template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN = 0>
 __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* tw) {
   static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
-  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN>(buf, tw);
+  NoIO none;
+  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN>(buf, tw, none, none);
+}
+
+// Same transform with a global loader for the first stage and/or a storer
+// for the last (NoIO for an LDS end).  With a loader the caller stages
+// nothing (the first stage fills the LDS buffer); with a storer the results
+// are not left in LDS and the call ends without a barrier.
+template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN, class LD, class ST>
+__device__ __forceinline__ void fft_io(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st) {
+  static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
+  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN>(buf, tw, ld, st);
 }
 
 }  // namespace glw

@@ -24,6 +24,9 @@ namespace glw {
 
 constexpr int kMaxFields = 48;
 constexpr int NT = 256;
+// FFT stages read twiddles from the global table (a few KB, L1/L2-resident)
+// rather than an LDS copy: the LDS goes to data tiles (more workgroups per CU).
+constexpr bool kTwiddlesInLds = false;
 
 // Spectral multiplier codes (functions of qx, qy, nu, 1/nu, q^2, A on a bin).
 enum MulCode : int {
@@ -144,50 +147,75 @@ struct ColInvArgs {
   int out_nfield, out_foff, ostride;
 };
 
+// Box column (damped, floored, multiplied, scaled) as the first stage's input;
+// element e of column c is box row wrap(e - my0, GY).
+template <typename T>
+struct ColInvLoad {
+  static constexpr bool SEQ_FAST = true;  // neighbouring columns are contiguous
+  const cpx<double>* spec;
+  const double* damp;
+  BoxTables tab;
+  double level, scale;
+  int code, col0, wx, wy, my0, gy;
+  __device__ __forceinline__ cpx<T> operator()(int c, int e) const {
+    const int col = col0 + c;
+    const int r = wrap(e - my0, gy);
+    if (col >= wx || r >= wy) return {T(0), T(0)};
+    const int b = r * wx + col;
+    cpx<double> s = spec[b];
+    if (level >= 0.0 && hypot(s.x, s.y) < level) s = {0.0, 0.0};
+    if (damp) s = {s.x * damp[b], s.y * damp[b]};
+    s = apply_mul(code, s, tab, b);
+    return {T(s.x * scale), T(s.y * scale)};
+  }
+};
+
+template <typename T>
+struct ColInvStore {
+  static constexpr bool SEQ_FAST = true;
+  cpx<T>* out;
+  int ostride, col0, wx;
+  __device__ __forceinline__ void operator()(int c, int e, cpx<T> v) {
+    if (col0 + c < wx) out[(size_t)e * ostride + col0 + c] = v;
+  }
+};
+
 template <typename T, int GY, int CW, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
-  __shared__ cpx<T> twl[GY];
-  stage_twiddles<T, GY, NT>(twl, tw);
-  const int col0 = blockIdx.x * CW;
+  __shared__ cpx<T> twl[kTwiddlesInLds ? GY : 1];
+  const cpx<T>* twp = tw;
+  if constexpr (kTwiddlesInLds) {  // read from the second stage on, after a barrier
+    stage_twiddles<T, GY, NT>(twl, tw);
+    twp = twl;
+  }
   const int f = blockIdx.y;
   const int task = blockIdx.z;
   const int set = task / a.sdiv;
   const int src = a.fsrc[f];
-  const int code = a.fcode[f];
-  const double scale = a.fscale[f];
   const int box = a.wx * a.wy;
-  const cpx<double>* spec = a.spec + ((size_t)set * a.nsrc + src) * box;
-  const double* damp = a.damp ? a.damp + (size_t)((task % a.dmod) * a.dmul + a.dadd) * box : nullptr;
-  double level = -1.0;
+  ColInvLoad<T> ld;
+  ld.spec = a.spec + ((size_t)set * a.nsrc + src) * box;
+  ld.damp = a.damp ? a.damp + (size_t)((task % a.dmod) * a.dmul + a.dadd) * box : nullptr;
+  ld.tab = a.tab;
+  ld.level = -1.0;
   if (a.fmax && ((a.floor_src_mask >> src) & 1)) {
     const size_t k = (size_t)set * a.fmax_stride + (a.fmax_stride == 1 ? 0 : src);
-    level = a.floor_level * __longlong_as_double((long long)a.fmax[k]);
+    ld.level = a.floor_level * __longlong_as_double((long long)a.fmax[k]);
   }
-  for (int i = threadIdx.x; i < CW * seq_stride<GY>(); i += NT) buf[i] = {T(0), T(0)};
-  __syncthreads();
-  for (int i = threadIdx.x; i < CW * a.wy; i += NT) {
-    const int r = i / CW;
-    const int c = i - r * CW;
-    const int col = col0 + c;
-    if (col < a.wx) {
-      const int b = r * a.wx + col;
-      cpx<double> s = spec[b];
-      if (level >= 0.0 && hypot(s.x, s.y) < level) s = {0.0, 0.0};
-      if (damp) s = {s.x * damp[b], s.y * damp[b]};
-      s = apply_mul(code, s, a.tab, b);
-      buf[lds_at<GY>(c, wrap(a.my0 + r, GY))] = {T(s.x * scale), T(s.y * scale)};
-    }
-  }
-  __syncthreads();
-  fft_lds<T, GY, CW, seq_stride<GY>(), NT, +1, PLAN>(buf, twl);
-  cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) +
-                ((size_t)task * a.out_nfield + a.out_foff + f) * GY * a.ostride;
-  for (int i = threadIdx.x; i < CW * GY; i += NT) {
-    const int y = i / CW;
-    const int c = i - y * CW;
-    if (col0 + c < a.wx) out[(size_t)y * a.ostride + col0 + c] = buf[lds_at<GY>(c, y)];
-  }
+  ld.scale = a.fscale[f];
+  ld.code = a.fcode[f];
+  ld.col0 = blockIdx.x * CW;
+  ld.wx = a.wx;
+  ld.wy = a.wy;
+  ld.my0 = a.my0;
+  ld.gy = GY;
+  ColInvStore<T> st;
+  st.out = reinterpret_cast<cpx<T>*>(a.out) + ((size_t)task * a.out_nfield + a.out_foff + f) * GY * a.ostride;
+  st.ostride = a.ostride;
+  st.col0 = ld.col0;
+  st.wx = a.wx;
+  fft_io<T, GY, CW, seq_stride<GY>(), NT, +1, PLAN>(buf, twp, ld, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -219,39 +247,67 @@ __device__ __forceinline__ void block_max_atomic(double v, uint64_t* dst) {
   }
 }
 
+template <typename T>
+struct ColFwdLoad {
+  static constexpr bool SEQ_FAST = true;
+  const cpx<T>* in;
+  int istride, col0, owx;
+  __device__ __forceinline__ cpx<T> operator()(int c, int e) const {
+    return (col0 + c < owx) ? in[(size_t)e * istride + col0 + c] : cpx<T>{T(0), T(0)};
+  }
+};
+
+// Keeps the output box rows (element e is box row wrap(e - omy0, GY)), masks,
+// scales and records the largest kept |x|.
+template <typename T>
+struct ColFwdStore {
+  static constexpr bool SEQ_FAST = true;
+  cpx<double>* out;
+  const uint8_t* mask;
+  double scale;
+  double vmax;
+  int col0, owx, owy, omy0, gy;
+  __device__ __forceinline__ void operator()(int c, int e, cpx<T> v) {
+    const int col = col0 + c;
+    const int r = wrap(e - omy0, gy);
+    if (col >= owx || r >= owy) return;
+    const int b = r * owx + col;
+    cpx<double> d = {double(v.x) * scale, double(v.y) * scale};
+    if (!mask[b]) d = {0.0, 0.0};
+    else vmax = fmax(vmax, hypot(d.x, d.y));
+    out[b] = d;
+  }
+};
+
 template <typename T, int GY, int CW, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
-  __shared__ cpx<T> twl[GY];
-  stage_twiddles<T, GY, NT>(twl, tw);
-  const int col0 = blockIdx.x * CW;
+  __shared__ cpx<T> twl[kTwiddlesInLds ? GY : 1];
+  const cpx<T>* twp = tw;
+  if constexpr (kTwiddlesInLds) {
+    stage_twiddles<T, GY, NT>(twl, tw);
+    twp = twl;
+  }
   const int s = blockIdx.y;
   const int task = blockIdx.z;
   const int nf = a.in_nfield ? a.in_nfield : a.nsrc;
-  const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + ((size_t)task * nf + a.in_foff + s) * GY * a.istride;
-  for (int i = threadIdx.x; i < CW * GY; i += NT) {
-    const int y = i / CW;
-    const int c = i - y * CW;
-    buf[lds_at<GY>(c, y)] = (col0 + c < a.owx) ? in[(size_t)y * a.istride + col0 + c] : cpx<T>{T(0), T(0)};
-  }
-  __syncthreads();
-  fft_lds<T, GY, CW, seq_stride<GY>(), NT, -1, PLAN>(buf, twl);
-  cpx<double>* out = a.out + ((size_t)task * a.nsrc + s) * a.owy * a.owx;
-  double vmax = 0.0;
-  for (int i = threadIdx.x; i < CW * a.owy; i += NT) {
-    const int r = i / CW;
-    const int c = i - r * CW;
-    const int col = col0 + c;
-    if (col < a.owx) {
-      const int b = r * a.owx + col;
-      cpx<T> v = buf[lds_at<GY>(c, wrap(a.omy0 + r, GY))];
-      cpx<double> d = {double(v.x) * a.scale, double(v.y) * a.scale};
-      if (!a.mask[b]) d = {0.0, 0.0};
-      else vmax = fmax(vmax, hypot(d.x, d.y));
-      out[b] = d;
-    }
-  }
-  if (a.fmax) block_max_atomic(vmax, a.fmax + (size_t)task * a.nsrc + s);
+  ColFwdLoad<T> ld;
+  ld.in = reinterpret_cast<const cpx<T>*>(a.in) + ((size_t)task * nf + a.in_foff + s) * GY * a.istride;
+  ld.istride = a.istride;
+  ld.col0 = blockIdx.x * CW;
+  ld.owx = a.owx;
+  ColFwdStore<T> st;
+  st.out = a.out + ((size_t)task * a.nsrc + s) * a.owy * a.owx;
+  st.mask = a.mask;
+  st.scale = a.scale;
+  st.vmax = 0.0;
+  st.col0 = ld.col0;
+  st.owx = a.owx;
+  st.owy = a.owy;
+  st.omy0 = a.omy0;
+  st.gy = GY;
+  fft_io<T, GY, CW, seq_stride<GY>(), NT, -1, PLAN>(buf, twp, ld, st);
+  if (a.fmax) block_max_atomic(st.vmax, a.fmax + (size_t)task * a.nsrc + s);
 }
 
 // ----------------------------------------------------------------------------
@@ -270,28 +326,64 @@ struct RowArgs {
   int out_per_field;        // k_rowfwd: output s is pruned to box (fmx0[s], fwx[s]) instead of (omx0, owx)
 };
 
+// Box row segments of sequence seq = field * RY + row as the first stage's
+// input: element e is box column wrap(e - mx0[field], GX).
+template <typename T, int GX, int RY>
+struct RowLoad {
+  static constexpr bool SEQ_FAST = false;  // elements of one row are contiguous
+  const cpx<T>* in;                        // task base: [field][gy][istride]
+  const int* mx0;                          // per field (LDS)
+  const int* wx;
+  int gy, y0, istride;
+  __device__ __forceinline__ cpx<T> operator()(int seq, int e) const {
+    const int f = seq / RY, r = seq - (seq / RY) * RY;
+    const int j = wrap(e - mx0[f], GX);
+    return j < wx[f] ? in[((size_t)f * gy + y0 + r) * istride + j] : cpx<T>{T(0), T(0)};
+  }
+};
+
+// Output box columns of sequence seq = output * RY + row.
+template <typename T, int GX, int RY>
+struct RowStore {
+  static constexpr bool SEQ_FAST = false;
+  cpx<T>* out;                             // task base: [output][gy][ostride]
+  int gy, y0, ostride, omx0, owx;
+  __device__ __forceinline__ void operator()(int seq, int e, cpx<T> v) {
+    const int s = seq / RY, r = seq - (seq / RY) * RY;
+    const int j = wrap(e - omx0, GX);
+    if (j < owx) out[((size_t)s * gy + y0 + r) * ostride + j] = v;
+  }
+};
+
 template <typename T, int GX, int NIN, int NOUT, int RY, class Op, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[NIN * RY * SS];
-  __shared__ cpx<T> twl[GX];
-  stage_twiddles<T, GX, NT>(twl, tw);
+  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  __shared__ int fmx0[NIN], fwx[NIN];
+  const cpx<T>* twp = tw;
+  if constexpr (kTwiddlesInLds) {
+    stage_twiddles<T, GX, NT>(twl, tw);
+    twp = twl;
+  }
+#pragma unroll
+  for (int f = 0; f < NIN; ++f)
+    if (threadIdx.x == f) {
+      fmx0[f] = a.fmx0[f];
+      fwx[f] = a.fwx[f];
+    }
+  __syncthreads();
   const int y0 = blockIdx.x * RY;
   const int task = blockIdx.y;
-  const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)task * a.nin * a.gy * a.istride;
-  for (int i = threadIdx.x; i < NIN * RY * SS; i += NT) buf[i] = {T(0), T(0)};
-  __syncthreads();
-#pragma unroll
-  for (int f = 0; f < NIN; ++f) {
-    const int wx = a.fwx[f], mx0 = a.fmx0[f];
-    for (int i = threadIdx.x; i < RY * wx; i += NT) {
-      const int r = i / wx;
-      const int j = i - r * wx;
-      buf[lds_at<GX>(f * RY + r, wrap(mx0 + j, GX))] = in[((size_t)f * a.gy + y0 + r) * a.istride + j];
-    }
-  }
-  __syncthreads();
-  fft_lds<T, GX, NIN * RY, SS, NT, +1, PLAN>(buf, twl);
+  RowLoad<T, GX, RY> ld;
+  ld.in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)task * a.nin * a.gy * a.istride;
+  ld.mx0 = fmx0;
+  ld.wx = fwx;
+  ld.gy = a.gy;
+  ld.y0 = y0;
+  ld.istride = a.istride;
+  NoIO none;
+  fft_io<T, GX, NIN * RY, SS, NT, +1, PLAN>(buf, twp, ld, none);
   for (int i = threadIdx.x; i < RY * GX; i += NT) {
     const int r = i / GX;
     const int x = i - r * GX;
@@ -304,14 +396,14 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
     for (int s = 0; s < NOUT; ++s) buf[lds_at<GX>(s * RY + r, x)] = o[s];
   }
   __syncthreads();
-  fft_lds<T, GX, NOUT * RY, SS, NT, -1, PLAN>(buf, twl);
-  cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) + (size_t)task * a.nout * a.gy * a.ostride;
-  for (int i = threadIdx.x; i < NOUT * RY * a.owx; i += NT) {
-    const int sr = i / a.owx;
-    const int j = i - sr * a.owx;
-    const int s = sr / RY, r = sr - (sr / RY) * RY;
-    out[((size_t)s * a.gy + y0 + r) * a.ostride + j] = buf[lds_at<GX>(s * RY + r, wrap(a.omx0 + j, GX))];
-  }
+  RowStore<T, GX, RY> st;
+  st.out = reinterpret_cast<cpx<T>*>(a.out) + (size_t)task * a.nout * a.gy * a.ostride;
+  st.gy = a.gy;
+  st.y0 = y0;
+  st.ostride = a.ostride;
+  st.omx0 = a.omx0;
+  st.owx = a.owx;
+  fft_io<T, GX, NOUT * RY, SS, NT, -1, PLAN>(buf, twp, none, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -321,32 +413,68 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
 // the output box columns (k_rowfwd).  Same results as k_row, element for
 // element: the transforms and the products are the same operations.
 // ----------------------------------------------------------------------------
+// One field's box row segments (element e is box column wrap(e - mx0, GX)).
+template <typename T, int GX>
+struct BoxRowLoad {
+  static constexpr bool SEQ_FAST = false;
+  const cpx<T>* in;  // first row of the tile
+  int istride, mx0, wx;
+  __device__ __forceinline__ cpx<T> operator()(int r, int e) const {
+    const int j = wrap(e - mx0, GX);
+    return j < wx ? in[(size_t)r * istride + j] : cpx<T>{T(0), T(0)};
+  }
+};
+
+// Full physical rows [row][GX].
+template <typename T, typename U, int GX>
+struct FullRowStore {
+  static constexpr bool SEQ_FAST = false;
+  U* out;            // first row of the tile
+  __device__ __forceinline__ void operator()(int r, int e, cpx<T> v) {
+    out[(size_t)r * GX + e] = {decltype(out->x)(v.x), decltype(out->y)(v.y)};
+  }
+};
+
+template <typename T, int GX>
+struct FullRowLoad {
+  static constexpr bool SEQ_FAST = false;
+  const cpx<T>* in;  // first row of the tile
+  __device__ __forceinline__ cpx<T> operator()(int r, int e) const { return in[(size_t)r * GX + e]; }
+};
+
+// Output box columns of one output (element e is column wrap(e - omx0, GX)).
+template <typename T, int GX>
+struct BoxRowStore {
+  static constexpr bool SEQ_FAST = false;
+  cpx<T>* out;       // first row of the tile
+  int ostride, omx0, owx;
+  __device__ __forceinline__ void operator()(int r, int e, cpx<T> v) {
+    const int j = wrap(e - omx0, GX);
+    if (j < owx) out[(size_t)r * ostride + j] = v;
+  }
+};
+
 template <typename T, int GX, int RY, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_rowinv(RowArgs a, cpx<T>* __restrict__ phys, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[RY * SS];
-  __shared__ cpx<T> twl[GX];
-  stage_twiddles<T, GX, NT>(twl, tw);
+  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  const cpx<T>* twp = tw;
+  if constexpr (kTwiddlesInLds) {
+    stage_twiddles<T, GX, NT>(twl, tw);
+    twp = twl;
+  }
   const int y0 = blockIdx.x * RY;
   const int f = blockIdx.y;
   const int task = blockIdx.z;
-  const int wx = a.fwx[f], mx0 = a.fmx0[f];
-  const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + ((size_t)task * a.nin + f) * a.gy * a.istride;
-  for (int i = threadIdx.x; i < RY * SS; i += NT) buf[i] = {T(0), T(0)};
-  __syncthreads();
-  for (int i = threadIdx.x; i < RY * wx; i += NT) {
-    const int r = i / wx;
-    const int j = i - r * wx;
-    buf[lds_at<GX>(r, wrap(mx0 + j, GX))] = in[(size_t)(y0 + r) * a.istride + j];
-  }
-  __syncthreads();
-  fft_lds<T, GX, RY, SS, NT, +1, PLAN>(buf, twl);
-  cpx<T>* out = phys + (((size_t)task * a.nin + f) * a.gy + y0) * GX;
-  for (int i = threadIdx.x; i < RY * GX; i += NT) {
-    const int r = i / GX;
-    const int x = i - r * GX;
-    out[(size_t)r * GX + x] = buf[lds_at<GX>(r, x)];
-  }
+  BoxRowLoad<T, GX> ld;
+  ld.in = reinterpret_cast<const cpx<T>*>(a.in) + (((size_t)task * a.nin + f) * a.gy + y0) * a.istride;
+  ld.istride = a.istride;
+  ld.mx0 = a.fmx0[f];
+  ld.wx = a.fwx[f];
+  FullRowStore<T, cpx<T>, GX> st;
+  st.out = phys + (((size_t)task * a.nin + f) * a.gy + y0) * GX;
+  fft_io<T, GX, RY, SS, NT, +1, PLAN>(buf, twp, ld, st);
 }
 
 // in: [task][NIN][npts], out: [task][NOUT][npts]
@@ -370,27 +498,23 @@ template <typename T, int GX, int RY, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_rowfwd(RowArgs a, const cpx<T>* __restrict__ phys, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[RY * SS];
-  __shared__ cpx<T> twl[GX];
-  stage_twiddles<T, GX, NT>(twl, tw);
+  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  const cpx<T>* twp = tw;
+  if constexpr (kTwiddlesInLds) {
+    stage_twiddles<T, GX, NT>(twl, tw);
+    twp = twl;
+  }
   const int y0 = blockIdx.x * RY;
   const int s = blockIdx.y;
   const int task = blockIdx.z;
-  const cpx<T>* src = phys + (((size_t)task * a.nout + s) * a.gy + y0) * GX;
-  for (int i = threadIdx.x; i < RY * GX; i += NT) {
-    const int r = i / GX;
-    const int x = i - r * GX;
-    buf[lds_at<GX>(r, x)] = src[(size_t)r * GX + x];
-  }
-  __syncthreads();
-  fft_lds<T, GX, RY, SS, NT, -1, PLAN>(buf, twl);
-  cpx<T>* out = reinterpret_cast<cpx<T>*>(a.out) + ((size_t)task * a.nout + s) * a.gy * a.ostride;
-  const int owx = a.out_per_field ? a.fwx[s] : a.owx;
-  const int omx0 = a.out_per_field ? a.fmx0[s] : a.omx0;
-  for (int i = threadIdx.x; i < RY * owx; i += NT) {
-    const int r = i / owx;
-    const int j = i - r * owx;
-    out[(size_t)(y0 + r) * a.ostride + j] = buf[lds_at<GX>(r, wrap(omx0 + j, GX))];
-  }
+  FullRowLoad<T, GX> ld;
+  ld.in = phys + (((size_t)task * a.nout + s) * a.gy + y0) * GX;
+  BoxRowStore<T, GX> st;
+  st.out = reinterpret_cast<cpx<T>*>(a.out) + (((size_t)task * a.nout + s) * a.gy + y0) * a.ostride;
+  st.ostride = a.ostride;
+  st.owx = a.out_per_field ? a.fwx[s] : a.owx;
+  st.omx0 = a.out_per_field ? a.fmx0[s] : a.omx0;
+  fft_io<T, GX, RY, SS, NT, -1, PLAN>(buf, twp, ld, st);
 }
 
 // Adjoint of the pointwise products: v [task][NIN][npts] (forward inputs),
@@ -426,27 +550,22 @@ template <typename T, int GX, int RY>
 __global__ __launch_bounds__(NT) void k_synth(SynthArgs a, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[RY * SS];
-  __shared__ cpx<T> twl[GX];
-  stage_twiddles<T, GX, NT>(twl, tw);
+  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  const cpx<T>* twp = tw;
+  if constexpr (kTwiddlesInLds) {
+    stage_twiddles<T, GX, NT>(twl, tw);
+    twp = twl;
+  }
   const int y0 = blockIdx.x * RY;
   const int seq = blockIdx.y;
-  const cpx<T>* in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)seq * a.in_seq;
-  for (int i = threadIdx.x; i < RY * SS; i += NT) buf[i] = {T(0), T(0)};
-  __syncthreads();
-  for (int i = threadIdx.x; i < RY * a.wx; i += NT) {
-    const int r = i / a.wx;
-    const int j = i - r * a.wx;
-    buf[lds_at<GX>(r, wrap(a.mx0 + j, GX))] = in[(size_t)(y0 + r) * a.istride + j];
-  }
-  __syncthreads();
-  fft_lds<T, GX, RY, SS, NT, +1>(buf, twl);
-  cpx<double>* out = a.out + (size_t)seq * a.gy * GX;
-  for (int i = threadIdx.x; i < RY * GX; i += NT) {
-    const int r = i / GX;
-    const int x = i - r * GX;
-    cpx<T> v = buf[lds_at<GX>(r, x)];
-    out[(size_t)(y0 + r) * GX + x] = {double(v.x), double(v.y)};
-  }
+  BoxRowLoad<T, GX> ld;
+  ld.in = reinterpret_cast<const cpx<T>*>(a.in) + (size_t)seq * a.in_seq + (size_t)y0 * a.istride;
+  ld.istride = a.istride;
+  ld.mx0 = a.mx0;
+  ld.wx = a.wx;
+  FullRowStore<T, cpx<double>, GX> st;
+  st.out = a.out + ((size_t)seq * a.gy + y0) * GX;
+  fft_io<T, GX, RY, SS, NT, +1, 0>(buf, twp, ld, st);
 }
 
 }  // namespace glw
