@@ -17,7 +17,7 @@ UNITS = [
     (CSRC / "glw_plan.hip", [], "glw_plan.o"),
     (CSRC / "glw_wit.hip", [], "glw_wit.o"),
     (CSRC / "glw_batch.hip", [], "glw_batch.o"),
-    (CSRC / "glw_launch_cols.hip", [], "glw_launch_cols.o"),
+    *[(CSRC / "glw_launch_cols.hip", [f"-DGLW_COL_SET={i}"], f"glw_launch_cols{i}.o") for i in range(4)],
     *[(CSRC / "glw_launch_rows.hip", [f"-DGLW_ROW_SET={i}"], f"glw_launch_rows{i}.o") for i in range(ROW_SETS)],
 ]
 SOURCES = sorted({u[0] for u in UNITS})

@@ -180,13 +180,13 @@ struct ColInvStore {
   }
 };
 
-template <typename T, int GY, int CW, int PLAN = 0>
-__global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
+template <typename T, int GY, int CW, int PLAN = 0, int NTH = NT>
+__global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
   __shared__ cpx<T> twl[kTwiddlesInLds ? GY : 1];
   const cpx<T>* twp = tw;
   if constexpr (kTwiddlesInLds) {  // read from the second stage on, after a barrier
-    stage_twiddles<T, GY, NT>(twl, tw);
+    stage_twiddles<T, GY, NTH>(twl, tw);
     twp = twl;
   }
   const int f = blockIdx.y;
@@ -215,7 +215,7 @@ __global__ __launch_bounds__(NT) void k_colinv(ColInvArgs a, const cpx<T>* __res
   st.ostride = a.ostride;
   st.col0 = ld.col0;
   st.wx = a.wx;
-  fft_io<T, GY, CW, seq_stride<GY>(), NT, +1, PLAN>(buf, twp, ld, st);
+  fft_io<T, GY, CW, seq_stride<GY>(), NTH, +1, PLAN>(buf, twp, ld, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -233,16 +233,17 @@ struct ColFwdArgs {
   uint64_t* fmax;           // [task][nsrc] or null
 };
 
+template <int NTH = NT>
 __device__ __forceinline__ void block_max_atomic(double v, uint64_t* dst) {
   // non-negative doubles order like their bit patterns
-  __shared__ double red[NT / 64];
+  __shared__ double red[NTH / 64];
   for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (lane == 0) red[wv] = v;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = red[0];
-    for (int i = 1; i < NT / 64; ++i) m = fmax(m, red[i]);
+    for (int i = 1; i < NTH / 64; ++i) m = fmax(m, red[i]);
     if (m > 0.0) atomicMax((unsigned long long*)dst, (unsigned long long)__double_as_longlong(m));
   }
 }
@@ -279,13 +280,13 @@ struct ColFwdStore {
   }
 };
 
-template <typename T, int GY, int CW, int PLAN = 0>
-__global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
+template <typename T, int GY, int CW, int PLAN = 0, int NTH = NT>
+__global__ __launch_bounds__(NTH) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
   __shared__ cpx<T> twl[kTwiddlesInLds ? GY : 1];
   const cpx<T>* twp = tw;
   if constexpr (kTwiddlesInLds) {
-    stage_twiddles<T, GY, NT>(twl, tw);
+    stage_twiddles<T, GY, NTH>(twl, tw);
     twp = twl;
   }
   const int s = blockIdx.y;
@@ -306,8 +307,8 @@ __global__ __launch_bounds__(NT) void k_colfwd(ColFwdArgs a, const cpx<T>* __res
   st.owy = a.owy;
   st.omy0 = a.omy0;
   st.gy = GY;
-  fft_io<T, GY, CW, seq_stride<GY>(), NT, -1, PLAN>(buf, twp, ld, st);
-  if (a.fmax) block_max_atomic(st.vmax, a.fmax + (size_t)task * a.nsrc + s);
+  fft_io<T, GY, CW, seq_stride<GY>(), NTH, -1, PLAN>(buf, twp, ld, st);
+  if (a.fmax) block_max_atomic<NTH>(st.vmax, a.fmax + (size_t)task * a.nsrc + s);
 }
 
 // ----------------------------------------------------------------------------

@@ -4,7 +4,7 @@
 // FFTs (plan 0: two passes of composite in-register radices; plan 1: greedy
 // radix 8/4/2/3) and the tile:
 //   column stages  tile 0: widest column tile within the LDS budget,
-//                  tile 1: half of it;
+//                  tile 1: half of it, tile 2: widest tile on 128 threads;
 //   row stages     tile 0: fused row kernel with the most rows per
 //                          workgroup that fit, tile 1: fused with one row,
 //                  tile 2: split path (x-IFFT to HBM, pointwise products,

@@ -1,8 +1,16 @@
 // Column passes, split-row transforms and row synthesis: launcher
-// instantiations for FP64 and FP32 tiles.
+// instantiations for FP64 and FP32 tiles, split by GLW_COL_SET into units the
+// build compiles in parallel (0: FP64 column-inverse + synthesis,
+// 1: FP32 column-inverse, 2: FP64 column-forward + split rows, 3: FP32 ditto).
 #include "glw_launch_impl.hpp"
 
+#ifndef GLW_COL_SET
+#error "GLW_COL_SET must select a launcher set"
+#endif
+
 namespace glw {
+
+#if GLW_COL_SET == 0
 
 bool length_supported(int n) {
   switch (n) {
@@ -34,13 +42,20 @@ int launch_synth(int gx, const SynthArgs& a, int nseq, const void* tw, hipStream
   return GLW_OK;
 }
 
+template int launch_colinv<double>(int, const ColInvArgs&, int, const void*, const Exec&, hipStream_t);
+#elif GLW_COL_SET == 1
+template int launch_colinv<float>(int, const ColInvArgs&, int, const void*, const Exec&, hipStream_t);
+#else
 #define GLW_INST(T)                                                                                        \
-  template int launch_colinv<T>(int, const ColInvArgs&, int, const void*, const Exec&, hipStream_t);       \
   template int launch_colfwd<T>(int, const ColFwdArgs&, int, const void*, const Exec&, hipStream_t);       \
   template int launch_rowinv<T>(int, const RowArgs&, int, const void*, int, void*, hipStream_t);           \
   template int launch_rowfwd<T>(int, const RowArgs&, int, const void*, int, const void*, hipStream_t);
+#if GLW_COL_SET == 2
 GLW_INST(double)
+#else
 GLW_INST(float)
+#endif
 #undef GLW_INST
+#endif
 
 }  // namespace glw

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <string>
+#include <type_traits>
 
 #include "glw_internal.hpp"
 #include "glw_launch.hpp"
@@ -44,38 +45,51 @@ constexpr int split_ry() {
 }
 
 // ---------------------------------------------------------------- columns
-template <typename T, int L, int CW, int PLAN>
+template <typename T, int L, int CW, int PLAN, int NTH>
 void colinv_go(const ColInvArgs& a, int ntask, const void* tw, hipStream_t st) {
-  hipLaunchKernelGGL((k_colinv<T, L, CW, PLAN>), dim3((a.wx + CW - 1) / CW, a.nfield, ntask), dim3(NT), 0, st, a,
-                     (const cpx<T>*)tw);
+  hipLaunchKernelGGL((k_colinv<T, L, CW, PLAN, NTH>), dim3((a.wx + CW - 1) / CW, a.nfield, ntask), dim3(NTH), 0, st,
+                     a, (const cpx<T>*)tw);
 }
 
-template <typename T, int L, int CW, int PLAN>
+template <typename T, int L, int CW, int PLAN, int NTH>
 void colfwd_go(const ColFwdArgs& a, int ntask, const void* tw, hipStream_t st) {
-  hipLaunchKernelGGL((k_colfwd<T, L, CW, PLAN>), dim3((a.owx + CW - 1) / CW, a.nsrc, ntask), dim3(NT), 0, st, a,
-                     (const cpx<T>*)tw);
+  hipLaunchKernelGGL((k_colfwd<T, L, CW, PLAN, NTH>), dim3((a.owx + CW - 1) / CW, a.nsrc, ntask), dim3(NTH), 0, st,
+                     a, (const cpx<T>*)tw);
+}
+
+// Column tiles: 0 = widest tile, 256 threads; 1 = half tile, 256 threads;
+// 2 = widest tile, 128 threads (fewer idle lanes in the first and last stages).
+template <typename T, int L, int PLAN, class F>
+void col_tile(int tile, F&& go) {
+  constexpr int CW0 = col_width<T, L>();
+  constexpr int CW1 = CW0 > 1 ? CW0 / 2 : 1;
+  if (tile == 1) go(std::integral_constant<int, CW1>(), std::integral_constant<int, NT>());
+  else if (tile == 2) go(std::integral_constant<int, CW0>(), std::integral_constant<int, NT / 2>());
+  else go(std::integral_constant<int, CW0>(), std::integral_constant<int, NT>());
 }
 
 template <typename T, int L>
 void colinv_len(const ColInvArgs& a, int ntask, const void* tw, Sched s, hipStream_t st) {
-  constexpr int CW0 = col_width<T, L>();
-  constexpr int CW1 = CW0 > 1 ? CW0 / 2 : 1;
   constexpr int P1 = canonical_plan<L, 1>();
-  if (s.plan == 0)
-    s.tile == 0 ? colinv_go<T, L, CW0, 0>(a, ntask, tw, st) : colinv_go<T, L, CW1, 0>(a, ntask, tw, st);
-  else
-    s.tile == 0 ? colinv_go<T, L, CW0, P1>(a, ntask, tw, st) : colinv_go<T, L, CW1, P1>(a, ntask, tw, st);
+  auto run = [&](auto plan) {
+    col_tile<T, L, decltype(plan)::value>(s.tile, [&](auto cw, auto nth) {
+      colinv_go<T, L, decltype(cw)::value, decltype(plan)::value, decltype(nth)::value>(a, ntask, tw, st);
+    });
+  };
+  if (s.plan == 0) run(std::integral_constant<int, 0>());
+  else run(std::integral_constant<int, P1>());
 }
 
 template <typename T, int L>
 void colfwd_len(const ColFwdArgs& a, int ntask, const void* tw, Sched s, hipStream_t st) {
-  constexpr int CW0 = col_width<T, L>();
-  constexpr int CW1 = CW0 > 1 ? CW0 / 2 : 1;
   constexpr int P1 = canonical_plan<L, 1>();
-  if (s.plan == 0)
-    s.tile == 0 ? colfwd_go<T, L, CW0, 0>(a, ntask, tw, st) : colfwd_go<T, L, CW1, 0>(a, ntask, tw, st);
-  else
-    s.tile == 0 ? colfwd_go<T, L, CW0, P1>(a, ntask, tw, st) : colfwd_go<T, L, CW1, P1>(a, ntask, tw, st);
+  auto run = [&](auto plan) {
+    col_tile<T, L, decltype(plan)::value>(s.tile, [&](auto cw, auto nth) {
+      colfwd_go<T, L, decltype(cw)::value, decltype(plan)::value, decltype(nth)::value>(a, ntask, tw, st);
+    });
+  };
+  if (s.plan == 0) run(std::integral_constant<int, 0>());
+  else run(std::integral_constant<int, P1>());
 }
 
 template <typename T>

@@ -2013,9 +2013,9 @@ int enable_events(glw_plan* P, bool on) {
 }
 
 // Candidate schedules, tried all at once: pass k gives every row stage
-// kRowSched[k] and every column stage kColSched[k % 4].
+// kRowSched[k] and every column stage kColSched[k].
 constexpr Sched kRowSched[6] = {{0, 0}, {0, 1}, {0, 2}, {1, 0}, {1, 1}, {1, 2}};
-constexpr Sched kColSched[4] = {{0, 0}, {0, 1}, {1, 0}, {1, 1}};
+constexpr Sched kColSched[6] = {{0, 0}, {0, 1}, {0, 2}, {1, 0}, {1, 1}, {1, 2}};
 
 int autotune(glw_plan* P, int kind, const double* s0, const double* times, int nt, double* a, double* b, int reps,
              hipStream_t st) {
@@ -2031,7 +2031,7 @@ int autotune(glw_plan* P, int kind, const double* s0, const double* times, int n
   std::map<std::string, double> agg;
   for (int k = 0; k < 6; ++k) {
     for (const std::string& nm : names)
-      P->sched[nm] = P->stage_kind[nm] == STAGE_ROW ? kRowSched[k] : kColSched[k % 4];
+      P->sched[nm] = P->stage_kind[nm] == STAGE_ROW ? kRowSched[k] : kColSched[k];
     if ((rc = glw_execute_kind(P, kind, s0, times, nt, a, b, st))) return rc;  // warm-up, sizes scratch
     GLW_HIP(hipStreamSynchronize(st));
     P->ev_used = 0;

@@ -208,8 +208,9 @@ int glw_adjoint_execute(glw_plan* plan, const double* s0, const double* times, i
  * FFTW/MATLAB/NumPy FFTs whose planning is internal).  Runs `kind` of the
  * plan (reps timed repetitions per candidate, plus warm-ups) with every
  * candidate schedule -- radix sequence of the LDS FFTs (two-pass composite
- * radices or greedy radix 8/4/2/3) x tile (column tile width; fused row
- * kernel with the widest or a one-row tile, or the split row path) -- times
+ * radices or greedy radix 8/4/2/3) x tile (column tile width and threads per
+ * workgroup; fused row kernel with the widest or a one-row tile, or the split
+ * row path) -- times
  * every stage with HIP events and keeps, per stage, the fastest.  The
  * outputs a, b are overwritten.  Synchronizes the stream; not capturable. */
 int glw_plan_autotune(glw_plan* plan, int32_t kind, const double* s0, const double* times, int32_t nt, double* a,
