@@ -25,6 +25,10 @@ namespace {
 
 thread_local std::string g_err;
 
+// Row stride of column-pass intermediates: box width rounded up to 8 complex
+// values, so every 8-column tile starts on a 128-byte line.
+inline int sw(int w) { return (w + 7) & ~7; }
+
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
@@ -764,15 +768,15 @@ int build(glw_plan* P) {
     if ((rc = P->upload(&P->p22_coef, d.o2.psi_coef, (size_t)4))) return rc;
     if ((rc = P->upload(&P->p22_sign, d.o2.psi_sign, (size_t)2))) return rc;
     const size_t g2 = (size_t)d.g2y;
-    if ((rc = P->reserve("e22_t", nt * J * 8 * g2 * d.p1.wx * csz))) return rc;
-    if ((rc = P->reserve("e22_u", nt * J * 2 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("e22_t", nt * J * 8 * g2 * sw(d.p1.wx) * csz))) return rc;
+    if ((rc = P->reserve("e22_u", nt * J * 2 * g2 * sw(d.p2.wx) * csz))) return rc;
     if ((rc = P->reserve("e22_s", nt * J * 2 * b2 * zsz))) return rc;
     if ((rc = P->reserve("e22_max", nt * J * 2 * sizeof(uint64_t)))) return rc;
-    if ((rc = P->reserve("p22_t", nt * 4 * 16 * g2 * d.p1.wx * csz))) return rc;
-    if ((rc = P->reserve("p22_u", nt * 4 * 8 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("p22_t", nt * 4 * 16 * g2 * sw(d.p1.wx) * csz))) return rc;
+    if ((rc = P->reserve("p22_u", nt * 4 * 8 * g2 * sw(d.p2.wx) * csz))) return rc;
     if ((rc = P->reserve("p22_s", nt * 4 * 8 * b2 * zsz))) return rc;
-    if ((rc = P->reserve("c22_t", nt * 2 * g2 * d.p1.wx * csz))) return rc;
-    if ((rc = P->reserve("c22_u", nt * 1 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("c22_t", nt * 2 * g2 * sw(d.p1.wx) * csz))) return rc;
+    if ((rc = P->reserve("c22_u", nt * 1 * g2 * sw(d.p2.wx) * csz))) return rc;
     if ((rc = P->reserve("c22_s", nt * 1 * b2 * zsz))) return rc;
     if ((rc = P->reserve("ep22", nt * 2 * b2 * zsz))) return rc;
   }
@@ -792,15 +796,15 @@ int build(glw_plan* P) {
     if ((rc = P->upload(&P->out_sh, d.outer_sinhr, O * b3))) return rc;
     if ((rc = P->upload(&P->out_coef, d.outer_coef, O))) return rc;
     const size_t g2 = d.g2y, g3 = d.g3y;
-    const size_t wmax = std::max(d.p1.wx, d.p2.wx);
-    if ((rc = P->reserve("tin", nt * O * I * 12 * g2 * d.p1.wx * csz))) return rc;
-    if ((rc = P->reserve("uin", nt * O * I * 4 * g2 * d.p2.wx * csz))) return rc;
+    const size_t wmax = sw(std::max(d.p1.wx, d.p2.wx));
+    if ((rc = P->reserve("tin", nt * O * I * 12 * g2 * sw(d.p1.wx) * csz))) return rc;
+    if ((rc = P->reserve("uin", nt * O * I * 4 * g2 * sw(d.p2.wx) * csz))) return rc;
     if ((rc = P->reserve("sin", nt * O * I * 4 * b2 * zsz))) return rc;
     if ((rc = P->reserve("max_in", nt * O * I * 4 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("pair", nt * O * 4 * b2 * zsz))) return rc;
     if ((rc = P->reserve("max_et", nt * O * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("tout", nt * O * 21 * g3 * wmax * csz))) return rc;
-    if ((rc = P->reserve("uout", nt * O * 4 * g3 * d.p3.wx * csz))) return rc;
+    if ((rc = P->reserve("uout", nt * O * 4 * g3 * sw(d.p3.wx) * csz))) return rc;
     if ((rc = P->reserve("sout", nt * O * 4 * b3 * zsz))) return rc;
     if ((rc = P->reserve("max_out", nt * O * 4 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("max_combo", nt * O * 2 * sizeof(uint64_t)))) return rc;
@@ -840,10 +844,10 @@ int build(glw_plan* P) {
     if ((rc = P->upload(&P->o4_qsh, o.quartic_sinhr, J * b4))) return rc;
     if ((rc = P->upload(&P->o4_qco, o.quartic_coef, J))) return rc;
     const size_t g2 = d.g2y, g3 = d.g3y, g4 = o.g4y;
-    const size_t w12 = std::max(d.p1.wx, d.p2.wx), w123 = std::max<size_t>(w12, d.p3.wx);
+    const size_t w12 = sw(std::max(d.p1.wx, d.p2.wx)), w123 = sw(std::max<int>((int)w12, d.p3.wx));
     const size_t np = nt * J * J * J;  // pair tasks of the third level (the larger pair batch)
-    if ((rc = P->reserve("o4_pt", np * 16 * g2 * d.p1.wx * csz))) return rc;
-    if ((rc = P->reserve("o4_pu", np * 6 * g2 * d.p2.wx * csz))) return rc;
+    if ((rc = P->reserve("o4_pt", np * 16 * g2 * sw(d.p1.wx) * csz))) return rc;
+    if ((rc = P->reserve("o4_pu", np * 6 * g2 * sw(d.p2.wx) * csz))) return rc;
     if ((rc = P->reserve("o4_ps", np * 6 * b2 * zsz))) return rc;
     if ((rc = P->reserve("o4_pm", np * 6 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_set3", nt * J * J * 5 * b2 * zsz))) return rc;
@@ -851,19 +855,19 @@ int build(glw_plan* P) {
     if ((rc = P->reserve("o4_set4", nt * J * 5 * b2 * zsz))) return rc;
     if ((rc = P->reserve("o4_max4", nt * J * 5 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_tt", nt * J * J * 31 * g3 * w12 * csz))) return rc;
-    if ((rc = P->reserve("o4_tu", nt * J * J * 4 * g3 * d.p3.wx * csz))) return rc;
+    if ((rc = P->reserve("o4_tu", nt * J * J * 4 * g3 * sw(d.p3.wx) * csz))) return rc;
     if ((rc = P->reserve("o4_ts", nt * J * J * 4 * b3 * zsz))) return rc;
     if ((rc = P->reserve("o4_tm", nt * J * J * 4 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_th", nt * J * 3 * b3 * zsz))) return rc;
     if ((rc = P->reserve("o4_thm", nt * J * 3 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_qt", nt * J * 45 * g4 * w123 * csz))) return rc;
-    if ((rc = P->reserve("o4_qu", nt * J * 2 * g4 * o.p4.wx * csz))) return rc;
+    if ((rc = P->reserve("o4_qu", nt * J * 2 * g4 * sw(o.p4.wx) * csz))) return rc;
     if ((rc = P->reserve("o4_qs", nt * J * 2 * b4 * zsz))) return rc;
     if ((rc = P->reserve("o4_qm", nt * J * 2 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_out", nt * b4 * zsz))) return rc;
     if ((rc = P->reserve("o4_outm", nt * sizeof(uint64_t)))) return rc;
   }
-  if ((rc = P->reserve("tfin", nt * 2 * (size_t)d.ny * wfin * zsz))) return rc;
+  if ((rc = P->reserve("tfin", nt * 2 * (size_t)d.ny * sw(wfin) * zsz))) return rc;
   return GLW_OK;
 }
 
@@ -926,15 +930,15 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
   a.out = tfin;
   a.out_nfield = nout;
   a.out_foff = 0;
-  a.ostride = box.wx;
+  a.ostride = sw(box.wx);
   int rc = GLW_STAGE(P, "synth_colinv", st, launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), P->exec(STAGE_COL), st));
   if (rc) return rc;
   for (int f = 0; f < nout; ++f) {
     // rows of field f of every instant: sequence t starts at (t*nout + f)*ny*wx
     SynthArgs s;
-    s.in = tfin + (size_t)f * d.ny * box.wx;
-    s.in_seq = (size_t)nout * d.ny * box.wx;
-    s.istride = box.wx;
+    s.in = tfin + (size_t)f * d.ny * sw(box.wx);
+    s.in_seq = (size_t)nout * d.ny * sw(box.wx);
+    s.istride = sw(box.wx);
     s.mx0 = box.mx0;
     s.wx = box.wx;
     s.gy = d.ny;
@@ -978,7 +982,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     }
     a.out = P->ws.at("tin").first;
     a.out_nfield = 12;
-    a.ostride = d.p1.wx;
+    a.ostride = sw(d.p1.wx);
     if ((rc = GLW_STAGE(P, "o3_inner_colinv", st, launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
@@ -986,14 +990,14 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     memset(&r, 0, sizeof(r));
     r.in = P->ws.at("tin").first;
     r.nin = 12;
-    r.istride = d.p1.wx;
+    r.istride = sw(d.p1.wx);
     for (int f = 0; f < 12; ++f) {
       r.fmx0[f] = d.p1.mx0;
       r.fwx[f] = d.p1.wx;
     }
     r.out = P->ws.at("uin").first;
     r.nout = 4;
-    r.ostride = d.p2.wx;
+    r.ostride = sw(d.p2.wx);
     r.omx0 = d.p2.mx0;
     r.owx = d.p2.wx;
     r.gy = d.g2y;
@@ -1004,7 +1008,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     ColFwdArgs c;
     c.in = P->ws.at("uin").first;
     c.nsrc = 4;
-    c.istride = d.p2.wx;
+    c.istride = sw(d.p2.wx);
     c.omx0 = d.p2.mx0;
     c.owx = d.p2.wx;
     c.omy0 = d.p2.my0;
@@ -1036,7 +1040,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     GLW_HIP(hipGetLastError());
   }
   // ---- outer stage: tasks (t, o)
-  const int wmax = std::max(d.p1.wx, d.p2.wx);
+  const int wmax = sw(std::max(d.p1.wx, d.p2.wx));
   {
     ColInvArgs a = colinv_base(stp, 1, O, d.p1, P->t1);
     a.damp = P->damp_o;
@@ -1089,7 +1093,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     }
     r.out = P->ws.at("uout").first;
     r.nout = 4;
-    r.ostride = d.p3.wx;
+    r.ostride = sw(d.p3.wx);
     r.omx0 = d.p3.mx0;
     r.owx = d.p3.wx;
     r.gy = d.g3y;
@@ -1101,7 +1105,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     ColFwdArgs c;
     c.in = P->ws.at("uout").first;
     c.nsrc = 4;
-    c.istride = d.p3.wx;
+    c.istride = sw(d.p3.wx);
     c.omx0 = d.p3.mx0;
     c.owx = d.p3.wx;
     c.omy0 = d.p3.my0;
@@ -1167,14 +1171,14 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     memset(&r, 0, sizeof(r));
     r.in = P->ws.at(tin).first;
     r.nin = nin;
-    r.istride = d.p1.wx;
+    r.istride = sw(d.p1.wx);
     for (int f = 0; f < nin; ++f) {
       r.fmx0[f] = d.p1.mx0;
       r.fwx[f] = d.p1.wx;
     }
     r.out = P->ws.at(uout).first;
     r.nout = nout;
-    r.ostride = d.p2.wx;
+    r.ostride = sw(d.p2.wx);
     r.omx0 = d.p2.mx0;
     r.owx = d.p2.wx;
     r.gy = d.g2y;
@@ -1184,7 +1188,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     ColFwdArgs c;
     c.in = P->ws.at(uin).first;
     c.nsrc = nsrc;
-    c.istride = d.p2.wx;
+    c.istride = sw(d.p2.wx);
     c.omx0 = d.p2.mx0;
     c.owx = d.p2.wx;
     c.omy0 = d.p2.my0;
@@ -1209,7 +1213,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     }
     a.out = P->ws.at("e22_t").first;
     a.out_nfield = 8;
-    a.ostride = d.p1.wx;
+    a.ostride = sw(d.p1.wx);
     if ((rc = GLW_STAGE(P, "o2_eta_colinv", st, launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
     if ((rc = GLW_STAGE(P, "o2_eta_row", st, (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
@@ -1254,7 +1258,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
       a.out = P->ws.at("p22_t").first;
       a.out_nfield = 16;
       a.out_foff = 8 * ch;
-      a.ostride = d.p1.wx;
+      a.ostride = sw(d.p1.wx);
       if ((rc = GLW_STAGE(P, "o2_psi_colinv", st, launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
     }
     if ((rc = GLW_STAGE(P, "o2_psi_row", st, (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
@@ -1274,7 +1278,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.fscale[1] = sc;
     a.out = P->ws.at("c22_t").first;
     a.out_nfield = 2;
-    a.ostride = d.p1.wx;
+    a.ostride = sw(d.p1.wx);
     if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
     if ((rc = GLW_STAGE(P, "o2_contact_row", st, (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
@@ -1335,21 +1339,21 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     fields(a, 16, nullptr, pair_codes, sc);
     a.out = P->ws.at("o4_pt").first;
     a.out_nfield = 16;
-    a.ostride = d.p1.wx;
+    a.ostride = sw(d.p1.wx);
     if ((rc2 = GLW_STAGE(P, "o4_pair_colinv", st, launch_colinv<T>(d.g2y, a, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
       return rc2;
     RowArgs r;
     memset(&r, 0, sizeof(r));
     r.in = P->ws.at("o4_pt").first;
     r.nin = 16;
-    r.istride = d.p1.wx;
+    r.istride = sw(d.p1.wx);
     for (int f = 0; f < 16; ++f) {
       r.fmx0[f] = d.p1.mx0;
       r.fwx[f] = d.p1.wx;
     }
     r.out = P->ws.at("o4_pu").first;
     r.nout = 6;
-    r.ostride = d.p2.wx;
+    r.ostride = sw(d.p2.wx);
     r.omx0 = d.p2.mx0;
     r.owx = d.p2.wx;
     r.gy = d.g2y;
@@ -1360,7 +1364,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     ColFwdArgs c;
     c.in = P->ws.at("o4_pu").first;
     c.nsrc = 6;
-    c.istride = d.p2.wx;
+    c.istride = sw(d.p2.wx);
     c.omx0 = d.p2.mx0;
     c.owx = d.p2.wx;
     c.omy0 = d.p2.my0;
@@ -1402,7 +1406,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
   static const int pair_out_codes[18] = {M_ONE, M_I_QX, M_I_QY, M_ONE, M_I_QX, M_I_QY, M_I_QX,  M_I_QY,  M_A,
                                          M_Q2,  M_A,    M_I_QX, M_I_QY, M_Q2,  M_A,    M_I_QXA, M_I_QYA, M_Q2A};
   // ---- third level (:537-637): tasks (t, o4, o3)
-  const int w12 = std::max(d.p1.wx, d.p2.wx);
+  const int w12 = sw(std::max(d.p1.wx, d.p2.wx));
   {
     const int ntask = nt * J * J;
     ColInvArgs a = colinv_base(stp, 1, J * J, d.p1, P->t1);
@@ -1438,7 +1442,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     }
     r.out = P->ws.at("o4_tu").first;
     r.nout = 4;
-    r.ostride = d.p3.wx;
+    r.ostride = sw(d.p3.wx);
     r.omx0 = d.p3.mx0;
     r.owx = d.p3.wx;
     r.gy = d.g3y;
@@ -1449,7 +1453,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     ColFwdArgs c;
     c.in = P->ws.at("o4_tu").first;
     c.nsrc = 4;
-    c.istride = d.p3.wx;
+    c.istride = sw(d.p3.wx);
     c.omx0 = d.p3.mx0;
     c.owx = d.p3.wx;
     c.omy0 = d.p3.my0;
@@ -1482,7 +1486,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     GLW_HIP(hipGetLastError());
   }
   // ---- quartic level (:696-750): tasks (t, o4)
-  const int w123 = std::max(w12, (int)d.p3.wx);
+  const int w123 = sw(std::max(w12, (int)d.p3.wx));
   {
     const int ntask = nt * J;
     ColInvArgs a = colinv_base(stp, 1, J, d.p1, P->t1);
@@ -1533,7 +1537,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     }
     r.out = P->ws.at("o4_qu").first;
     r.nout = 2;
-    r.ostride = o.p4.wx;
+    r.ostride = sw(o.p4.wx);
     r.omx0 = o.p4.mx0;
     r.owx = o.p4.wx;
     r.gy = o.g4y;
@@ -1544,7 +1548,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     ColFwdArgs c;
     c.in = P->ws.at("o4_qu").first;
     c.nsrc = 2;
-    c.istride = o.p4.wx;
+    c.istride = sw(o.p4.wx);
     c.omx0 = o.p4.mx0;
     c.owx = o.p4.wx;
     c.omy0 = o.p4.my0;
@@ -1719,7 +1723,7 @@ ColInvArgs col_idft(const cpx<double>* spec, int nsrc, const glw_box& box, const
   }
   a.out = out;
   a.out_nfield = nsrc;
-  a.ostride = box.wx;
+  a.ostride = sw(box.wx);
   return a;
 }
 
@@ -1760,25 +1764,25 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
   GLW_WS("adj_gs", (size_t)b1 * z);
   if (d.order >= 2) GLW_WS("adj_ones2", (size_t)b2);
   const int J = d.o2.eta_rank, O = d.outer_rank, I = d.inner_rank;
-  const int wmax = std::max(d.p1.wx, d.p2.wx);
+  const int wmax = sw(std::max(d.p1.wx, d.p2.wx));
   if (orders & 2) {
     GLW_WS("adj_e22", (size_t)b2 * z);
     GLW_WS("adj_s22", (size_t)J * 2 * b2 * z);
-    GLW_WS("adj_u22", (size_t)J * 2 * d.g2y * d.p2.wx * z);
-    GLW_WS("adj_t22", (size_t)J * 8 * d.g2y * d.p1.wx * z);
+    GLW_WS("adj_u22", (size_t)J * 2 * d.g2y * sw(d.p2.wx) * z);
+    GLW_WS("adj_t22", (size_t)J * 8 * d.g2y * sw(d.p1.wx) * z);
     GLW_WS("adj_x22", (size_t)J * 8 * b1 * z);
   }
   if (orders & 4) {
     GLW_WS("adj_e33", (size_t)b3 * z);
     GLW_WS("adj_s33", (size_t)O * 2 * b3 * z);
-    GLW_WS("adj_u33", (size_t)O * 2 * d.g3y * d.p3.wx * z);
+    GLW_WS("adj_u33", (size_t)O * 2 * d.g3y * sw(d.p3.wx) * z);
     GLW_WS("adj_t33", (size_t)O * 21 * d.g3y * wmax * z);
     GLW_WS("adj_x33a", (size_t)O * 12 * b1 * z);
     GLW_WS("adj_x33b", (size_t)O * 9 * b2 * z);
     GLW_WS("adj_pair", (size_t)O * 4 * b2 * z);
     GLW_WS("adj_sin", (size_t)O * I * 4 * b2 * z);
-    GLW_WS("adj_uin", (size_t)O * I * 4 * d.g2y * d.p2.wx * z);
-    GLW_WS("adj_tin", (size_t)O * I * 12 * d.g2y * d.p1.wx * z);
+    GLW_WS("adj_uin", (size_t)O * I * 4 * d.g2y * sw(d.p2.wx) * z);
+    GLW_WS("adj_tin", (size_t)O * I * 12 * d.g2y * sw(d.p1.wx) * z);
     GLW_WS("adj_xin", (size_t)O * I * 12 * b1 * z);
   }
 #undef GLW_WS
@@ -1846,12 +1850,12 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
       return rc;
     glw_box in_boxes[8], g_boxes[2] = {d.p2, d.p2};
     for (int f = 0; f < 8; ++f) in_boxes[f] = d.p1;
-    const RowArgs fwd = row_desc(P->ws.at("e22_t").first, 8, d.p1.wx, in_boxes, d.g2y);
-    const RowArgs gin = row_desc(B("adj_u22"), 2, d.p2.wx, g_boxes, d.g2y);
-    const RowArgs gout = row_grad_out(B("adj_t22"), 8, d.p1.wx, in_boxes, d.g2y);
+    const RowArgs fwd = row_desc(P->ws.at("e22_t").first, 8, sw(d.p1.wx), in_boxes, d.g2y);
+    const RowArgs gin = row_desc(B("adj_u22"), 2, sw(d.p2.wx), g_boxes, d.g2y);
+    const RowArgs gout = row_grad_out(B("adj_t22"), 8, sw(d.p1.wx), in_boxes, d.g2y);
     if ((rc = GLW_STAGE(P, "adj_o2_row", st, (launch_row_adjoint<double, 8, 2, Eta22Op>(d.g2x, fwd, gin, gout, J, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
       return rc;
-    const ColFwdArgs c = col_dft(B("adj_t22"), 8, 0, 8, d.p1.wx, d.p1, ones1, B("adj_x22"));
+    const ColFwdArgs c = col_dft(B("adj_t22"), 8, 0, 8, sw(d.p1.wx), d.p1, ones1, B("adj_x22"));
     if ((rc = GLW_STAGE(P, "adj_o2_colfwd", st, launch_colfwd<double>(d.g2y, c, J, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
       return rc;
     ColInvArgs fa = colinv_base(stp, 1, J, d.p1, P->t1);
@@ -1891,7 +1895,7 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     glw_box fb[21], g3b[2] = {d.p3, d.p3};
     for (int f = 0; f < 21; ++f) fb[f] = f < 12 ? d.p1 : d.p2;
     const RowArgs fwd = row_desc(P->ws.at("tout").first, 21, wmax, fb, d.g3y);
-    const RowArgs gin = row_desc(B("adj_u33"), 2, d.p3.wx, g3b, d.g3y);
+    const RowArgs gin = row_desc(B("adj_u33"), 2, sw(d.p3.wx), g3b, d.g3y);
     const RowArgs gout = row_grad_out(B("adj_t33"), 21, wmax, fb, d.g3y);
     if ((rc = GLW_STAGE(P, "adj_o3_outer_row", st, (launch_row_adjoint<double, 21, 2, OuterOp>(d.g3x, fwd, gin, gout, O, P->tw(d.g3x, false), P->exec(STAGE_ROW), st)))))
       return rc;
@@ -1956,12 +1960,12 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
       return rc;
     glw_box ib[12], g2b[4] = {d.p2, d.p2, d.p2, d.p2};
     for (int f = 0; f < 12; ++f) ib[f] = d.p1;
-    const RowArgs ifwd = row_desc(P->ws.at("tin").first, 12, d.p1.wx, ib, d.g2y);
-    const RowArgs igin = row_desc(B("adj_uin"), 4, d.p2.wx, g2b, d.g2y);
-    const RowArgs igout = row_grad_out(B("adj_tin"), 12, d.p1.wx, ib, d.g2y);
+    const RowArgs ifwd = row_desc(P->ws.at("tin").first, 12, sw(d.p1.wx), ib, d.g2y);
+    const RowArgs igin = row_desc(B("adj_uin"), 4, sw(d.p2.wx), g2b, d.g2y);
+    const RowArgs igout = row_grad_out(B("adj_tin"), 12, sw(d.p1.wx), ib, d.g2y);
     if ((rc = GLW_STAGE(P, "adj_o3_inner_row", st, (launch_row_adjoint<double, 12, 4, InnerOp>(d.g2x, ifwd, igin, igout, O * I, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
       return rc;
-    const ColFwdArgs ci = col_dft(B("adj_tin"), 12, 0, 12, d.p1.wx, d.p1, ones1, B("adj_xin"));
+    const ColFwdArgs ci = col_dft(B("adj_tin"), 12, 0, 12, sw(d.p1.wx), d.p1, ones1, B("adj_xin"));
     if ((rc = GLW_STAGE(P, "adj_o3_inner_colfwd", st, launch_colfwd<double>(d.g2y, ci, O * I, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
       return rc;
     static const int icodes[12] = {M_ONE, M_NU, M_NU2, M_NU3, M_QX_NU, M_QY_NU, M_QX, M_QY, M_Q2_NU, M_Q2, M_NU_QX, M_NU_QY};
