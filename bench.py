@@ -342,7 +342,7 @@ def bench_wit(args):
     name = f"n{args.modes:04d}"
     inp = formats.read_dir4in(ROOT / "tests" / "golden" / f"wit_{name}_input.bin")
     plan = wit.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
-                       dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
+                       dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"], precision=args.precision)
     for _ in range(args.warmup):
         out, cnt = plan.sector44_device(inp["amplitudes"])
     torch.cuda.synchronize()
@@ -352,6 +352,15 @@ def bench_wit(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     c = cnt.cpu().numpy()
+    vs64 = None
+    if args.precision == "fp32":  # the FP64 sum of the same quartets, untimed
+        p64 = wit.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                          dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
+        o64, _ = p64.sector44_device(inp["amplitudes"])
+        torch.cuda.synchronize()
+        vs64 = {"eta": rel_l2(out[0].cpu().numpy(), o64[0].cpu().numpy()),
+                "psi": rel_l2(out[1].cpu().numpy(), o64[1].cpu().numpy())}
+        p64.close()
     cpu = None
     if not args.no_cpu:
         from oracle import wit as ow
@@ -374,7 +383,9 @@ def bench_wit(args):
         "metric": f"WIT sector-44 quartets/s, M={inp['modes'].shape[0]}", "value": c[1] * args.steps / wall,
         "unit": "quartets/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": f"reference frozen input paper/order4/inputs/runtime/{name} (copied as a fixture)",
+        "dtype": "f64" if args.precision == "fp64" else "f32 evaluation, f64 accumulation",
+        "rel_l2_vs_fp64": vs64,
+        "data": f"reference frozen input paper/order4/inputs/runtime/{name} (copied as a fixture)",
         "config": {"workload": f"WIT sector 44, M={inp['modes'].shape[0]}, {inp['nx']}x{inp['ny']}",
                    "quartets": int(c[0])},
         "cpu_baseline": cpu,

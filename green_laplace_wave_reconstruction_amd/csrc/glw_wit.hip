@@ -46,25 +46,34 @@ int wfail(int code, const std::string& m) { return glw_report_error(code, m); }
 
 constexpr int WNT = 256;
 
-struct Rec {
-  cpx<double> eta, phi, sh0, sh1, sv;
-  double kx, ky, om, q, k2;
-  double zero_src;  // strict-zero source magnitude of this subset (0 if not strict zero)
-  double zero;      // 1 if this subset is a strict zero, else 0
+// Subset record: state of one subset (multiset of parents) in arithmetic T.
+template <typename T>
+struct RecT {
+  cpx<T> eta, phi, sh0, sh1, sv;
+  T kx, ky, om, q, k2;
+  T zero_src;  // strict-zero source magnitude of this subset (0 if not strict zero)
+  T zero;      // 1 if this subset is a strict zero, else 0
 };
+using Rec = RecT<double>;
 
-__device__ __forceinline__ cpx<double> cm(cpx<double> a, cpx<double> b) { return glw::cmul(a, b); }
-__device__ __forceinline__ cpx<double> ca(cpx<double> a, cpx<double> b) { return glw::cadd(a, b); }
-__device__ __forceinline__ cpx<double> cs(cpx<double> a, double s) { return {a.x * s, a.y * s}; }
-__device__ __forceinline__ cpx<double> ci(double s, cpx<double> a) { return {-s * a.y, s * a.x}; }  // (i s) a
-__device__ __forceinline__ double cabs2(cpx<double> a) { return hypot(a.x, a.y); }
+template <typename T>
+__device__ __forceinline__ cpx<T> cm(cpx<T> a, cpx<T> b) { return glw::cmul(a, b); }
+template <typename T>
+__device__ __forceinline__ cpx<T> ca(cpx<T> a, cpx<T> b) { return glw::cadd(a, b); }
+template <typename T>
+__device__ __forceinline__ cpx<T> cs(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> ci(T s, cpx<T> a) { return {-s * a.y, s * a.x}; }  // (i s) a
+template <typename T>
+__device__ __forceinline__ T cabs2(cpx<T> a) { return hypot(a.x, a.y); }
 
 __host__ __device__ constexpr int popc(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1) + ((m >> 3) & 1); }
 __host__ __device__ constexpr int lowbit(int m) { return m & (-m); }
 
-__device__ __forceinline__ double zd(const Rec& r, int n) {
+template <typename T>
+__device__ __forceinline__ T zd(const RecT<T>& r, int n) {
   switch (n) {
-    case 0: return 1.0;
+    case 0: return T(1);
     case 1: return r.q;
     case 2: return r.k2;
     case 3: return r.k2 * r.q;
@@ -74,100 +83,112 @@ __device__ __forceinline__ double zd(const Rec& r, int n) {
 
 struct Model {
   double g, h, thr;
-  __device__ double q(double kx, double ky, double* k2) const {
-    const double k = hypot(kx, ky);
+  template <typename T>
+  __device__ T q(T kx, T ky, T* k2) const {
+    const T k = hypot(kx, ky);
     *k2 = k * k;
-    return k * tanh(h * k);
+    return k * tanh(T(h) * k);
   }
 };
 
+template <typename T>
 struct TopOut {
-  cpx<double> eta, psi, phi, src0, src1, retained;
-  double rho;
+  cpx<T> eta, psi, phi, src0, src1, retained;
+  T rho;
   bool zero, near;
-  double zero_src;
+  T zero_src;
+};
+
+template <typename T>
+struct Limits;
+template <>
+struct Limits<double> {
+  static constexpr double eps = 2.220446049250313e-16, tiny = 2.2250738585072014e-308;
+};
+template <>
+struct Limits<float> {
+  static constexpr float eps = 1.1920928955078125e-7f, tiny = 1.1754943508222875e-38f;
 };
 
 // State of the full set FULL = (1<<D)-1 from the records of its proper
 // subsets.  `rec(m)` returns the record of local sub-mask m.
-template <int D, class Acc>
-__device__ __forceinline__ void subset_state(const Acc& rec, const Rec& S, const Model& md, TopOut& o) {
+template <int D, class Acc, typename T>
+__device__ __forceinline__ void subset_state(const Acc& rec, const RecT<T>& S, const Model& md, TopOut<T>& o) {
   constexpr int FULL = (1 << D) - 1;
-  cpx<double> horiz = {0, 0}, kin = {0, 0}, taylor = {0, 0}, lv = {0, 0}, lt = {0, 0};
+  const T g = T(md.g);
+  cpx<T> horiz = {0, 0}, kin = {0, 0}, taylor = {0, 0}, lv = {0, 0}, lt = {0, 0};
 #pragma unroll
   for (int L = 1; L < FULL; ++L) {
     const int R = FULL ^ L;
-    const Rec& a = rec(L);
-    const Rec& b = rec(R);
+    const RecT<T>& a = rec(L);
+    const RecT<T>& b = rec(R);
     horiz = ca(horiz, cm(ca(ci(b.kx, a.sh0), ci(b.ky, a.sh1)), b.eta));
-    kin = ca(kin, cs(ca(ca(cm(a.sh0, b.sh0), cm(a.sh1, b.sh1)), cm(a.sv, b.sv)), 0.5));
+    kin = ca(kin, cs(ca(ca(cm(a.sh0, b.sh0), cm(a.sh1, b.sh1)), cm(a.sv, b.sv)), T(0.5)));
   }
 #pragma unroll
   for (int P = 1; P < FULL; ++P) {
     const int C = FULL ^ P;
-    const Rec& p = rec(P);
-    cpx<double> t = {0, 0}, v = {0, 0};
-    const cpx<double> e1 = rec(C).eta;
+    const RecT<T>& p = rec(P);
+    cpx<T> t = {0, 0}, v = {0, 0};
+    const cpx<T> e1 = rec(C).eta;
     t = ca(t, cs(e1, zd(p, 1)));
     v = ca(v, cs(e1, zd(p, 2)));
-    if constexpr (true) {
-      if (popc(C) == 2) {
-        const int a = lowbit(C), b = C ^ a;
-        const cpx<double> e2 = cm(rec(a).eta, rec(b).eta);
-        t = ca(t, cs(e2, zd(p, 2)));
-        v = ca(v, cs(e2, zd(p, 3)));
-      } else if (popc(C) == 3) {
-        const int a = lowbit(C), b = lowbit(C ^ a), c = C ^ a ^ b;
-        const cpx<double> e2 = ca(ca(cm(rec(a).eta, rec(b | c).eta), cm(rec(b).eta, rec(a | c).eta)),
-                                  cm(rec(c).eta, rec(a | b).eta));
-        const cpx<double> e3 = cm(cm(rec(a).eta, rec(b).eta), rec(c).eta);
-        t = ca(ca(t, cs(e2, zd(p, 2))), cs(e3, zd(p, 3)));
-        v = ca(ca(v, cs(e2, zd(p, 3))), cs(e3, zd(p, 4)));
-      }
+    if (popc(C) == 2) {
+      const int a = lowbit(C), b = C ^ a;
+      const cpx<T> e2 = cm(rec(a).eta, rec(b).eta);
+      t = ca(t, cs(e2, zd(p, 2)));
+      v = ca(v, cs(e2, zd(p, 3)));
+    } else if (popc(C) == 3) {
+      const int a = lowbit(C), b = lowbit(C ^ a), c = C ^ a ^ b;
+      const cpx<T> e2 = ca(ca(cm(rec(a).eta, rec(b | c).eta), cm(rec(b).eta, rec(a | c).eta)),
+                           cm(rec(c).eta, rec(a | b).eta));
+      const cpx<T> e3 = cm(cm(rec(a).eta, rec(b).eta), rec(c).eta);
+      t = ca(ca(t, cs(e2, zd(p, 2))), cs(e3, zd(p, 3)));
+      v = ca(ca(v, cs(e2, zd(p, 3))), cs(e3, zd(p, 4)));
     }
-    const cpx<double> pt = cm(p.phi, t);
+    const cpx<T> pt = cm(p.phi, t);
     taylor = ca(taylor, pt);
     lv = ca(lv, cm(p.phi, v));
     lt = ca(lt, ci(-p.om, pt));
   }
   // source = [-(horiz - lv) - q taylor, -(lt + kin) - i Omega taylor]
   o.src0 = {-(horiz.x - lv.x) - S.q * taylor.x, -(horiz.y - lv.y) - S.q * taylor.y};
-  const cpx<double> it = ci(S.om, taylor);
+  const cpx<T> it = ci(S.om, taylor);
   o.src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
-  const double mag = sqrt(S.k2);
-  o.zero = mag <= 100.0 * 2.220446049250313e-16 * fmax(mag, 1.0);
+  const T mag = sqrt(S.k2);
+  o.zero = mag <= T(100) * Limits<T>::eps * fmax(mag, T(1));
   o.near = false;
-  o.rho = 0.0;
+  o.rho = T(0);
   o.retained = {0, 0};
-  o.zero_src = 0.0;
+  o.zero_src = T(0);
   if (o.zero) {
     o.eta = o.psi = {0, 0};
     o.zero_src = hypot(cabs2(o.src0), cabs2(o.src1));
   } else {
     bool solve = true;
     if constexpr (D == 4) {
-      const double lin = md.g * S.q, tmp = S.om * S.om;
-      o.rho = fabs(lin - tmp) / fmax(lin + tmp, 2.2250738585072014e-308);
-      if (o.rho < md.thr) {
-        const double w = sqrt(lin), ms = sqrt(w / (2.0 * md.g)), ns = sqrt(md.g / (2.0 * w));
-        const double sg = S.om >= 0.0 ? 1.0 : -1.0;
-        const cpx<double> a0 = cs(o.src0, 1.0 / (2.0 * ms));
-        const cpx<double> a1 = ci(sg, cs(o.src1, 1.0 / (2.0 * ns)));
+      const T lin = g * S.q, tmp = S.om * S.om;
+      o.rho = fabs(lin - tmp) / fmax(lin + tmp, Limits<T>::tiny);
+      if (o.rho < T(md.thr)) {
+        const T w = sqrt(lin), ms = sqrt(w / (T(2) * g)), ns = sqrt(g / (T(2) * w));
+        const T sg = S.om >= T(0) ? T(1) : T(-1);
+        const cpx<T> a0 = cs(o.src0, T(1) / (T(2) * ms));
+        const cpx<T> a1 = ci(sg, cs(o.src1, T(1) / (T(2) * ns)));
         o.retained = ca(a0, a1);
-        const cpx<double> comp = glw::csub(a0, a1);
-        const double den = S.om + sg * w;
-        o.eta = cs(ci(ms, comp), 1.0 / den);
+        const cpx<T> comp = glw::csub(a0, a1);
+        const T den = S.om + sg * w;
+        o.eta = cs(ci(ms, comp), T(1) / den);
         o.psi = cs(comp, -sg * ns / den);
         o.near = true;
         solve = false;
       }
     }
     if (solve) {
-      const double den = S.om * S.om - md.g * S.q;
-      const cpx<double> a = glw::csub(ci(S.om, o.src0), cs(o.src1, S.q));
-      const cpx<double> b = ca(cs(o.src0, md.g), ci(S.om, o.src1));
-      o.eta = cs(a, 1.0 / den);
-      o.psi = cs(b, 1.0 / den);
+      const T den = S.om * S.om - g * S.q;
+      const cpx<T> a = glw::csub(ci(S.om, o.src0), cs(o.src1, S.q));
+      const cpx<T> b = ca(cs(o.src0, g), ci(S.om, o.src1));
+      o.eta = cs(a, T(1) / den);
+      o.psi = cs(b, T(1) / den);
     }
   }
   o.phi = glw::csub(o.psi, taylor);
@@ -214,6 +235,9 @@ struct WitDev {
   Rec* single;           // [M]
   Rec* pair;             // [M(M+1)/2]
   Rec* triple;           // [C(M+2,3)]
+  RecT<float>* single32; // FP32 copies of the tables (FP32 quartet evaluation only)
+  RecT<float>* pair32;
+  RecT<float>* triple32;
   const long long* offP; // [M+1]  pairs with first index < i
   const long long* offT; // [M+1]  triples with first index < i
 };
@@ -223,8 +247,9 @@ __device__ __forceinline__ long long tidx(const WitDev& w, int i, int j, int k) 
   return w.offT[i] + (w.offP[j] - w.offP[i]) + (k - j);
 }
 
-__device__ void base_record(const WitDev& w, const Rec** sub, int D, Rec& S) {
-  double kx = 0, ky = 0, om = 0;
+template <typename T>
+__device__ void base_record(const WitDev& w, const RecT<T>** sub, int D, RecT<T>& S) {
+  T kx = 0, ky = 0, om = 0;
   for (int s = 0; s < D; ++s) {
     kx += sub[s]->kx;
     ky += sub[s]->ky;
@@ -254,18 +279,39 @@ __global__ void k_wit_single_fill(WitDev w, double dkx, double dky) {
   w.single[i] = r;
 }
 
-struct Acc2 {
-  const Rec* r[4];
-  __device__ const Rec& operator()(int m) const { return *r[m]; }
+template <typename T, int N>
+struct AccT {
+  const RecT<T>* r[N];
+  __device__ const RecT<T>& operator()(int m) const { return *r[m]; }
 };
-struct Acc3 {
-  const Rec* r[8];
-  __device__ const Rec& operator()(int m) const { return *r[m]; }
+using Acc2 = AccT<double, 4>;
+using Acc3 = AccT<double, 8>;
+
+template <typename T>
+struct Tables;
+template <>
+struct Tables<double> {
+  static __device__ const Rec* single(const WitDev& w) { return w.single; }
+  static __device__ const Rec* pair(const WitDev& w) { return w.pair; }
+  static __device__ const Rec* triple(const WitDev& w) { return w.triple; }
 };
-struct Acc4 {
-  const Rec* r[16];
-  __device__ const Rec& operator()(int m) const { return *r[m]; }
+template <>
+struct Tables<float> {
+  static __device__ const RecT<float>* single(const WitDev& w) { return w.single32; }
+  static __device__ const RecT<float>* pair(const WitDev& w) { return w.pair32; }
+  static __device__ const RecT<float>* triple(const WitDev& w) { return w.triple32; }
 };
+
+__device__ __forceinline__ RecT<float> to_f32(const Rec& r) {
+  auto c = [](cpx<double> v) { return cpx<float>{(float)v.x, (float)v.y}; };
+  return RecT<float>{c(r.eta), c(r.phi), c(r.sh0), c(r.sh1), c(r.sv), (float)r.kx, (float)r.ky, (float)r.om,
+                     (float)r.q, (float)r.k2, (float)r.zero_src, (float)r.zero};
+}
+
+__global__ void k_wit_to_f32(const Rec* src, RecT<float>* dst, long long n) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = to_f32(src[i]);
+}
 
 __global__ void k_wit_pairs(WitDev w) {
   const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -284,7 +330,7 @@ __global__ void k_wit_pairs(WitDev w) {
   const Rec* sub[2] = {a.r[1], a.r[2]};
   Rec S;
   base_record(w, sub, 2, S);
-  TopOut o;
+  TopOut<double> o;
   subset_state<2>(a, S, w.md, o);
   S.eta = o.eta;
   S.phi = o.phi;
@@ -321,7 +367,7 @@ __global__ void k_wit_triples(WitDev w, long long nt) {
   const Rec* sub[3] = {a.r[1], a.r[2], a.r[4]};
   Rec S;
   base_record(w, sub, 3, S);
-  TopOut o;
+  TopOut<double> o;
   subset_state<3>(a, S, w.md, o);
   S.eta = o.eta;
   S.phi = o.phi;
@@ -359,6 +405,11 @@ __device__ __forceinline__ double wmin_d(double v) {
 }
 
 // One workgroup per leading pair (i, j); threads sweep the (k, l) triangle.
+// T = arithmetic of the degree-4 evaluation (the subset tables are built in
+// FP64; the FP32 path reads FP32 copies).  Weights and accumulation are FP64
+// in both: every FP32 term is exact in FP64 and the FP64 sums bound the
+// summation error far below a compensated FP32 sum.
+template <typename T>
 __global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
   const WitDev& w = g.w;
   const long long p = blockIdx.x;
@@ -390,44 +441,48 @@ __global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
       continue;
     }
     if (abs(tx) >= w.nx / 2 || abs(ty) >= w.ny / 2) continue;
-    Acc4 a;
-    a.r[1] = &w.single[i];
-    a.r[2] = &w.single[j];
-    a.r[4] = &w.single[k];
-    a.r[8] = &w.single[l];
-    a.r[3] = &w.pair[pidx(w, i, j)];
-    a.r[5] = &w.pair[pidx(w, i, k)];
-    a.r[9] = &w.pair[pidx(w, i, l)];
-    a.r[6] = &w.pair[pidx(w, j, k)];
-    a.r[10] = &w.pair[pidx(w, j, l)];
-    a.r[12] = &w.pair[pidx(w, k, l)];
-    a.r[7] = &w.triple[tidx(w, i, j, k)];
-    a.r[11] = &w.triple[tidx(w, i, j, l)];
-    a.r[13] = &w.triple[tidx(w, i, k, l)];
-    a.r[14] = &w.triple[tidx(w, j, k, l)];
-    const Rec* sub[4] = {a.r[1], a.r[2], a.r[4], a.r[8]};
-    Rec S;
+    const RecT<T>* single = Tables<T>::single(w);
+    const RecT<T>* pair = Tables<T>::pair(w);
+    const RecT<T>* triple = Tables<T>::triple(w);
+    AccT<T, 16> a;
+    a.r[1] = &single[i];
+    a.r[2] = &single[j];
+    a.r[4] = &single[k];
+    a.r[8] = &single[l];
+    a.r[3] = &pair[pidx(w, i, j)];
+    a.r[5] = &pair[pidx(w, i, k)];
+    a.r[9] = &pair[pidx(w, i, l)];
+    a.r[6] = &pair[pidx(w, j, k)];
+    a.r[10] = &pair[pidx(w, j, l)];
+    a.r[12] = &pair[pidx(w, k, l)];
+    a.r[7] = &triple[tidx(w, i, j, k)];
+    a.r[11] = &triple[tidx(w, i, j, l)];
+    a.r[13] = &triple[tidx(w, i, k, l)];
+    a.r[14] = &triple[tidx(w, j, k, l)];
+    const RecT<T>* sub[4] = {a.r[1], a.r[2], a.r[4], a.r[8]};
+    RecT<T> S;
     base_record(w, sub, 4, S);
-    TopOut o;
+    TopOut<T> o;
     subset_state<4>(a, S, w.md, o);
     // strict-zero intermediates of the ten proper subsets of degree >= 2
     double zcount = 0.0, zsrc = 0.0;
     constexpr int kInner[10] = {3, 5, 6, 9, 10, 12, 7, 11, 13, 14};
 #pragma unroll
     for (int q = 0; q < 10; ++q) {
-      zcount += a.r[kInner[q]]->zero;
-      zsrc = fmax(zsrc, a.r[kInner[q]]->zero_src);
+      zcount += (double)a.r[kInner[q]]->zero;
+      zsrc = fmax(zsrc, (double)a.r[kInner[q]]->zero_src);
     }
     // paired residual (:413-429)
-    cpx<double> er = glw::csub(glw::csub(ci(-S.om, o.eta), cs(o.psi, S.q)), o.src0);
-    cpx<double> pr = glw::csub(ca(ci(-S.om, o.psi), cs(o.eta, w.md.g)), o.src1);
+    const T gT = T(w.md.g);
+    cpx<T> er = glw::csub(glw::csub(ci(-S.om, o.eta), cs(o.psi, S.q)), o.src0);
+    cpx<T> pr = glw::csub(ca(ci(-S.om, o.psi), cs(o.eta, gT)), o.src1);
     if (o.near) {
-      const double ww = sqrt(w.md.g * S.q), ms = sqrt(ww / (2.0 * w.md.g)), ns = sqrt(w.md.g / (2.0 * ww));
-      const double sg = S.om >= 0.0 ? 1.0 : -1.0;
+      const T ww = sqrt(gT * S.q), ms = sqrt(ww / (T(2) * gT)), ns = sqrt(gT / (T(2) * ww));
+      const T sg = S.om >= T(0) ? T(1) : T(-1);
       er = ca(er, cs(o.retained, ms));
       pr = ca(pr, ci(-sg * ns, o.retained));
     }
-    const double res = hypot(cabs2(er), cabs2(pr));
+    const double res = (double)hypot(cabs2(er), cabs2(pr));
     // repeated-index factor (:486-499)
     const int id[4] = {i, j, k, l};
     int fac = 1, run = 1;
@@ -442,20 +497,22 @@ __global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
     }
     for (int v = 2; v <= run; ++v) fac *= v;
     const size_t ix = (size_t)(((tx % w.nx) + w.nx) % w.nx), iy = (size_t)(((ty % w.ny) + w.ny) % w.ny);
+    const cpx<double> eta = {(double)o.eta.x, (double)o.eta.y}, psi = {(double)o.psi.x, (double)o.psi.y};
+    const cpx<double> ret = {(double)o.retained.x, (double)o.retained.y};
     for (int c = 0; c < g.cases; ++c) {
       const cpx<double>* ac = g.amps + (size_t)c * w.M;
       cpx<double> wt = cm(cm(cm(cs(ac[i], 2.0), ac[j]), ac[k]), ac[l]);
       wt = cs(wt, 1.0 / (double)fac);
       const size_t loc = (size_t)c * w.nx * w.ny + ix + (size_t)w.nx * iy;
-      atomic_addc(g.out + 2 * loc, cm(wt, o.eta));
-      atomic_addc(g.out + 2 * (plane + loc), cm(wt, o.psi));
-      if (o.near) atomic_addc(g.out + 2 * (4 * plane + loc), cm(wt, o.retained));
+      atomic_addc(g.out + 2 * loc, cm(wt, eta));
+      atomic_addc(g.out + 2 * (plane + loc), cm(wt, psi));
+      if (o.near) atomic_addc(g.out + 2 * (4 * plane + loc), cm(wt, ret));
     }
     ++kept;
     nearc += o.near ? 1 : 0;
     zi += (unsigned long long)zcount;
     maxres = fmax(maxres, res);
-    mindet = fmin(mindet, o.rho);
+    mindet = fmin(mindet, (double)o.rho);
     maxz = fmax(maxz, zsrc);
   }
   kept = wsum_u64(kept);
@@ -523,6 +580,7 @@ int glw_wit_create(const glw_wit_desc* d, glw_wit** out) {
   *out = nullptr;
   if (d->modes_count < 1 || !d->modes || d->nx < 2 || d->ny < 2 || d->gravity <= 0 || d->depth <= 0)
     return wfail(GLW_EINVAL, "invalid WIT descriptor");
+  if (d->precision != GLW_PREC_FP64 && d->precision != GLW_PREC_FP32) return wfail(GLW_EINVAL, "bad WIT precision");
   glw_wit* W = new glw_wit_s();
   W->d = *d;
   int rc = GLW_OK;
@@ -578,6 +636,23 @@ int glw_wit_create(const glw_wit_desc* d, glw_wit** out) {
   hipLaunchKernelGGL(k_wit_single_fill, dim3((M + 255) / 256), dim3(256), 0, 0, w, d->dkx, d->dky);
   hipLaunchKernelGGL(k_wit_pairs, dim3((np + 255) / 256), dim3(256), 0, 0, w);
   hipLaunchKernelGGL(k_wit_triples, dim3((nt + 255) / 256), dim3(256), 0, 0, w, nt);
+  w.single32 = w.pair32 = w.triple32 = nullptr;
+  if (d->precision == GLW_PREC_FP32) {  // FP32 copies of the FP64 tables
+    void *s32, *p32, *t32;
+    rc = W->alloc(&s32, sizeof(RecT<float>) * M);
+    if (!rc) rc = W->alloc(&p32, sizeof(RecT<float>) * np);
+    if (!rc) rc = W->alloc(&t32, sizeof(RecT<float>) * nt);
+    if (rc) {
+      delete W;
+      return rc;
+    }
+    w.single32 = (RecT<float>*)s32;
+    w.pair32 = (RecT<float>*)p32;
+    w.triple32 = (RecT<float>*)t32;
+    hipLaunchKernelGGL(k_wit_to_f32, dim3((M + 255) / 256), dim3(256), 0, 0, w.single, w.single32, (long long)M);
+    hipLaunchKernelGGL(k_wit_to_f32, dim3((np + 255) / 256), dim3(256), 0, 0, w.pair, w.pair32, np);
+    hipLaunchKernelGGL(k_wit_to_f32, dim3((nt + 255) / 256), dim3(256), 0, 0, w.triple, w.triple32, nt);
+  }
   if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
     delete W;
     return wfail(GLW_EDEVICE, "WIT subset tables failed");
@@ -605,7 +680,10 @@ int glw_wit_sector44(glw_wit* W, const double* amps, int32_t cases, double* out,
   g.cnt = W->cnt;
   g.dmax = W->dmax;
   const long long M = W->d.modes_count;
-  hipLaunchKernelGGL(k_wit_quartets, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), 0, st, g);
+  if (W->d.precision == GLW_PREC_FP32)
+    hipLaunchKernelGGL(k_wit_quartets<float>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), 0, st, g);
+  else
+    hipLaunchKernelGGL(k_wit_quartets<double>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), 0, st, g);
   const double total = (double)M * (M + 1) * (M + 2) * (M + 3) / 24.0;
   hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, total, counters);
   W_HIP(hipGetLastError());

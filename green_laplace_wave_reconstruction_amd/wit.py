@@ -25,7 +25,7 @@ COUNTERS = ("total", "kept", "near", "strict_zero_output", "strict_zero_intermed
 class CWitDesc(ctypes.Structure):
     _fields_ = [("modes_count", c_int32), ("modes", POINTER(c_int32)), ("nx", c_int32), ("ny", c_int32),
                 ("gravity", c_double), ("depth", c_double), ("dkx", c_double), ("dky", c_double),
-                ("threshold", c_double)]
+                ("threshold", c_double), ("precision", c_int32)]
 
 
 def _bind(lib):
@@ -40,7 +40,7 @@ def _bind(lib):
 
 class WitPlan:
     def __init__(self, modes, *, nx: int, ny: int, gravity: float, depth: float, dkx: float, dky: float,
-                 threshold: float):
+                 threshold: float, precision: str = "fp64"):
         import torch
 
         if not torch.cuda.is_available():
@@ -51,8 +51,11 @@ class WitPlan:
             raise ValueError("at most 4096 modes (triple table size)")
         self.modes = m
         self.nx, self.ny = int(nx), int(ny)
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
         d = CWitDesc(m.shape[0], m.ctypes.data_as(POINTER(c_int32)), self.nx, self.ny, gravity, depth, dkx, dky,
-                     threshold)
+                     threshold, 0 if precision == "fp64" else 1)
         h = c_void_p()
         _check(self.lib.glw_wit_create(ctypes.byref(d), ctypes.byref(h)))
         self._h = h

@@ -161,6 +161,8 @@ typedef struct glw_wit_desc {
   int32_t nx, ny;               /* output grid */
   double gravity, depth, dkx, dky;
   double threshold;             /* near-resonance threshold on rho */
+  int32_t precision;            /* GLW_PREC_FP64, or GLW_PREC_FP32: degree-4 evaluation in FP32 from FP32
+                                   copies of the FP64 subset tables, FP64 weights and accumulation */
 } glw_wit_desc;
 
 typedef struct glw_wit_s glw_wit;

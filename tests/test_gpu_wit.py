@@ -68,3 +68,23 @@ def test_wit_homogeneity_degree_four(witmod):
     b = plan.sector44(2.0 * inp["amplitudes"])
     plan.close()
     assert rel(b["eta"], 16.0 * a["eta"]) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["n0050", "n0100", "n0382"])
+def test_wit_fp32_evaluation_vs_reference_outputs(witmod, name):
+    """FP32 degree-4 evaluation (FP32 copies of the FP64 subset tables), FP64 weights and accumulation.
+
+    Bound 1e-6 relative L2 (measured 2-3e-8): each quartet term carries FP32
+    rounding (~1e-7 relative); the sum itself is exact to FP64 rounding.
+    """
+    inp = formats.read_dir4in(GOLDEN / f"wit_{name}_input.bin")
+    plan = witmod.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                          dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"], precision="fp32")
+    r = plan.sector44(inp["amplitudes"])
+    plan.close()
+    eta, psi, c = golden(name)
+    e1, e2 = rel(r["eta"], eta), rel(r["psi"], psi)
+    print(f"WIT {name} FP32: eta {e1:.2e} psi {e2:.2e}", flush=True)
+    assert e1 < 1e-6 and e2 < 1e-6
+    for k in ("total", "kept", "near", "strict_zero_output"):
+        assert r[k] == c[k], k
