@@ -44,3 +44,31 @@ def small_case():
                          [0.017 * np.exp(0.13j), 0.012 * np.exp(-0.41j), 0.009 * np.exp(0.77j), 0.006 * np.exp(-1.04j)]):
         s[b % n, a % n] = n * n * v
     return s, qx, qy
+
+
+# The reference's order-4 C++ smoke (paper/order4/prepare_order4_cpp_smoke.m:4-10): 32^2, three
+# modes, g = h = 1, dk = 0.5, peak kh = 1; GL-vs-WIT relative L2 of the physical eta44 recorded in
+# paper/order4/validation/cpp_matlab_smoke.json ("gl_wit_raw_relative_l2", GL6/8/10).
+SMOKE32_GL_WIT = {6: 0.028693176699293679, 8: 0.009226906849614044, 10: 0.0034847786460708808}
+
+
+def smoke32():
+    """(spectrum [32,32], qx, qy, WIT DIR4IN01-style dict) of the reference smoke case."""
+    n = 32
+    modes = np.array([[2, 0], [3, 1], [3, -1]], dtype=np.int32)
+    c = np.array([0.012, 0.007 * np.exp(0.31j), 0.005 * np.exp(-0.47j)])
+    s = np.zeros((n, n), dtype=complex)
+    for (mx, my), cc in zip(modes, c):
+        s[my % n, mx] = n * n * cc
+    ax = np.rint(np.fft.fftfreq(n) * n)
+    mx, my = np.meshgrid(ax, ax)
+    wit = dict(modes=modes, amplitudes=c[None, :], gravity=1.0, depth=1.0, dkx=0.5, dky=0.5, nx=n, ny=n,
+               threshold=1e-6)
+    return s, 0.5 * mx, 0.5 * my, wit
+
+
+def smoke_metric(gl_field, wit_eta):
+    """validate_order4_cpp_smoke.m:14,27: ||Re GL - real(ifft2(n^2 WIT))|| / ||real(ifft2(n^2 WIT))||."""
+    n = wit_eta.shape[0]
+    ref = np.real(np.fft.ifft2(wit_eta * n * n))
+    return float(np.linalg.norm(np.real(gl_field) - ref) / np.linalg.norm(ref))

@@ -61,3 +61,20 @@ def test_glrti_gle44_drop_in(glw, tmp_path):
     out = formats.read_gle44(tmp_path / "gl_out.bin")
     l2, _ = metrics(h * np.real(out["field"]), ref)
     assert out["rank"] == 8 and abs(l2 - N0382_METRICS[8][0]) < 1e-8 * N0382_METRICS[8][0]
+
+
+@pytest.mark.parametrize("rank", [6, 8, 10])
+def test_eta44_and_wit_reproduce_reference_smoke(glw, rank):
+    """GPU eta44 and GPU WIT reproduce the GL-vs-WIT metric of the reference's 32^2 three-mode smoke
+    (paper/order4/validation/cpp_matlab_smoke.json) to 1e-9 relative."""
+    from green_laplace_wave_reconstruction_amd import api, wit as W
+    from order4_cases import SMOKE32_GL_WIT, smoke32, smoke_metric
+
+    s, qx, qy, w = smoke32()
+    plan = W.WitPlan(w["modes"], nx=32, ny=32, gravity=1.0, depth=1.0, dkx=0.5, dky=0.5, threshold=1e-6)
+    r = plan.sector44(w["amplitudes"])
+    plan.close()
+    eta, _, _ = api.gl_pure_sum_order4(s, qx, qy, 1.0, rank)
+    m = smoke_metric(eta, r["eta"][0])
+    print(f"smoke GL{rank}: {m:.15f} vs {SMOKE32_GL_WIT[rank]:.15f}", flush=True)
+    assert abs(m - SMOKE32_GL_WIT[rank]) < 1e-9 * SMOKE32_GL_WIT[rank]

@@ -175,3 +175,19 @@ def test_eta44_homogeneity_degree_four():
     a = O.eta44(s * 4.0, qx, qy, 1.0, 4)
     b = O.eta44(s * 8.0, qx, qy, 1.0, 4)
     assert rel(b, 16.0 * a) < 1e-14
+
+
+@pytest.mark.parametrize("rank", [6, 8, 10])
+def test_eta44_and_wit_oracles_reproduce_reference_smoke(rank):
+    """GL-vs-WIT metric of the reference's 32^2 three-mode smoke (cpp_matlab_smoke.json), from the
+    eta44 restatement and the WIT restatement together (measured agreement ~1e-13 relative)."""
+    from oracle import wit as ow
+    from order4_cases import SMOKE32_GL_WIT, smoke32, smoke_metric
+
+    s, qx, qy, wit = smoke32()
+    r = ow.sector44(wit)
+    assert r["kept"] == 15 and r["near"] == 0 and r["max_residual"] < 1e-10
+    e = O.eta44(s, qx, qy, 1.0, rank)
+    e = e[0] if isinstance(e, tuple) else e
+    m = smoke_metric(e, r["eta"][0])
+    assert abs(m - SMOKE32_GL_WIT[rank]) < 1e-9 * SMOKE32_GL_WIT[rank]
