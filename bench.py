@@ -406,6 +406,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="order3: skip the FFT schedule autotuner")
+    ap.add_argument("--chunk", type=int, default=INSTANTS,
+                    help="order3: instants per plan evaluation (the 64 instants run in chunks of this size)")
     ap.add_argument("--schedule", default="", help="order3: use this FFT schedule (stage=plan/tile;...) instead of tuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
@@ -451,7 +453,7 @@ def main():
     host = PL.build_plan(spec, s.depth * kxg, s.depth * kyg, depth=s.depth, gravity=s.gravity, order=3,
                          peak_q=c["peak_depth"], precision=prec)
     host.omega = host.box1.take(omega)
-    dp = DevicePlan(host, host.box1.take(spec), max_instants=INSTANTS)
+    dp = DevicePlan(host, host.box1.take(spec), max_instants=args.chunk)
     times = torch.linspace(0.0, 10.0 * 12.0, INSTANTS, dtype=torch.float64, device=dp.device)
     shape = (INSTANTS, s.ny, s.nx)
     eta = torch.empty(shape, dtype=torch.complex128, device=dp.device)
@@ -518,7 +520,8 @@ def main():
         "config": {
             "workload": "GL order 3 (inner GL4 x outer GL4) eta33+psi33, 512x512, L=2000 m, h=30 m, Hs=6 m, "
                         "Tp=12 s, gamma=3.3, s=15, 3120 parents kh>0.5, 64 instants over [0, 120 s]",
-            "instants_per_step": INSTANTS, "grid": [s.ny, s.nx], "precision": args.precision,
+            "instants_per_step": INSTANTS, "instants_per_chunk": args.chunk, "grid": [s.ny, s.nx],
+            "precision": args.precision,
             "working_grids": {"quadratic": list(host.g2[::-1]), "cubic": list(host.g3[::-1])},
             "fft_schedule": schedule,
         },
