@@ -16,4 +16,9 @@ Pinning (what checks each function besides re-evaluating its own formula):
   tolerances, the linear closed form, exact homogeneity of degree 2 and 3, and
   (eta22) the ordered-pair brute force of the same GL kernel.
 * ``wit``: see ``oracle/wit_ref.c`` and its header.
+* ``modal_oracle`` (single-direction time path): the reference's two
+  MATLAB-generated time records (eta22/eta33 at 3e-13, eta20 at 8e-11), the
+  on-grid identity with the spatial graph at 3e-14, the off-grid ladder
+  convergence, degree-2/3 homogeneity, and the GL node sums against the
+  Laplace integrals they approximate.
 """
