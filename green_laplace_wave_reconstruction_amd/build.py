@@ -17,6 +17,8 @@ UNITS = [
     (CSRC / "glw_plan.hip", [], "glw_plan.o"),
     (CSRC / "glw_wit.hip", [], "glw_wit.o"),
     (CSRC / "glw_batch.hip", [], "glw_batch.o"),
+    (CSRC / "glw_modal.hip", [], "glw_modal.o"),
+    *[(CSRC / "glw_modal_launch.hip", [f"-DGLW_MODAL_SET={i}"], f"glw_modal_launch{i}.o") for i in range(4)],
     *[(CSRC / "glw_launch_cols.hip", [f"-DGLW_COL_SET={i}"], f"glw_launch_cols{i}.o") for i in range(4)],
     *[(CSRC / "glw_launch_rows.hip", [f"-DGLW_ROW_SET={i}"], f"glw_launch_rows{i}.o") for i in range(ROW_SETS)],
 ]

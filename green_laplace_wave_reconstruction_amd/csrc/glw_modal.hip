@@ -1,0 +1,359 @@
+// C ABI of the single-direction modal time path (glw_modal_execute,
+// glw_eta20_execute): workspace layout, twiddle tables, the kernel sequence
+// and the synthesis of the output records.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "glw_internal.hpp"
+#include "glw_modal.hpp"
+#include "glw_modal_launch.hpp"
+
+using glw::modal::Args;
+using glw::modal::C;
+using glw::modal::Dep;
+
+namespace {
+
+constexpr int kBlock = 256;
+
+int mfail(int code, const std::string& msg) { return glw_report_error(code, msg); }
+
+bool t_length_ok(int n) {
+  switch (n) {
+#define GLW_CASE(L) case L:
+    GLW_MODAL_T_LENGTHS(GLW_CASE)
+    return true;
+#undef GLW_CASE
+    default: return false;
+  }
+}
+bool q_length_ok(int n) {
+  switch (n) {
+#define GLW_CASE(L) case L:
+    GLW_MODAL_Q_LENGTHS(GLW_CASE)
+    return true;
+#undef GLW_CASE
+    default: return false;
+  }
+}
+
+// tw[m] = exp(-2 pi i m / n), m < n
+__global__ void k_modal_twiddles(C* tw, int n) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double s, c;
+  sincospi(2.0 * (double)m / (double)n, &s, &c);
+  tw[m] = {c, -s};
+}
+
+// Output records (modal_time.py:89-94, 253, 328): value at sample n of
+// sum_b coef_b exp(-2 pi i b n / N) over the kept bins, with the phase read
+// from the N-point table at (b n) mod N.
+struct SynthArgs {
+  int count, base, span, order;
+  const Dep* dep;
+  const C* tw_n;
+  const C* coef2;  // [2 span + 1], bins 2 base + w
+  const C* coef3;  // [3 span + 1], bins 3 base + w
+  const double* energy2;
+  const double* energy3;
+  double* eta11;
+  C* eta22;
+  C* eta33;
+  double* audit;
+};
+
+__device__ __forceinline__ C bin_sum(const C* coef, int nw, int b0, int n, int count, const C* tw) {
+  double re = 0.0, im = 0.0;
+  long long idx = ((long long)b0 * n) % count;
+  for (int w = 0; w < nw; ++w) {
+    const C e = tw[idx];
+    const C c = coef[w];
+    re += c.x * e.x - c.y * e.y;
+    im += c.x * e.y + c.y * e.x;
+    idx += n;
+    if (idx >= count) idx -= count;
+  }
+  return {re, im};
+}
+
+__global__ void k_modal_synth(SynthArgs s) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && s.audit) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int w = 0; w < 2 * s.span + 1; ++w) {
+      v[0] += s.energy2[2 * w + 1];
+      v[1] += s.energy2[2 * w];
+    }
+    if (s.order == 3) {
+      for (int w = 0; w < 3 * s.span + 1; ++w) {
+        v[2] += s.energy3[2 * w + 1];
+        v[3] += s.energy3[2 * w];
+      }
+    }
+    for (int k = 0; k < 4; ++k) s.audit[k] = v[k];
+  }
+  if (n >= s.count) return;
+  if (s.eta11) {
+    double re = 0.0;
+    long long idx = ((long long)s.base * n) % s.count;
+    for (int w = 0; w <= s.span; ++w) {
+      const C e = s.tw_n[idx];
+      re += s.dep[w].re * e.x - s.dep[w].im * e.y;
+      idx += n;
+      if (idx >= s.count) idx -= s.count;
+    }
+    s.eta11[n] = re;
+  }
+  s.eta22[n] = bin_sum(s.coef2, 2 * s.span + 1, 2 * s.base, n, s.count, s.tw_n);
+  if (s.order == 3 && s.eta33) s.eta33[n] = bin_sum(s.coef3, 3 * s.span + 1, 3 * s.base, n, s.count, s.tw_n);
+}
+
+// eta20 record: Hermitian projection of the signed-bin coefficients
+// (_eta20.py:175-185) and the real part of their synthesis (_eta20.py:125):
+// c'_0 = Re c_0, c'_s = (c_s + conj c_-s) / 2, eta20 = c'_0 + 2 Re sum_s>0 c'_s e^{-2 pi i s n / N}.
+__global__ void k_eta20_synth(const C* coef, int span, int count, const C* tw_n, double* out) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= count) return;
+  double acc = 0.0;
+  long long idx = 0;
+  for (int sb = 1; sb <= span; ++sb) {
+    idx += n;
+    if (idx >= count) idx -= count;
+    const C cp = coef[span + sb], cm = coef[span - sb];
+    const double re = 0.5 * (cp.x + cm.x), im = 0.5 * (cp.y - cm.y);
+    const C e = tw_n[idx];
+    acc += re * e.x - im * e.y;
+  }
+  out[n] = coef[span].x + 2.0 * acc;
+}
+
+size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Layout {
+  size_t tw_t, tw_q, tw_n, src, pair, forcing, coef2, coef3, energy2, energy3, total;
+};
+
+Layout modal_layout(const glw_modal_desc& d) {
+  Layout L{};
+  const size_t cs = sizeof(C);
+  const size_t nw2 = 2 * (size_t)d.span + 1, nw3 = 3 * (size_t)d.span + 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align_up(bytes);
+    return at;
+  };
+  L.tw_t = take(d.t_len * cs);
+  L.tw_q = take(d.q_len * cs);
+  L.tw_n = take((size_t)d.count * cs);
+  L.src = take(2 * nw2 * d.q_len * cs);
+  L.pair = d.order == 3 ? take(glw::modal::kPairFields * nw2 * d.q_len * cs) : 0;
+  L.forcing = d.order == 3 ? take(2 * nw3 * d.q_len * cs) : 0;
+  L.coef2 = take(nw2 * cs);
+  L.coef3 = d.order == 3 ? take(nw3 * cs) : 0;
+  L.energy2 = take(2 * nw2 * sizeof(double));
+  L.energy3 = d.order == 3 ? take(2 * nw3 * sizeof(double)) : 0;
+  L.total = off;
+  return L;
+}
+
+int check_modal(const glw_modal_desc* d) {
+  if (!d) return mfail(GLW_EINVAL, "null modal descriptor");
+  if (d->order != 2 && d->order != 3) return mfail(GLW_EINVAL, "modal order must be 2 or 3");
+  if (d->count < 9 || d->base_bin < 1 || d->span < 0) return mfail(GLW_EINVAL, "modal record or band is empty");
+  if ((long long)d->order * (d->base_bin + d->span) >= d->count)
+    return mfail(GLW_EINVAL, "the nonlinear output band does not fit in the time record");
+  if (!t_length_ok(d->t_len)) return mfail(GLW_EUNSUPPORTED, "unsupported modal t_len " + std::to_string(d->t_len));
+  if (!q_length_ok(d->q_len)) return mfail(GLW_EUNSUPPORTED, "unsupported modal q_len " + std::to_string(d->q_len));
+  if (d->t_len < d->order * d->span + 1) return mfail(GLW_EINVAL, "t_len below order * span + 1");
+  if (d->kspan < 1 || d->k0 < 0 || d->q_len < d->order * d->kspan + 1)
+    return mfail(GLW_EINVAL, "q_len below order * kspan + 1 (or invalid deposit band)");
+  if (d->rank < 1 || d->rank > glw::modal::kMaxRank) return mfail(GLW_EINVAL, "GL rank must be 1..16");
+  if (!(d->dq > 0) || !(d->fstep > 0) || !(d->depth > 0) || !(d->lambda2 > 0) ||
+      (d->order == 3 && (!(d->lambda_inner > 0) || !(d->lambda3 > 0))))
+    return mfail(GLW_EINVAL, "modal spacing, frequency step, depth and GL scales must be positive");
+  return GLW_OK;
+}
+
+int twiddles(C* tw, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_modal_twiddles, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, tw, n);
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+int launched(int rc, const char* what) {
+  if (rc == 1) return mfail(GLW_EUNSUPPORTED, std::string(what) + ": unsupported length or launch shape");
+  if (rc != 0) return mfail(GLW_EDEVICE, std::string(what) + ": launch failed");
+  return GLW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int glw_modal_length_supported(int32_t n) { return (t_length_ok(n) ? 1 : 0) | (q_length_ok(n) ? 2 : 0); }
+
+int glw_modal_workspace_size(const glw_modal_desc* d, size_t* bytes) {
+  if (!bytes) return mfail(GLW_EINVAL, "null argument");
+  if (int rc = check_modal(d)) return rc;
+  *bytes = modal_layout(*d).total;
+  return GLW_OK;
+}
+
+int glw_modal_execute(const glw_modal_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,
+                      double* eta11, double* eta22, double* eta33, double* audit, void* stream) {
+  if (int rc = check_modal(d)) return rc;
+  if (!deposit || !workspace || !eta22 || (d->order == 3 && !eta33)) return mfail(GLW_EINVAL, "null device pointer");
+  const Layout L = modal_layout(*d);
+  if (workspace_bytes < L.total) return mfail(GLW_EINVAL, "modal workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  Args a{};
+  a.q_len = d->q_len;
+  a.t_len = d->t_len;
+  a.count = d->count;
+  a.base = d->base_bin;
+  a.span = d->span;
+  a.k0 = d->k0;
+  a.kspan = d->kspan;
+  a.order = d->order;
+  a.dq = d->dq;
+  a.fstep = d->fstep;
+  a.depth = d->depth;
+  a.dep = (const Dep*)deposit;
+  a.tw_t = (const C*)(ws + L.tw_t);
+  a.tw_q = (const C*)(ws + L.tw_q);
+  a.src = (C*)(ws + L.src);
+  a.pair = d->order == 3 ? (C*)(ws + L.pair) : nullptr;
+  a.forcing = d->order == 3 ? (C*)(ws + L.forcing) : nullptr;
+  a.rank = d->rank;
+  a.lam2 = d->lambda2;
+  a.lam_in = d->lambda_inner;
+  a.lam3 = d->lambda3;
+  for (int i = 0; i < glw::modal::kMaxRank; ++i) {
+    a.nodes[i] = i < d->rank ? d->nodes[i] : 0.0;
+    a.weights[i] = i < d->rank ? d->weights[i] : 0.0;
+  }
+  if (int rc = twiddles((C*)(ws + L.tw_t), d->t_len, st)) return rc;
+  if (int rc = twiddles((C*)(ws + L.tw_q), d->q_len, st)) return rc;
+  if (int rc = twiddles((C*)(ws + L.tw_n), d->count, st)) return rc;
+  const int nw2 = 2 * d->span + 1, nw3 = 3 * d->span + 1;
+  // order-2 sources, eta22 coefficients (+ pair fields)
+  if (int rc = launched(glw::modal::launch_t_order2(a, st), "modal time pass (order 2)")) return rc;
+  a.coef = (C*)(ws + L.coef2);
+  a.energy = (double*)(ws + L.energy2);
+  if (int rc = launched(glw::modal::launch_q_order2(a, nw2, st), "modal pass (eta22)")) return rc;
+  if (d->order == 3) {
+    if (int rc = launched(glw::modal::launch_t_order3(a, st), "modal time pass (order 3)")) return rc;
+    a.coef = (C*)(ws + L.coef3);
+    a.energy = (double*)(ws + L.energy3);
+    if (int rc = launched(glw::modal::launch_q_order3(a, nw3, st), "modal pass (eta33)")) return rc;
+  }
+  SynthArgs s{};
+  s.count = d->count;
+  s.base = d->base_bin;
+  s.span = d->span;
+  s.order = d->order;
+  s.dep = a.dep;
+  s.tw_n = (const C*)(ws + L.tw_n);
+  s.coef2 = (const C*)(ws + L.coef2);
+  s.coef3 = d->order == 3 ? (const C*)(ws + L.coef3) : nullptr;
+  s.energy2 = (const double*)(ws + L.energy2);
+  s.energy3 = d->order == 3 ? (const double*)(ws + L.energy3) : nullptr;
+  s.eta11 = eta11;
+  s.eta22 = (C*)eta22;
+  s.eta33 = (C*)eta33;
+  s.audit = audit;
+  hipLaunchKernelGGL(k_modal_synth, dim3((d->count + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+static int check_eta20(const glw_eta20_desc* d) {
+  if (!d) return mfail(GLW_EINVAL, "null eta20 descriptor");
+  if (d->count < 9 || d->base_bin < 1 || d->span < 0 || 2 * d->span + 1 > d->count)
+    return mfail(GLW_EINVAL, "eta20 record or band is invalid");
+  if (!t_length_ok(d->t_len) || d->t_len < 2 * d->span + 1)
+    return mfail(GLW_EUNSUPPORTED, "eta20 t_len unsupported or below 2 span + 1");
+  if (!q_length_ok(d->q_len) || d->kspan < 1 || d->q_len < 2 * d->kspan + 1)
+    return mfail(GLW_EUNSUPPORTED, "eta20 q_len unsupported or below 2 kspan + 1");
+  if (d->rank < 1 || d->rank > glw::modal::kMaxRank) return mfail(GLW_EINVAL, "GL rank must be 1..16");
+  if (!(d->dq > 0) || !(d->fstep > 0) || !(d->depth > 0) || !(d->delta_q > 0))
+    return mfail(GLW_EINVAL, "eta20 spacing, frequency step, depth and delta_q must be positive");
+  return GLW_OK;
+}
+
+static Layout eta20_layout(const glw_eta20_desc& d) {
+  Layout L{};
+  const size_t cs = sizeof(C), nw = 2 * (size_t)d.span + 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += align_up(bytes);
+    return at;
+  };
+  L.tw_t = take(d.t_len * cs);
+  L.tw_q = take(d.q_len * cs);
+  L.tw_n = take((size_t)d.count * cs);
+  L.src = take(2 * nw * d.q_len * cs);
+  L.coef2 = take(nw * cs);
+  L.energy2 = take(2 * nw * sizeof(double));
+  L.total = off;
+  return L;
+}
+
+int glw_eta20_workspace_size(const glw_eta20_desc* d, size_t* bytes) {
+  if (!bytes) return mfail(GLW_EINVAL, "null argument");
+  if (int rc = check_eta20(d)) return rc;
+  *bytes = eta20_layout(*d).total;
+  return GLW_OK;
+}
+
+int glw_eta20_execute(const glw_eta20_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,
+                      double* eta20, void* stream) {
+  if (int rc = check_eta20(d)) return rc;
+  if (!deposit || !workspace || !eta20) return mfail(GLW_EINVAL, "null device pointer");
+  const Layout L = eta20_layout(*d);
+  if (workspace_bytes < L.total) return mfail(GLW_EINVAL, "eta20 workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  char* ws = (char*)workspace;
+  Args a{};
+  a.q_len = d->q_len;
+  a.t_len = d->t_len;
+  a.count = d->count;
+  a.base = d->base_bin;
+  a.span = d->span;
+  a.k0 = d->k0;
+  a.kspan = d->kspan;
+  a.order = 2;
+  a.dq = d->dq;
+  a.fstep = d->fstep;
+  a.depth = d->depth;
+  a.dep = (const Dep*)deposit;
+  a.tw_t = (const C*)(ws + L.tw_t);
+  a.tw_q = (const C*)(ws + L.tw_q);
+  a.src = (C*)(ws + L.src);
+  a.coef = (C*)(ws + L.coef2);
+  a.energy = (double*)(ws + L.energy2);
+  a.rank = d->rank;
+  a.lam3 = d->delta_q;
+  for (int i = 0; i < glw::modal::kMaxRank; ++i) {
+    a.nodes[i] = i < d->rank ? d->nodes[i] : 0.0;
+    a.weights[i] = i < d->rank ? d->weights[i] : 0.0;
+  }
+  if (int rc = twiddles((C*)(ws + L.tw_t), d->t_len, st)) return rc;
+  if (int rc = twiddles((C*)(ws + L.tw_q), d->q_len, st)) return rc;
+  if (int rc = twiddles((C*)(ws + L.tw_n), d->count, st)) return rc;
+  if (int rc = launched(glw::modal::launch_t_eta20(a, st), "eta20 time pass")) return rc;
+  if (int rc = launched(glw::modal::launch_q_eta20(a, 2 * d->span + 1, st), "eta20 modal pass")) return rc;
+  hipLaunchKernelGGL(k_eta20_synth, dim3((d->count + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     (const C*)(ws + L.coef2), d->span, d->count, (const C*)(ws + L.tw_n), eta20);
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,545 @@
+// Single-direction modal time path: kernels of the one-dimensional modal GL
+// resolvent (python/src/green_laplace/modal_time.py:187-341) and of the
+// difference-frequency eta20 (python/src/green_laplace/_eta20.py:13-132).
+//
+// Grids.  The reference works on [P][W] arrays: P modal wavenumbers
+// q_k = k dq by W = order*span + 1 work frequencies nu_w = (base + w) fstep.
+// The input spectrum is a deposit: every input bin w puts its amplitude on the
+// two modal nodes k_lo(w), k_lo(w)+1 that bracket its dispersion wavenumber.
+// All parents therefore lie in the modal band [k0, k0 + K], the n-th order
+// products in [n k0, n k0 + n K], and all frequencies in [n base, n (base +
+// span)].  The kernels keep every field band-shifted: modal index m = k - n k0
+// on an x grid of Q >= order K + 1 points, frequency index w = b - n base on a
+// t grid of T >= order span + 1 points.  Products of band-shifted fields are
+// band-shifted products, and with those lengths no product wraps, so every
+// spectrum the reference reads is reproduced exactly (up to rounding) on grids
+// about half the reference's size.  eta20 multiplies a field by a conjugate,
+// which cancels the shifts: its spectra are signed, m in [-K, K] and
+// w in [-span, span].
+//
+// Layout in HBM.  x-space working arrays are [field][w][Q] (one contiguous
+// column of Q modal points per frequency), so the modal pass reads and writes
+// whole columns and the time pass touches RB consecutive x rows per
+// frequency.
+//
+// Kernels.
+//   k_modal_t<MODE>  one workgroup per RB x rows: builds each first-order
+//                    field's row from the deposit (two nodes per frequency,
+//                    exp(2 pi i m j / Q) from the Q-point table), or loads a
+//                    pair field's row, transforms it over t in LDS, folds it
+//                    into the pointwise products held in registers, and
+//                    finally inverse-transforms the products and stores the
+//                    kept frequencies.
+//   k_modal_q<MODE>  one workgroup per frequency column: forward modal FFT
+//                    of the two forcing columns, GL node response on the
+//                    valid set s > sqrt(|q| tanh |q|), workgroup sum into
+//                    the output coefficient of that frequency (and the
+//                    source-energy audit); for order 3 it also forms the
+//                    pair spectra e2, p2 and inverse-transforms the five
+//                    modal multiples the time pass needs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "glw_fft.hpp"
+
+namespace glw {
+namespace modal {
+
+using C = cpx<double>;
+constexpr int kMaxRank = 16;
+constexpr int kPairFields = 5;  // e2, e2 q, p2 q, p2 q tanh q, p2 q^2
+
+enum Mode : int { MODE_O2 = 2, MODE_O3 = 3, MODE_E20 = 20 };
+
+// Deposit of one input bin: amplitude, bracket weights, lower node index.
+struct Dep {
+  double re, im, wlo, whi, klo;
+};
+
+struct Args {
+  int q_len, t_len;      // reduced grid lengths Q, T
+  int count;             // samples of the record
+  int base, span;        // input bins base .. base + span (work index w = bin - base)
+  int k0, kspan;         // deposit band k0 .. k0 + kspan on the modal grid
+  int order;             // 2 or 3 (modal path)
+  double dq, fstep, depth;
+  const Dep* dep;        // [span + 1]
+  const C* tw_t;         // exp(-2 pi i m / T), m < T
+  const C* tw_q;         // exp(-2 pi i m / Q), m < Q
+  C* src;                // [2][2 span + 1][Q]  order-2 sources (d, k); eta20: forcing (d, k), [2][2 span + 1][Q]
+  C* pair;               // [5][2 span + 1][Q]  pair fields in x
+  C* forcing;            // [2][3 span + 1][Q]  order-3 forcing (d, k)
+  C* coef;               // [nw] output coefficient per kept frequency of the modal pass
+  double* energy;        // [nw][2] source energy (total, invalid) per kept frequency
+  int rank;
+  double lam2, lam_in, lam3;  // GL scales: eta22, inner pair, eta33 (eta20: delta q)
+  double nodes[kMaxRank], weights[kMaxRank];
+};
+
+__device__ __forceinline__ C conj_(C a) { return {a.x, -a.y}; }
+__device__ __forceinline__ C cfma(C acc, C a, C b) {  // acc + a b
+  return {acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ C cmulr(C a, double s) { return {a.x * s, a.y * s}; }
+
+// Frequency multipliers of the time-pass fields (functions of nu_w).
+enum WMul : int { W_ONE, W_NU, W_NU2, W_INU, W_I, W_MI_NU, W_MI_INU, W_MNU2, W_MONE };
+
+__device__ __forceinline__ C wmul(int wm, C v, double nu) {
+  switch (wm) {
+    case W_NU: return cmulr(v, nu);
+    case W_NU2: return cmulr(v, nu * nu);
+    case W_INU: return cmulr(v, 1.0 / fmax(nu, 2.2250738585072014e-308));
+    case W_I: return {-v.y, v.x};
+    case W_MI_NU: return {v.y * nu, -v.x * nu};
+    case W_MI_INU: {
+      const double inu = 1.0 / fmax(nu, 2.2250738585072014e-308);
+      return {v.y * inu, -v.x * inu};
+    }
+    case W_MNU2: return cmulr(v, -(nu * nu));
+    case W_MONE: return {-v.x, -v.y};
+    default: return v;
+  }
+}
+
+// One field of the time pass: a first-order field (modal power q^idx of the
+// deposit) or a pair field (x-space array idx of the modal pass), with a
+// frequency multiplier.  Pair multipliers use nu_in = (2 base + w) fstep.
+struct Recipe {
+  bool pair;
+  int idx, wm;
+};
+
+// Field sequences.  The order keeps at most five per-point values live:
+//   MODE_O2  v, vn, vn2, hx, jx, radial                 (modal_time.py:220-227)
+//   MODE_O3  eta, phizzz, phitzz, Gphizz, Gphitz, etax, phix, phixz, Getax,
+//            Gphix, phiz, phizz, Gphiz, Geta, phitz     (modal_time.py:273-310)
+//   MODE_E20 one, dno, nu, bx, qx, radial               (_eta20.py:64-86)
+template <int MODE>
+struct Steps;
+template <>
+struct Steps<MODE_O2> {
+  static constexpr int N = 6;
+};
+template <>
+struct Steps<MODE_O3> {
+  static constexpr int N = 15;
+};
+template <>
+struct Steps<MODE_E20> {
+  static constexpr int N = 6;
+};
+
+__device__ __forceinline__ Recipe recipe(int mode, int step) {
+  if (mode == MODE_O3) {
+    switch (step) {
+      case 0: return {false, 0, W_ONE};     // eta
+      case 1: return {false, 2, W_MI_NU};   // phizzz = -i q^2 nu
+      case 2: return {false, 2, W_MONE};    // phitzz = -q^2
+      case 3: return {true, 4, W_ONE};      // Gphizz = p2 q^2
+      case 4: return {true, 3, W_MI_NU};    // Gphitz = -i nu_in p2 A
+      case 5: return {false, 1, W_I};       // etax = i q
+      case 6: return {false, 1, W_INU};     // phix = q / nu
+      case 7: return {false, 1, W_NU};      // phixz = nu q
+      case 8: return {true, 1, W_I};        // Getax = i e2 q
+      case 9: return {true, 2, W_I};        // Gphix = i p2 q
+      case 10: return {false, 0, W_MI_NU};  // phiz = -i nu
+      case 11: return {false, 2, W_MI_INU}; // phizz = -i q^2 / nu
+      case 12: return {true, 3, W_ONE};     // Gphiz = p2 A
+      case 13: return {true, 0, W_ONE};     // Geta = e2
+      default: return {false, 0, W_MNU2};   // phitz = -nu^2
+    }
+  }
+  switch (step) {  // MODE_O2 and MODE_E20 share the six first-order fields
+    case 0: return {false, 0, W_ONE};
+    case 1: return {false, 0, mode == MODE_O2 ? W_NU : W_NU2};
+    case 2: return {false, 0, mode == MODE_O2 ? W_NU2 : W_NU};
+    case 3: return {false, 1, W_INU};
+    case 4: return {false, 1, W_ONE};
+    default: return {false, 2, W_INU};
+  }
+}
+
+// First-order row element (x row j, work frequency w < span + 1) with modal
+// power QP: sum over the two deposit nodes k of A_w wt_k (k dq)^QP
+// exp(2 pi i (k - k0) j / Q), the phase read from the Q-point table.
+__device__ __forceinline__ C first_value(const Args& a, int qp, int j, int w) {
+  const Dep d = a.dep[w];
+  const int klo = (int)d.klo;
+  const int i0 = (int)(((long long)(klo - a.k0) * j) % a.q_len);
+  int i1 = i0 + j;
+  if (i1 >= a.q_len) i1 -= a.q_len;
+  const C e0 = conj_(a.tw_q[i0]), e1 = conj_(a.tw_q[i1]);
+  double c0 = d.wlo, c1 = d.whi;
+  if (qp > 0) {
+    const double q0 = (double)klo * a.dq, q1 = (double)(klo + 1) * a.dq;
+    c0 *= qp == 1 ? q0 : q0 * q0;
+    c1 *= qp == 1 ? q1 : q1 * q1;
+  }
+  const C s = {c0 * e0.x + c1 * e1.x, c0 * e0.y + c1 * e1.y};
+  return {d.re * s.x - d.im * s.y, d.re * s.y + d.im * s.x};
+}
+
+// Store of the kept frequencies of an inverse t transform, scaled by 1/T:
+// w < nw (shifted spectra) or signed w in [-span, span] at column w + span.
+template <bool SIGNED>
+struct TStore {
+  static constexpr bool SEQ_FAST = true;
+  C* out;  // out + field * nw * Q + j0
+  int q_len, t_len, nw, span;
+  double scale;
+  __device__ void operator()(int r, int w, C v) const {
+    int c = w;
+    if constexpr (SIGNED) {
+      if (w <= span) c = w + span;
+      else if (w >= t_len - span) c = w - t_len + span;
+      else return;
+    } else {
+      if (w >= nw) return;
+    }
+    out[(size_t)c * q_len + r] = cmulr(v, scale);
+  }
+};
+
+// Contiguous column loader / storer of the modal pass.
+struct ColLoad {
+  static constexpr bool SEQ_FAST = false;
+  const C* col;
+  __device__ C operator()(int, int m) const { return col[m]; }
+};
+struct ColStore {
+  static constexpr bool SEQ_FAST = false;
+  C* col;
+  __device__ void operator()(int, int m, C v) const { col[m] = v; }
+};
+
+// Launch shape of the time pass at length T: RB rows per workgroup, NT
+// threads, PT points per thread (about four).
+__host__ __device__ constexpr int t_rows(int T) { return T <= 512 ? 4 : (T <= 1024 ? 2 : 1); }
+__host__ __device__ constexpr int t_threads(int T) {
+  const int n = (t_rows(T) * T / 4) / 64 * 64;
+  return n < 64 ? 64 : (n > 512 ? 512 : n);
+}
+__host__ __device__ constexpr int q_threads(int Q) { return Q <= 512 ? 64 : (Q <= 2048 ? 256 : 512); }
+
+// Time pass.  MODE_O2: sources d = 2 v vn2 + vn^2 - hx^2, k = 2 v radial +
+// 2 hx jx (modal_time.py:220-227) into a.src.  MODE_O3: forcing
+// (modal_time.py:273-310) into a.forcing.  MODE_E20: difference forcing
+// (_eta20.py:64-86, the qy = 0 plane) into a.src at signed columns.
+// Every field runs through the same staged LDS transform (one FFT instance).
+template <int T, int MODE>
+__global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
+  constexpr int RB = t_rows(T), NT = t_threads(T);
+  constexpr int SS = seq_stride<T>();
+  constexpr int PT = (RB * T + NT - 1) / NT;
+  __shared__ C buf[RB * SS];
+  const int j0 = blockIdx.x * RB;
+  const int nbin = a.span + 1;
+  const int nw2 = 2 * a.span + 1;
+  const size_t fs2 = (size_t)nw2 * a.q_len;
+  const C* tw = a.tw_t;
+  C f[PT], e[PT], x[PT], h[PT], acc_d[PT], acc_k[PT];
+#pragma unroll
+  for (int i = 0; i < PT; ++i) acc_d[i] = acc_k[i] = x[i] = h[i] = e[i] = {0.0, 0.0};
+#pragma unroll 1
+  for (int step = 0; step < Steps<MODE>::N; ++step) {
+    const Recipe rc = recipe(MODE, step);
+    __syncthreads();  // previous transform fully read
+    for (int el = threadIdx.x; el < RB * T; el += NT) {
+      const int r = el % RB, w = el / RB;  // rows fastest: pair loads coalesce over x
+      C v = {0.0, 0.0};
+      if (!rc.pair) {
+        if (w < nbin) v = wmul(rc.wm, first_value(a, rc.idx, j0 + r, w), (double)(a.base + w) * a.fstep);
+      } else if (w < nw2) {
+        v = wmul(rc.wm, a.pair[rc.idx * fs2 + (size_t)w * a.q_len + j0 + r], (double)(2 * a.base + w) * a.fstep);
+      }
+      buf[lds_at<T>(r, w)] = v;
+    }
+    __syncthreads();
+    fft_lds<double, T, RB, SS, NT, -1, 0>(buf, tw);
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+      const int p = threadIdx.x + i * NT;
+      f[i] = p < RB * T ? buf[lds_at<T>(p / T, p % T)] : C{0.0, 0.0};
+    }
+    if constexpr (MODE == MODE_O2) {
+      // e = v, h = hx
+#pragma unroll
+      for (int i = 0; i < PT; ++i) {
+        switch (step) {
+          case 0: e[i] = f[i]; break;
+          case 1: acc_d[i] = cmul(f[i], f[i]); break;
+          case 2: acc_d[i] = cfma(acc_d[i], cmulr(e[i], 2.0), f[i]); break;
+          case 3: h[i] = f[i]; acc_d[i] = csub(acc_d[i], cmul(f[i], f[i])); break;
+          case 4: acc_k[i] = cmul(cmulr(h[i], 2.0), f[i]); break;
+          default: acc_k[i] = cfma(acc_k[i], cmulr(e[i], 2.0), f[i]); break;
+        }
+      }
+    } else if constexpr (MODE == MODE_O3) {
+      // fk = (eta phixz etax - eta^2 phizzz / 2) / 4 + (phix Getax + Gphix etax - eta Gphizz - Geta phizz) / 2
+      // fd = (eta^2 phitzz / 2 + eta (phix phixz + phiz phizz)) / 4
+      //      + (eta Gphitz + Geta phitz + phix Gphix + phiz Gphiz) / 2
+      // live values: e = eta; x = etax, then phiz, then Geta; h = phix, then phizz
+#pragma unroll
+      for (int i = 0; i < PT; ++i) {
+        switch (step) {
+          case 0: e[i] = f[i]; break;
+          case 1: acc_k[i] = cmul(cmulr(cmul(e[i], e[i]), -0.125), f[i]); break;
+          case 2: acc_d[i] = cmul(cmulr(cmul(e[i], e[i]), 0.125), f[i]); break;
+          case 3: acc_k[i] = csub(acc_k[i], cmul(cmulr(e[i], 0.5), f[i])); break;
+          case 4: acc_d[i] = cfma(acc_d[i], cmulr(e[i], 0.5), f[i]); break;
+          case 5: x[i] = f[i]; break;
+          case 6: h[i] = f[i]; break;
+          case 7: {
+            const C ef = cmulr(cmul(e[i], f[i]), 0.25);
+            acc_k[i] = cfma(acc_k[i], ef, x[i]);
+            acc_d[i] = cfma(acc_d[i], ef, h[i]);
+            break;
+          }
+          case 8: acc_k[i] = cfma(acc_k[i], cmulr(h[i], 0.5), f[i]); break;
+          case 9: {
+            const C g = cmulr(f[i], 0.5);
+            acc_k[i] = cfma(acc_k[i], g, x[i]);
+            acc_d[i] = cfma(acc_d[i], g, h[i]);
+            break;
+          }
+          case 10: x[i] = f[i]; break;
+          case 11: h[i] = f[i]; acc_d[i] = cfma(acc_d[i], cmulr(cmul(e[i], x[i]), 0.25), f[i]); break;
+          case 12: acc_d[i] = cfma(acc_d[i], cmulr(x[i], 0.5), f[i]); break;
+          case 13: x[i] = f[i]; acc_k[i] = csub(acc_k[i], cmul(cmulr(f[i], 0.5), h[i])); break;
+          default: acc_d[i] = cfma(acc_d[i], cmulr(x[i], 0.5), f[i]); break;
+        }
+      }
+    } else {
+      // fd = -dno one* - one dno* + nu nu* + bx bx*
+      // fk = i (radial one* - bx qx* + qx bx* - one radial*);   e = one, h = bx
+#pragma unroll
+      for (int i = 0; i < PT; ++i) {
+        switch (step) {
+          case 0: e[i] = f[i]; break;
+          case 1: acc_d[i] = csub(cmul(cmulr(f[i], -1.0), conj_(e[i])), cmul(e[i], conj_(f[i]))); break;
+          case 2: acc_d[i] = cfma(acc_d[i], f[i], conj_(f[i])); break;
+          case 3: h[i] = f[i]; acc_d[i] = cfma(acc_d[i], f[i], conj_(f[i])); break;
+          case 4: acc_k[i] = csub(cmul(f[i], conj_(h[i])), cmul(h[i], conj_(f[i]))); break;
+          default: {
+            const C t = csub(cadd(acc_k[i], cmul(f[i], conj_(e[i]))), cmul(e[i], conj_(f[i])));
+            acc_k[i] = {-t.y, t.x};
+            break;
+          }
+        }
+      }
+    }
+  }
+  // inverse t transforms of (d, k), kept frequencies stored at 1/T
+  C* out = MODE == MODE_O3 ? a.forcing : a.src;
+  const int nw = MODE == MODE_O3 ? 3 * a.span + 1 : nw2;
+  NoIO none;
+#pragma unroll 1
+  for (int o = 0; o < 2; ++o) {
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PT; ++i) {
+      const int p = threadIdx.x + i * NT;
+      if (p < RB * T) buf[lds_at<T>(p / T, p % T)] = o == 0 ? acc_d[i] : acc_k[i];
+    }
+    __syncthreads();
+    TStore<MODE == MODE_E20> st{out + (size_t)o * nw * a.q_len + j0, a.q_len, T, nw, a.span, 1.0 / T};
+    fft_io<double, T, RB, SS, NT, +1, 0>(buf, tw, none, st);
+  }
+}
+
+// GL node sums on the valid set s > av (modal_time.py:383-399):
+// S = sum (E+ - E-) / (2 av), C = sum (E+ + E-) / 2 with
+// E+- = (w / lam) exp(x - (s -+ av) x / lam); at av == 0 the limits
+// S = sum (w / lam) exp(x - s tau) tau, C = sum (w / lam) exp(x - s tau).
+__device__ __forceinline__ void node_sums(const Args& a, double lam, double s, double av, double& S, double& Cs) {
+  S = 0.0;
+  Cs = 0.0;
+  for (int n = 0; n < a.rank; ++n) {
+    const double x = a.nodes[n];
+    const double tau = x / lam;
+    const double wl = a.weights[n] / lam;
+    if (av > 0.0) {
+      const double plus = wl * exp(x - (s - av) * tau);
+      const double minus = wl * exp(x - (s + av) * tau);
+      S += (plus - minus) / (2.0 * av);
+      Cs += (plus + minus) / 2.0;
+    } else {
+      const double b = wl * exp(x - s * tau);
+      S += b * tau;
+      Cs += b;
+    }
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void block_sum4(double (&v)[4], double* red) {
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double x = v[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    v[k] = x;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) red[wv * 4 + k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < 4; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < NT / 64; ++i) s += red[i * 4 + k];
+      v[k] = s;
+    }
+  }
+}
+
+// Modal pass over frequency column w = blockIdx.x.
+//   MODE_O2: eta22 coefficient of bin 2 base + w from the sources, and (order
+//            3) the pair spectra e2, p2 (modal_time.py:240-271) transformed
+//            to x as e2, e2 q, p2 q, p2 A, p2 q^2.
+//   MODE_O3: eta33 coefficient of bin 3 base + w (modal_time.py:316-327).
+//   MODE_E20: eta20 coefficient of signed bin w - span (_eta20.py:103-123).
+template <int Q, int MODE>
+__global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
+  constexpr int NT = q_threads(Q);
+  constexpr int SS = seq_stride<Q>();
+  constexpr int PQ = (Q + NT - 1) / NT;
+  __shared__ C buf[SS];
+  __shared__ double red[NT / 64 * 4];
+  const int w = blockIdx.x;
+  const int nw = MODE == MODE_O3 ? 3 * a.span + 1 : 2 * a.span + 1;
+  const C* in = MODE == MODE_O3 ? a.forcing : a.src;
+  const double inv_q = 1.0 / Q;
+  NoIO none;
+  // forward modal FFT of the two forcing columns: d kept in registers, k left in LDS
+  C fd[PQ];
+#pragma unroll 1
+  for (int o = 0; o < 2; ++o) {
+    __syncthreads();
+    ColLoad ld{in + ((size_t)o * nw + w) * Q};
+    fft_io<double, Q, 1, SS, NT, -1, Q >= 3072 ? 1 : 0>(buf, a.tw_q, ld, none);
+    if (o == 0) {
+#pragma unroll
+      for (int i = 0; i < PQ; ++i) {
+        const int m = threadIdx.x + i * NT;
+        fd[i] = m < Q ? cmulr(buf[lds_pad(m)], inv_q) : C{0.0, 0.0};
+      }
+    }
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};  // coefficient re, im; energy total, invalid
+  const int n_ord = MODE == MODE_O3 ? 3 : 2;
+  const int mmax = n_ord * a.kspan;  // kept modal band 0 .. n K (shifted) or -K .. K (signed)
+  // modal_time.py:241-252 (eta22), 316-327 (eta33); _eta20.py:105-123 (eta20)
+  const double s = MODE == MODE_E20 ? (double)(w - a.span) * a.fstep : (double)(n_ord * a.base + w) * a.fstep;
+  const double lam = MODE == MODE_O2 ? a.lam2 : a.lam3;
+  const bool pairs = MODE == MODE_O2 && a.order == 3;
+#pragma unroll
+  for (int i = 0; i < PQ; ++i) {
+    const int m = threadIdx.x + i * NT;
+    if (m >= Q) continue;
+    const C sd = fd[i], sk = cmulr(buf[lds_pad(m)], inv_q);
+    if (pairs) {  // fd[i] becomes e2, the LDS element p2 (zero off the valid band)
+      fd[i] = C{0.0, 0.0};
+      buf[lds_pad(m)] = C{0.0, 0.0};
+    }
+    int k;
+    if constexpr (MODE == MODE_E20) {
+      if (m <= a.kspan) k = m;
+      else if (m >= Q - a.kspan) k = m - Q;
+      else continue;
+    } else {
+      if (m > mmax) continue;
+      k = n_ord * a.k0 + m;
+    }
+    const double q = fabs((double)k * a.dq);
+    const double aq = q * tanh(q);
+    const double av = sqrt(aq);
+    if constexpr (MODE != MODE_E20) {
+      const double energy = sd.x * sd.x + sd.y * sd.y + sk.x * sk.x + sk.y * sk.y;
+      acc[2] += energy;
+      if (!(s > av)) {
+        acc[3] += energy;
+        continue;
+      }
+      double S, Cs;
+      node_sums(a, lam, s, av, S, Cs);
+      if constexpr (MODE == MODE_O2) {
+        // (-A S sd + C sk) / (4 h)
+        acc[0] += (-aq * S * sd.x + Cs * sk.x) / (4.0 * a.depth);
+        acc[1] += (-aq * S * sd.y + Cs * sk.y) / (4.0 * a.depth);
+        if (pairs) {
+          double Si, Ci;
+          node_sums(a, a.lam_in, s, av, Si, Ci);
+          // e2 = (-A S sd + C sk) / 4, p2 = i (C sd - S sk) / 4
+          fd[i] = {(-aq * Si * sd.x + Ci * sk.x) / 4.0, (-aq * Si * sd.y + Ci * sk.y) / 4.0};
+          const C t = {Ci * sd.x - Si * sk.x, Ci * sd.y - Si * sk.y};
+          buf[lds_pad(m)] = {-t.y / 4.0, t.x / 4.0};
+        }
+      } else {
+        // (A S fd - i C fk) / h^2
+        const double h2 = a.depth * a.depth;
+        acc[0] += (aq * S * sd.x + Cs * sk.y) / h2;
+        acc[1] += (aq * S * sd.y - Cs * sk.x) / h2;
+      }
+    } else {
+      if (!(q > 0.0 && av > fabs(s))) continue;
+      double md = 0.0, mki = 0.0;  // mk = i mki
+      for (int n = 0; n < a.rank; ++n) {
+        const double x = a.nodes[n];
+        const double tau = x / lam;
+        const double wl = a.weights[n] / lam;
+        const double plus = wl * exp(x - (av - s) * tau);
+        const double minus = wl * exp(x - (av + s) * tau);
+        md -= av * (plus + minus) / 4.0;
+        mki += (plus - minus) / 4.0;
+      }
+      // (md fd + i mki fk) / (2 h)
+      acc[0] += (md * sd.x - mki * sk.y) / (2.0 * a.depth);
+      acc[1] += (md * sd.y + mki * sk.x) / (2.0 * a.depth);
+    }
+  }
+  block_sum4<NT>(acc, red);
+  if (threadIdx.x == 0) {
+    a.coef[w] = {acc[0], acc[1]};
+    a.energy[2 * w] = acc[2];
+    a.energy[2 * w + 1] = acc[3];
+  }
+  if constexpr (MODE == MODE_O2) {
+    if (!pairs) return;
+    // pair fields in x (modal_time.py:285-292, _spectral_field = fft_t(ifft_q(.) P)):
+    // e2, e2 q, p2 q, p2 A, p2 q^2, each inverse-transformed over the modal axis
+    const size_t fs = (size_t)nw * Q;
+    C p2[PQ];
+#pragma unroll
+    for (int i = 0; i < PQ; ++i) {
+      const int m = threadIdx.x + i * NT;
+      p2[i] = m < Q ? buf[lds_pad(m)] : C{0.0, 0.0};  // own elements (block_sum4 synchronised)
+    }
+#pragma unroll 1
+    for (int f = 0; f < kPairFields; ++f) {
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < PQ; ++i) {
+        const int m = threadIdx.x + i * NT;
+        if (m >= Q) continue;
+        const double q = (double)(2 * a.k0 + m) * a.dq;
+        double mul = 1.0;
+        if (f == 1 || f == 2) mul = q;
+        else if (f == 3) mul = q * tanh(q);
+        else if (f == 4) mul = q * q;
+        buf[lds_pad(m)] = cmulr(f == 0 || f == 1 ? fd[i] : p2[i], mul);
+      }
+      __syncthreads();
+      ColStore st{a.pair + f * fs + (size_t)w * Q};
+      fft_io<double, Q, 1, SS, NT, +1, Q >= 3072 ? 1 : 0>(buf, a.tw_q, none, st);
+    }
+  }
+}
+
+}  // namespace modal
+}  // namespace glw

@@ -1,0 +1,70 @@
+// Launchers of the modal-path kernels, one switch over the supported grid
+// lengths per kernel family.  GLW_MODAL_SET picks the families built by this
+// translation unit (the units are compiled in parallel):
+//   0: time pass, order-2 sources and eta20 forcing
+//   1: time pass, order-3 forcing
+//   2: modal pass, eta22 (+ pair fields)
+//   3: modal pass, eta33 and eta20
+#include "glw_modal.hpp"
+#include "glw_modal_launch.hpp"
+
+namespace glw {
+namespace modal {
+
+namespace {
+
+template <int T, int MODE>
+int t_go(const Args& a, hipStream_t st) {
+  if (a.q_len % t_rows(T)) return 1;
+  hipLaunchKernelGGL((k_modal_t<T, MODE>), dim3(a.q_len / t_rows(T)), dim3(t_threads(T)), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+template <int Q, int MODE>
+int q_go(const Args& a, int ncol, hipStream_t st) {
+  hipLaunchKernelGGL((k_modal_q<Q, MODE>), dim3(ncol), dim3(q_threads(Q)), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+}  // namespace
+
+#define GLW_T_CASE(L) \
+  case L:             \
+    return t_go<L, MODE>(a, st);
+#define GLW_Q_CASE(L) \
+  case L:             \
+    return q_go<L, MODE>(a, ncol, st);
+
+#if GLW_MODAL_SET == 0 || GLW_MODAL_SET == 1
+template <int MODE>
+static int t_switch(const Args& a, hipStream_t st) {
+  switch (a.t_len) {
+    GLW_MODAL_T_LENGTHS(GLW_T_CASE)
+    default: return 1;
+  }
+}
+#endif
+#if GLW_MODAL_SET >= 2
+template <int MODE>
+static int q_switch(const Args& a, int ncol, hipStream_t st) {
+  switch (a.q_len) {
+    GLW_MODAL_Q_LENGTHS(GLW_Q_CASE)
+    default: return 1;
+  }
+}
+#endif
+
+#if GLW_MODAL_SET == 0
+int launch_t_order2(const Args& a, hipStream_t st) { return t_switch<MODE_O2>(a, st); }
+int launch_t_eta20(const Args& a, hipStream_t st) { return t_switch<MODE_E20>(a, st); }
+#elif GLW_MODAL_SET == 1
+int launch_t_order3(const Args& a, hipStream_t st) { return t_switch<MODE_O3>(a, st); }
+#elif GLW_MODAL_SET == 2
+int launch_q_order2(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_O2>(a, ncol, st); }
+#elif GLW_MODAL_SET == 3
+int launch_q_order3(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_O3>(a, ncol, st); }
+int launch_q_eta20(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_E20>(a, ncol, st); }
+#endif
+
+}  // namespace modal
+}  // namespace glw

@@ -1,0 +1,27 @@
+// Launchers of the modal-path kernels (glw_modal_launch.hip).  Each returns
+// 0, 1 for an unsupported length or launch shape, 2 for a launch error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "glw_modal.hpp"
+
+// Time-grid and modal-grid lengths with kernels (2^a 3^b, multiples of 16).
+#define GLW_MODAL_T_LENGTHS(X) \
+  X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048)
+#define GLW_MODAL_Q_LENGTHS(X)                                                                          \
+  X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) \
+  X(3072) X(4096) X(6144) X(8192)
+
+namespace glw {
+namespace modal {
+
+int launch_t_order2(const Args& a, hipStream_t st);
+int launch_t_order3(const Args& a, hipStream_t st);
+int launch_t_eta20(const Args& a, hipStream_t st);
+int launch_q_order2(const Args& a, int ncol, hipStream_t st);
+int launch_q_order3(const Args& a, int ncol, hipStream_t st);
+int launch_q_eta20(const Args& a, int ncol, hipStream_t st);
+
+}  // namespace modal
+}  // namespace glw

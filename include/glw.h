@@ -255,6 +255,66 @@ int glw_plan_buffer(glw_plan* plan, const char* name, void** ptr, size_t* bytes)
 /* Largest FFT length the kernels are instantiated for, and whether a length is one. */
 int glw_fft_length_supported(int32_t n);
 
+/* ---- Single-direction modal time path -------------------------------------
+ *
+ * glw_modal_execute replaces _modal_sum_components
+ * (python/src/green_laplace/modal_time.py:187-341): eta22 and eta33 of one
+ * level of the modal-points ladder, from a deposit of the first-order band
+ * on the modal grid q_k = k dq.  The caller (host) selects the band, solves
+ * the dispersion relation, brackets each q on the reference's P-point modal
+ * axis (modal_time.py:344-368) and picks the reduced grids:
+ *   q_len >= order * kspan + 1  (x points of the band-shifted modal axis),
+ *   t_len >= order * span + 1   (t points of the band-shifted frequency axis),
+ * lengths that glw_modal_length_supported accepts (bit 1: t_len, bit 2: q_len).
+ * Input bins are base_bin ..
+ * base_bin + span (work index w); deposit[w] = {Re A, Im A, w_lo, w_hi, k_lo}
+ * (device, FP64, k_hi = k_lo + 1, k0 = min k_lo, kspan = max k_lo + 1 - k0).
+ * Outputs (device): eta11 (double[count], may be NULL) = Re fft of the
+ * positive-bin amplitudes, eta22 and eta33 (complex128[count], the analytic
+ * components; eta33 only for order 3, may be NULL otherwise), audit
+ * (double[4], may be NULL) = {eta22 invalid, eta22 total, eta33 invalid,
+ * eta33 total} source energy.  The workspace (device, `workspace_bytes` >=
+ * glw_modal_workspace_size) is used as scratch; nothing is allocated. */
+typedef struct glw_modal_desc {
+  int32_t order;           /* 2 or 3 */
+  int32_t count;           /* samples N of the record */
+  int32_t base_bin, span;  /* input bins base_bin .. base_bin + span */
+  int32_t q_len, t_len;    /* reduced grid lengths */
+  int32_t k0, kspan;       /* deposit band k0 .. k0 + kspan */
+  int32_t rank;            /* Gauss--Laguerre rank (<= 16) */
+  double dq;               /* modal spacing (dimensionless wavenumber) */
+  double fstep;            /* 2 pi / (N dt) sqrt(h / g) */
+  double depth;
+  double lambda2;          /* eta22 scale sqrt(4 qp tanh qp - 2 qp tanh 2qp) */
+  double lambda_inner;     /* pair scale 2 nu_p - sqrt(2 qp tanh 2qp) */
+  double lambda3;          /* eta33 scale 3 nu_p - sqrt(3 qp tanh 3qp) */
+  double nodes[16], weights[16];
+} glw_modal_desc;
+
+int glw_modal_length_supported(int32_t n);
+int glw_modal_workspace_size(const glw_modal_desc* desc, size_t* bytes);
+int glw_modal_execute(const glw_modal_desc* desc, const double* deposit, void* workspace, size_t workspace_bytes,
+                      double* eta11, double* eta22, double* eta33, double* audit, void* stream);
+
+/* glw_eta20_execute replaces unidirectional_eta20
+ * (python/src/green_laplace/_eta20.py:13-132): the difference-frequency
+ * elevation on the reference's nx x ny modal grid, evaluated on its qy = 0
+ * plane (the unidirectional input deposits nothing else).  The deposit
+ * brackets each q on the eta20 qx axis (nx points, _eta20.py:44,156-172);
+ * q_len >= 2 kspan + 1, t_len >= 2 span + 1.  Output eta20 (device,
+ * double[count]) after the Hermitian projection (_eta20.py:175-185).
+ * delta_q is the GL scale (_eta20.py:90-93); the caller returns zeros for
+ * the degenerate case. */
+typedef struct glw_eta20_desc {
+  int32_t count, base_bin, span, q_len, t_len, k0, kspan, rank;
+  double dq, fstep, depth, delta_q;
+  double nodes[16], weights[16];
+} glw_eta20_desc;
+
+int glw_eta20_workspace_size(const glw_eta20_desc* desc, size_t* bytes);
+int glw_eta20_execute(const glw_eta20_desc* desc, const double* deposit, void* workspace, size_t workspace_bytes,
+                      double* eta20, void* stream);
+
 const char* glw_last_error(void);
 const char* glw_version(void);
 

@@ -1,0 +1,108 @@
+"""GPU single-direction modal time path vs the reference's time records and the oracle.
+
+Tolerances: the reference's own for its two MATLAB records (eta22/eta33 3e-13,
+eta20 8e-11, python/tests/test_temporal.py) and for the on-grid identity with
+the spatial graph (rtol 3e-14, python/tests/test_modal_time.py:57-63); 1e-12
+relative against the FP64 oracle on the same level (the device grids are the
+band-shifted halves of the reference's, so the rounding differs: measured
+~1e-14); degree-2/3 homogeneity at 1e-13 on the largest modal grid.
+"""
+
+import numpy as np
+import pytest
+
+from cases import rel
+from modal_cases import (broadband_record, eta20_fixture_kwargs, off_grid_case, on_grid_case, order3_fixture_kwargs,
+                         unidirectional)
+from oracle import gl_oracle as O
+from oracle import modal_oracle as M
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def MD():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from green_laplace_wave_reconstruction_amd import modal
+
+    return modal
+
+
+def test_fixture_order3(MD):
+    d = unidirectional("modal_order3")
+    kw = order3_fixture_kwargs(d)
+    r = MD.reconstruct_unidirectional_modal_timeseries(d["eta1"], d["time"], **kw)
+    print("fixture order3", rel(r.components["eta22"], d["eta22"]), rel(r.components["eta33"], d["eta33"]), flush=True)
+    assert rel(r.components["eta22"], d["eta22"]) < 3e-13
+    assert rel(r.components["eta33"], d["eta33"]) < 3e-13
+    o = M.reconstruct_unidirectional_modal_timeseries(d["eta1"], d["time"], **kw)
+    for name in ("eta11", "eta22_analytic", "eta33_analytic"):
+        assert rel(r.components[name], o.components[name]) < 1e-12, name
+    assert r.audit["eta33_invalid_source_energy_fraction"] == pytest.approx(
+        o.audit["eta33_invalid_source_energy_fraction"], rel=1e-6, abs=1e-20)
+    assert not r.audit["auxiliary_space_domain"]
+
+
+def test_fixture_eta20(MD):
+    d = unidirectional("time_eta20")
+    kw = eta20_fixture_kwargs(d)
+    r = MD.reconstruct_unidirectional_timeseries(d["eta1"], d["time"], **kw)
+    o = M.reconstruct_unidirectional_modal_timeseries(d["eta1"], d["time"], **kw)
+    print("fixture eta20", rel(r.components["eta20"], d["eta20"]), rel(r.components["eta20"], o.components["eta20"]),
+          flush=True)
+    assert rel(r.components["eta20"], d["eta20"]) < 8e-11
+    assert rel(r.components["eta20"], o.components["eta20"]) < 1e-12
+    assert rel(r.components["eta22"], o.components["eta22"]) < 1e-12
+    assert rel(r.eta, o.eta) < 1e-12
+
+
+def test_on_grid_matches_spatial(MD):
+    eta1_t, t, tk, eta1_x, sk = on_grid_case()
+    spatial = O.reconstruct_directional_field(eta1_x, **sk)["components"]
+    modal = MD.reconstruct_unidirectional_modal_timeseries(eta1_t, t, **tk)
+    for name in ("eta11", "eta22", "eta33"):
+        assert np.isclose(modal.components[name][0], spatial[name][0, 0], rtol=3e-14, atol=1e-16), name
+    assert modal.audit["wavevector_projection_rms_relative"] < 1e-14
+    assert modal.audit["converged"] and modal.audit["modal_points"] == 256
+
+
+def test_off_grid_ladder(MD):
+    eta1, t, kw = off_grid_case()
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta1, t, **kw)
+    o = M.reconstruct_unidirectional_modal_timeseries(eta1, t, **kw)
+    assert r.audit["converged"] and r.audit["modal_points"] == 1024
+    proj = [lv["wavevector_projection_rms_relative"] for lv in r.audit["levels"]]
+    assert all(b < a for a, b in zip(proj, proj[1:]))
+    for lr, lo in zip(r.audit["levels"][1:], o.audit["levels"][1:]):
+        assert lr["eta22_change"] == pytest.approx(lo["eta22_change"], rel=1e-9)
+        assert lr["eta33_change"] == pytest.approx(lo["eta33_change"], rel=1e-9)
+    assert rel(r.components["eta33_analytic"], o.components["eta33_analytic"]) < 1e-12
+
+
+@pytest.mark.parametrize("points", [2048, 4096])
+def test_broadband_level_vs_oracle(MD, points):
+    eta, t, h = broadband_record()
+    kw = dict(depth=h, quadrature_rank=8, order=3, modal_points=points, include_eta20=True)
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    o = M.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    e22, e33 = rel(r.components["eta22_analytic"], o.components["eta22_analytic"]), \
+        rel(r.components["eta33_analytic"], o.components["eta33_analytic"])
+    print("broadband", points, r.audit["reduced_grid"], e22, e33, rel(r.components["eta20"], o.components["eta20"]),
+          flush=True)
+    assert e22 < 1e-12 and e33 < 1e-12
+    assert rel(r.components["eta20"], o.components["eta20"]) < 1e-12
+
+
+def test_homogeneity_largest_grid(MD):
+    """Degree 2 and 3 in the record at 16384 modal points (modal length 8192)."""
+    eta, t, h = broadband_record()
+    kw = dict(depth=h, quadrature_rank=8, order=3, modal_points=16384, include_eta20=False)
+    r1 = MD.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    r2 = MD.reconstruct_unidirectional_modal_timeseries(1.7 * eta, t, **kw)
+    print("homogeneity", r1.audit["reduced_grid"], flush=True)
+    assert r1.audit["reduced_grid"]["q_len"] == 8192
+    assert rel(r2.components["eta22_analytic"], 1.7**2 * r1.components["eta22_analytic"]) < 1e-13
+    assert rel(r2.components["eta33_analytic"], 1.7**3 * r1.components["eta33_analytic"]) < 1e-13

@@ -1,0 +1,79 @@
+"""Host side of the GPU modal time path (CPU): band, deposit and reduced-grid planning.
+
+The level plan must reproduce the oracle's band and modal bracket exactly, and
+its reduced grids must hold every product without wrap (q_len >= order K + 1,
+t_len >= order span + 1), which the kernels rely on.
+"""
+
+import numpy as np
+import pytest
+
+from green_laplace_wave_reconstruction_amd import _native
+from green_laplace_wave_reconstruction_amd import modal as MD
+from modal_cases import broadband_record, off_grid_case, order3_fixture_kwargs, unidirectional
+from oracle import modal_oracle as M
+
+
+def _lib_or_skip():
+    if not _native.LIB_PATH.exists():
+        pytest.skip("libglw.so not built")
+
+
+def test_band_and_deposit_match_oracle():
+    _lib_or_skip()
+    d = unidirectional("modal_order3")
+    kw = order3_fixture_kwargs(d)
+    band = MD._select_band(d["eta1"], d["time"], depth=kw["depth"], gravity=kw["gravity"], order=3,
+                           omega_max=kw["omega_max"], energy_fraction=1.0)
+    ob = M.select_band(d["eta1"], d["time"], depth=kw["depth"], gravity=kw["gravity"], order=3,
+                       omega_max=kw["omega_max"], energy_fraction=1.0)
+    assert np.array_equal(band.bins, ob.bins)
+    assert np.allclose(band.q, ob.q, rtol=1e-15, atol=0)
+    lp = MD.modal_level_plan(band, depth=kw["depth"], gravity=kw["gravity"], rank=kw["quadrature_rank"], order=3,
+                             points=kw["modal_points"], modal_limit=kw["modal_limit"])
+    axis, dq = M.modal_axis(kw["modal_points"], float(np.max(ob.q)), 3, kw["modal_limit"])
+    joint, proj = M.deposit_1d(ob.amplitudes, ob.bins - ob.bins.min(), ob.q, axis, dq, 3 * (ob.bins.size - 1) + 1)
+    assert lp.projection == proj and lp.desc.dq == dq
+    # rebuild the joint spectrum from the device deposit table
+    P = kw["modal_points"]
+    re = np.zeros_like(joint)
+    for w, (ar, ai, wl, wu, kl) in enumerate(lp.deposit):
+        re[int(kl) + P // 2, w] += (ar + 1j * ai) * wl
+        re[int(kl) + 1 + P // 2, w] += (ar + 1j * ai) * wu
+    assert np.array_equal(re, joint)
+
+
+@pytest.mark.parametrize("points", [256, 1024, 4096, 16384])
+def test_reduced_grids_hold_every_product(points):
+    _lib_or_skip()
+    eta, t, h = broadband_record()
+    band = MD._select_band(eta, t, depth=h, gravity=9.81, order=3, omega_max=None, energy_fraction=0.999)
+    lp = MD.modal_level_plan(band, depth=h, gravity=9.81, rank=8, order=3, points=points, modal_limit=None)
+    d = lp.desc
+    assert d.q_len >= 3 * d.kspan + 1 and d.t_len >= 3 * d.span + 1
+    assert d.q_len <= points // 2 + 16  # the band-shifted axis is at most about half the reference's
+    assert d.k0 >= 0 and 3 * (d.base_bin + d.span) < d.count
+    assert _native.load_library().glw_modal_length_supported(d.q_len) & 2
+    assert _native.load_library().glw_modal_length_supported(d.t_len) & 1
+
+
+def test_eta20_plan_and_validation():
+    _lib_or_skip()
+    d = unidirectional("time_eta20")
+    t = d["time"]
+    band = MD._select_band(d["eta1"], t, depth=1.0, gravity=9.81, order=2,
+                           omega_max=2 * np.pi * 8 / (t.size * (t[1] - t[0])) + 1e-12, energy_fraction=1.0)
+    desc, dep, audit = MD.eta20_plan(band.amplitudes, band.omega, band.q, band.count, band.dt, depth=1.0,
+                                     gravity=9.81, rank=16, nx=128, ny=16)
+    assert audit["delta_q_rms_pair"] == pytest.approx(float(d["delta_q_rms_pair"]), rel=1e-13)
+    assert desc.q_len >= 2 * desc.kspan + 1 and desc.t_len >= 2 * desc.span + 1
+    with pytest.raises(ValueError, match="rank"):
+        MD.eta20_plan(band.amplitudes, band.omega, band.q, band.count, band.dt, depth=1.0, gravity=9.81, rank=8,
+                      nx=128, ny=16)
+    with pytest.raises(TypeError, match="unknown time-reconstruction options"):
+        MD.reconstruct_unidirectional_timeseries(d["eta1"], t, depth=1.0, bogus=1)
+    eta1, tt, kw = off_grid_case()
+    with pytest.raises(ValueError, match="increasing sequence"):
+        MD.reconstruct_unidirectional_modal_timeseries(eta1, tt, **dict(kw, modal_points=(512, 256)))
+    with pytest.raises(ValueError, match="rank 4, 6, 8, or 10"):
+        MD.reconstruct_unidirectional_modal_timeseries(eta1, tt, **dict(kw, quadrature_rank=5))
