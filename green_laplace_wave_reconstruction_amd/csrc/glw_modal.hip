@@ -66,50 +66,74 @@ struct SynthArgs {
   double* audit;
 };
 
-__device__ __forceinline__ C bin_sum(const C* coef, int nw, int b0, int n, int count, const C* tw) {
+// Lane part of sum_w coef_w exp(-2 pi i (b0 + w) n / N) over w = lane, lane + 64, ...
+// (phase index advanced by 64 n mod N per step).
+__device__ __forceinline__ C lane_sum(const C* coef, int nw, int b0, int n, int count, const C* tw, int lane) {
   double re = 0.0, im = 0.0;
-  long long idx = ((long long)b0 * n) % count;
-  for (int w = 0; w < nw; ++w) {
+  if (lane >= nw) return {0.0, 0.0};
+  long long idx = ((long long)(b0 + lane) * n) % count;
+  const long long step = (64LL * n) % count;
+  for (int w = lane; w < nw; w += 64) {
     const C e = tw[idx];
     const C c = coef[w];
     re += c.x * e.x - c.y * e.y;
     im += c.x * e.y + c.y * e.x;
-    idx += n;
+    idx += step;
     if (idx >= count) idx -= count;
   }
   return {re, im};
 }
 
-__global__ void k_modal_synth(SynthArgs s) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && s.audit) {
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// One wave per output sample n (4 per workgroup); wave 0 also sums the audit energies.
+__global__ __launch_bounds__(256) void k_modal_synth(SynthArgs s) {
+  const int n = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (n == 0 && s.audit) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int w = 0; w < 2 * s.span + 1; ++w) {
+    for (int w = lane; w < 2 * s.span + 1; w += 64) {
       v[0] += s.energy2[2 * w + 1];
       v[1] += s.energy2[2 * w];
     }
     if (s.order == 3) {
-      for (int w = 0; w < 3 * s.span + 1; ++w) {
+      for (int w = lane; w < 3 * s.span + 1; w += 64) {
         v[2] += s.energy3[2 * w + 1];
         v[3] += s.energy3[2 * w];
       }
     }
-    for (int k = 0; k < 4; ++k) s.audit[k] = v[k];
+    for (int k = 0; k < 4; ++k) v[k] = wave_sum(v[k]);
+    if (lane == 0)
+      for (int k = 0; k < 4; ++k) s.audit[k] = v[k];
   }
   if (n >= s.count) return;
-  if (s.eta11) {
+  if (s.eta11) {  // Re sum_w A_w exp(-2 pi i (base + w) n / N)
     double re = 0.0;
-    long long idx = ((long long)s.base * n) % s.count;
-    for (int w = 0; w <= s.span; ++w) {
-      const C e = s.tw_n[idx];
-      re += s.dep[w].re * e.x - s.dep[w].im * e.y;
-      idx += n;
-      if (idx >= s.count) idx -= s.count;
+    if (lane <= s.span) {
+      long long idx = ((long long)(s.base + lane) * n) % s.count;
+      const long long step = (64LL * n) % s.count;
+      for (int w = lane; w <= s.span; w += 64) {
+        const C e = s.tw_n[idx];
+        re += s.dep[w].re * e.x - s.dep[w].im * e.y;
+        idx += step;
+        if (idx >= s.count) idx -= s.count;
+      }
     }
-    s.eta11[n] = re;
+    re = wave_sum(re);
+    if (lane == 0) s.eta11[n] = re;
   }
-  s.eta22[n] = bin_sum(s.coef2, 2 * s.span + 1, 2 * s.base, n, s.count, s.tw_n);
-  if (s.order == 3 && s.eta33) s.eta33[n] = bin_sum(s.coef3, 3 * s.span + 1, 3 * s.base, n, s.count, s.tw_n);
+  C v = lane_sum(s.coef2, 2 * s.span + 1, 2 * s.base, n, s.count, s.tw_n, lane);
+  v = {wave_sum(v.x), wave_sum(v.y)};
+  if (lane == 0) s.eta22[n] = v;
+  if (s.order == 3 && s.eta33) {
+    C u = lane_sum(s.coef3, 3 * s.span + 1, 3 * s.base, n, s.count, s.tw_n, lane);
+    u = {wave_sum(u.x), wave_sum(u.y)};
+    if (lane == 0) s.eta33[n] = u;
+  }
 }
 
 // eta20 record: Hermitian projection of the signed-bin coefficients
@@ -150,9 +174,9 @@ Layout modal_layout(const glw_modal_desc& d) {
   L.tw_t = take(d.t_len * cs);
   L.tw_q = take(d.q_len * cs);
   L.tw_n = take((size_t)d.count * cs);
-  L.src = take(2 * nw2 * d.q_len * cs);
+  L.src = take(2 * (size_t)glw::modal::out_ld((int)nw2) * d.q_len * cs);
   L.pair = d.order == 3 ? take(glw::modal::kPairFields * nw2 * d.q_len * cs) : 0;
-  L.forcing = d.order == 3 ? take(2 * nw3 * d.q_len * cs) : 0;
+  L.forcing = d.order == 3 ? take(2 * (size_t)glw::modal::out_ld((int)nw3) * d.q_len * cs) : 0;
   L.coef2 = take(nw2 * cs);
   L.coef3 = d.order == 3 ? take(nw3 * cs) : 0;
   L.energy2 = take(2 * nw2 * sizeof(double));
@@ -204,13 +228,22 @@ int glw_modal_workspace_size(const glw_modal_desc* d, size_t* bytes) {
   return GLW_OK;
 }
 
-int glw_modal_execute(const glw_modal_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,
-                      double* eta11, double* eta22, double* eta33, double* audit, void* stream) {
+}  // extern "C"
+
+// Stages bracketed by `ev` (when given): twiddles, order-2 time pass, eta22
+// modal pass, order-3 time pass, eta33 modal pass, output synthesis.
+constexpr int kModalStages = 6;
+
+static int modal_run(const glw_modal_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,
+                     double* eta11, double* eta22, double* eta33, double* audit, hipStream_t st, hipEvent_t* ev) {
   if (int rc = check_modal(d)) return rc;
   if (!deposit || !workspace || !eta22 || (d->order == 3 && !eta33)) return mfail(GLW_EINVAL, "null device pointer");
   const Layout L = modal_layout(*d);
   if (workspace_bytes < L.total) return mfail(GLW_EINVAL, "modal workspace too small");
-  hipStream_t st = (hipStream_t)stream;
+  auto mark = [&](int i) -> int {
+    if (ev) GLW_HIP_CHECK(hipEventRecord(ev[i], st));
+    return GLW_OK;
+  };
   char* ws = (char*)workspace;
   Args a{};
   a.q_len = d->q_len;
@@ -238,21 +271,29 @@ int glw_modal_execute(const glw_modal_desc* d, const double* deposit, void* work
     a.nodes[i] = i < d->rank ? d->nodes[i] : 0.0;
     a.weights[i] = i < d->rank ? d->weights[i] : 0.0;
   }
+  if (int rc = mark(0)) return rc;
   if (int rc = twiddles((C*)(ws + L.tw_t), d->t_len, st)) return rc;
   if (int rc = twiddles((C*)(ws + L.tw_q), d->q_len, st)) return rc;
   if (int rc = twiddles((C*)(ws + L.tw_n), d->count, st)) return rc;
+  if (int rc = mark(1)) return rc;
   const int nw2 = 2 * d->span + 1, nw3 = 3 * d->span + 1;
   // order-2 sources, eta22 coefficients (+ pair fields)
   if (int rc = launched(glw::modal::launch_t_order2(a, st), "modal time pass (order 2)")) return rc;
+  if (int rc = mark(2)) return rc;
   a.coef = (C*)(ws + L.coef2);
   a.energy = (double*)(ws + L.energy2);
   if (int rc = launched(glw::modal::launch_q_order2(a, nw2, st), "modal pass (eta22)")) return rc;
+  if (int rc = mark(3)) return rc;
   if (d->order == 3) {
     if (int rc = launched(glw::modal::launch_t_order3(a, st), "modal time pass (order 3)")) return rc;
+    if (int rc = mark(4)) return rc;
     a.coef = (C*)(ws + L.coef3);
     a.energy = (double*)(ws + L.energy3);
     if (int rc = launched(glw::modal::launch_q_order3(a, nw3, st), "modal pass (eta33)")) return rc;
+  } else {
+    if (int rc = mark(4)) return rc;
   }
+  if (int rc = mark(5)) return rc;
   SynthArgs s{};
   s.count = d->count;
   s.base = d->base_bin;
@@ -268,9 +309,34 @@ int glw_modal_execute(const glw_modal_desc* d, const double* deposit, void* work
   s.eta22 = (C*)eta22;
   s.eta33 = (C*)eta33;
   s.audit = audit;
-  hipLaunchKernelGGL(k_modal_synth, dim3((d->count + kBlock - 1) / kBlock), dim3(kBlock), 0, st, s);
+  hipLaunchKernelGGL(k_modal_synth, dim3((d->count + 3) / 4), dim3(256), 0, st, s);
   GLW_HIP_CHECK(hipGetLastError());
-  return GLW_OK;
+  return mark(6);
+}
+
+extern "C" {
+
+int glw_modal_execute(const glw_modal_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,
+                      double* eta11, double* eta22, double* eta33, double* audit, void* stream) {
+  return modal_run(d, deposit, workspace, workspace_bytes, eta11, eta22, eta33, audit, (hipStream_t)stream, nullptr);
+}
+
+int glw_modal_stage_times(const glw_modal_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,
+                          double* eta11, double* eta22, double* eta33, double* audit, void* stream, float* ms) {
+  if (!ms) return mfail(GLW_EINVAL, "null stage-time array");
+  hipEvent_t ev[kModalStages + 1] = {};
+  int rc = GLW_OK;
+  for (int i = 0; i <= kModalStages && rc == GLW_OK; ++i)
+    if (hipEventCreate(&ev[i]) != hipSuccess) rc = mfail(GLW_EDEVICE, "hipEventCreate failed");
+  if (rc == GLW_OK)
+    rc = modal_run(d, deposit, workspace, workspace_bytes, eta11, eta22, eta33, audit, (hipStream_t)stream, ev);
+  if (rc == GLW_OK && hipEventSynchronize(ev[kModalStages]) != hipSuccess)
+    rc = mfail(GLW_EDEVICE, "hipEventSynchronize failed");
+  for (int i = 0; i < kModalStages && rc == GLW_OK; ++i)
+    if (hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]) != hipSuccess) rc = mfail(GLW_EDEVICE, "hipEventElapsedTime failed");
+  for (int i = 0; i <= kModalStages; ++i)
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  return rc;
 }
 
 static int check_eta20(const glw_eta20_desc* d) {
@@ -299,7 +365,7 @@ static Layout eta20_layout(const glw_eta20_desc& d) {
   L.tw_t = take(d.t_len * cs);
   L.tw_q = take(d.q_len * cs);
   L.tw_n = take((size_t)d.count * cs);
-  L.src = take(2 * nw * d.q_len * cs);
+  L.src = take(2 * (size_t)glw::modal::out_ld((int)nw) * d.q_len * cs);
   L.coef2 = take(nw * cs);
   L.energy2 = take(2 * nw * sizeof(double));
   L.total = off;

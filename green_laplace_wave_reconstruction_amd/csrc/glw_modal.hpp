@@ -17,10 +17,12 @@
 // which cancels the shifts: its spectra are signed, m in [-K, K] and
 // w in [-span, span].
 //
-// Layout in HBM.  x-space working arrays are [field][w][Q] (one contiguous
-// column of Q modal points per frequency), so the modal pass reads and writes
-// whole columns and the time pass touches RB consecutive x rows per
-// frequency.
+// Layout in HBM.  Every array is written with whole 128-byte lines: the time
+// pass writes its products row-contiguous, [field][Q][ldw] (ldw = kept
+// frequencies rounded up to 8), and the modal pass writes the pair fields
+// column-contiguous, [field][w][Q].  Each pass reads the other's layout
+// across the grain; workgroups that share lines are scheduled on one XCD
+// (row_block, col_block) so the lines are fetched into its L2 once.
 //
 // Kernels.
 //   k_modal_t<MODE>  one workgroup per RB x rows: builds each first-order
@@ -67,9 +69,9 @@ struct Args {
   const Dep* dep;        // [span + 1]
   const C* tw_t;         // exp(-2 pi i m / T), m < T
   const C* tw_q;         // exp(-2 pi i m / Q), m < Q
-  C* src;                // [2][2 span + 1][Q]  order-2 sources (d, k); eta20: forcing (d, k), [2][2 span + 1][Q]
+  C* src;                // [2][Q][ldw2]  order-2 sources (d, k); eta20: forcing (d, k) at signed columns
   C* pair;               // [5][2 span + 1][Q]  pair fields in x
-  C* forcing;            // [2][3 span + 1][Q]  order-3 forcing (d, k)
+  C* forcing;            // [2][Q][ldw3]  order-3 forcing (d, k)
   C* coef;               // [nw] output coefficient per kept frequency of the modal pass
   double* energy;        // [nw][2] source energy (total, invalid) per kept frequency
   int rank;
@@ -161,33 +163,42 @@ __device__ __forceinline__ Recipe recipe(int mode, int step) {
   }
 }
 
-// First-order row element (x row j, work frequency w < span + 1) with modal
-// power QP: sum over the two deposit nodes k of A_w wt_k (k dq)^QP
-// exp(2 pi i (k - k0) j / Q), the phase read from the Q-point table.
-__device__ __forceinline__ C first_value(const Args& a, int qp, int j, int w) {
-  const Dep d = a.dep[w];
+// First-order row element (x row j, work frequency w < span + 1) for the
+// modal powers q^0, q^1, q^2: sum over the two deposit nodes k of
+// A_w wt_k (k dq)^p exp(2 pi i (k - k0) j / Q), the phase read from the
+// Q-point table ((k - k0) j < Q^2 fits 32 bits for Q <= 8192).
+struct Powers {
+  C p0, p1, p2;
+};
+__device__ __forceinline__ Powers first_values(const Dep* __restrict__ dep, const C* __restrict__ twq, int k0, int q_len,
+                                               double dq, int j, int w) {
+  const Dep d = dep[w];
   const int klo = (int)d.klo;
-  const int i0 = (int)(((long long)(klo - a.k0) * j) % a.q_len);
+  const int i0 = ((klo - k0) * j) % q_len;
   int i1 = i0 + j;
-  if (i1 >= a.q_len) i1 -= a.q_len;
-  const C e0 = conj_(a.tw_q[i0]), e1 = conj_(a.tw_q[i1]);
-  double c0 = d.wlo, c1 = d.whi;
-  if (qp > 0) {
-    const double q0 = (double)klo * a.dq, q1 = (double)(klo + 1) * a.dq;
-    c0 *= qp == 1 ? q0 : q0 * q0;
-    c1 *= qp == 1 ? q1 : q1 * q1;
-  }
-  const C s = {c0 * e0.x + c1 * e1.x, c0 * e0.y + c1 * e1.y};
-  return {d.re * s.x - d.im * s.y, d.re * s.y + d.im * s.x};
+  if (i1 >= q_len) i1 -= q_len;
+  const C e0 = conj_(twq[i0]), e1 = conj_(twq[i1]);
+  const C amp = {d.re, d.im};
+  const C u0 = cmulr(cmul(amp, e0), d.wlo), u1 = cmulr(cmul(amp, e1), d.whi);
+  const double q0 = (double)klo * dq, q1 = (double)(klo + 1) * dq;
+  Powers b;
+  b.p0 = cadd(u0, u1);
+  b.p1 = {u0.x * q0 + u1.x * q1, u0.y * q0 + u1.y * q1};
+  b.p2 = {u0.x * (q0 * q0) + u1.x * (q1 * q1), u0.y * (q0 * q0) + u1.y * (q1 * q1)};
+  return b;
 }
 
-// Store of the kept frequencies of an inverse t transform, scaled by 1/T:
-// w < nw (shifted spectra) or signed w in [-span, span] at column w + span.
+// Row stride of a time-pass output with nw kept frequencies (whole lines).
+__host__ __device__ constexpr int out_ld(int nw) { return (nw + 7) & ~7; }
+
+// Store of the kept frequencies of an inverse t transform, scaled by 1/T,
+// into rows of `out` (row j0 + r at r * ld): w < nw (shifted spectra) or
+// signed w in [-span, span] at column w + span.
 template <bool SIGNED>
 struct TStore {
-  static constexpr bool SEQ_FAST = true;
-  C* out;  // out + field * nw * Q + j0
-  int q_len, t_len, nw, span;
+  static constexpr bool SEQ_FAST = false;  // lanes walk the contiguous frequency axis
+  C* out;  // out + field * Q * ld + j0 * ld
+  int ld, t_len, nw, span;
   double scale;
   __device__ void operator()(int r, int w, C v) const {
     int c = w;
@@ -198,15 +209,16 @@ struct TStore {
     } else {
       if (w >= nw) return;
     }
-    out[(size_t)c * q_len + r] = cmulr(v, scale);
+    out[(size_t)r * ld + c] = cmulr(v, scale);
   }
 };
 
-// Contiguous column loader / storer of the modal pass.
+// Column loader of the modal pass: element m of a column at stride ld.
 struct ColLoad {
   static constexpr bool SEQ_FAST = false;
   const C* col;
-  __device__ C operator()(int, int m) const { return col[m]; }
+  int ld;
+  __device__ C operator()(int, int m) const { return col[(size_t)m * ld]; }
 };
 struct ColStore {
   static constexpr bool SEQ_FAST = false;
@@ -214,14 +226,48 @@ struct ColStore {
   __device__ void operator()(int, int m, C v) const { col[m] = v; }
 };
 
-// Launch shape of the time pass at length T: RB rows per workgroup, NT
-// threads, PT points per thread (about four).
-__host__ __device__ constexpr int t_rows(int T) { return T <= 512 ? 4 : (T <= 1024 ? 2 : 1); }
+// Launch shape of the time pass at length T: RB rows per workgroup (RB T <=
+// kRowWork), NT threads, about kPoints points per thread.
+#ifndef GLW_MODAL_ROW_WORK
+#define GLW_MODAL_ROW_WORK 2048
+#endif
+#ifndef GLW_MODAL_POINTS
+#define GLW_MODAL_POINTS 4
+#endif
+constexpr int kRowWork = GLW_MODAL_ROW_WORK;
+constexpr int kPoints = GLW_MODAL_POINTS;
+__host__ __device__ constexpr int t_rows(int T) { return T >= kRowWork ? 1 : (kRowWork / T > 4 ? 4 : kRowWork / T); }
 __host__ __device__ constexpr int t_threads(int T) {
-  const int n = (t_rows(T) * T / 4) / 64 * 64;
-  return n < 64 ? 64 : (n > 512 ? 512 : n);
+  const int n = (t_rows(T) * T / kPoints) / 64 * 64;
+  return n < 64 ? 64 : (n > 1024 ? 1024 : n);
 }
+// Radix plan of the time-pass transforms (1: greedy radix 8/4/2/3, fewer
+// registers per butterfly than the composite radix-16 plan 0).
+#ifndef GLW_MODAL_TPLAN
+#define GLW_MODAL_TPLAN 1
+#endif
+constexpr int kTPlan = GLW_MODAL_TPLAN;
 __host__ __device__ constexpr int q_threads(int Q) { return Q <= 512 ? 64 : (Q <= 2048 ? 256 : 512); }
+
+// Row block of workgroup b.  A 128-byte line of a [w][Q] array holds 8
+// consecutive x rows; the G = 8 / RB workgroups that share those lines are
+// given consecutive dispatch slots of one XCD (workgroups are dealt to the
+// 8 XCDs round robin), so that XCD's L2 merges their partial-line writes and
+// serves their shared line reads.  Identity when the grid is not a multiple
+// of 8 G.
+__device__ __forceinline__ int row_block(int b, int nblocks, int rb) {
+  const int g = 8 / rb;
+  if (g <= 1 || b >= nblocks / (8 * g) * (8 * g)) return b;
+  return g * (8 * (b / (8 * g)) + b % 8) + (b / 8) % g;
+}
+
+// Column of modal-pass workgroup b: the 8 columns that share the lines of a
+// row-contiguous time-pass output go to consecutive slots of one XCD (the
+// tail beyond a multiple of 64 keeps its own index).
+__device__ __forceinline__ int col_block(int b, int nblocks) {
+  if (b >= nblocks / 64 * 64) return b;
+  return 8 * (8 * (b / 64) + b % 8) + (b / 8) % 8;
+}
 
 // Time pass.  MODE_O2: sources d = 2 v vn2 + vn^2 - hx^2, k = 2 v radial +
 // 2 hx jx (modal_time.py:220-227) into a.src.  MODE_O3: forcing
@@ -234,7 +280,7 @@ __global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
   constexpr int SS = seq_stride<T>();
   constexpr int PT = (RB * T + NT - 1) / NT;
   __shared__ C buf[RB * SS];
-  const int j0 = blockIdx.x * RB;
+  const int j0 = row_block(blockIdx.x, gridDim.x, RB) * RB;
   const int nbin = a.span + 1;
   const int nw2 = 2 * a.span + 1;
   const size_t fs2 = (size_t)nw2 * a.q_len;
@@ -242,22 +288,55 @@ __global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
   C f[PT], e[PT], x[PT], h[PT], acc_d[PT], acc_k[PT];
 #pragma unroll
   for (int i = 0; i < PT; ++i) acc_d[i] = acc_k[i] = x[i] = h[i] = e[i] = {0.0, 0.0};
+  // the first-order rows (q^0, q^1, q^2) of this thread's staging elements,
+  // element el = tid + k NT being (row el % RB, w = el / RB): built once and
+  // rescaled per field where the registers allow (order 2, eta20), rebuilt
+  // per field for order 3 (whose fifteen-field products need the registers)
+  constexpr bool kCache = MODE != MODE_O3;
+  constexpr int PC = kCache ? PT : 1;
+  C b0[PC], b1[PC], b2[PC];
+  if constexpr (kCache) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int el = threadIdx.x + k * NT;
+      const int r = el % RB, w = el / RB;
+      Powers b = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+      if (el < RB * T && w < nbin) b = first_values(a.dep, a.tw_q, a.k0, a.q_len, a.dq, j0 + r, w);
+      b0[k] = b.p0;
+      b1[k] = b.p1;
+      b2[k] = b.p2;
+    }
+  }
 #pragma unroll 1
   for (int step = 0; step < Steps<MODE>::N; ++step) {
     const Recipe rc = recipe(MODE, step);
     __syncthreads();  // previous transform fully read
-    for (int el = threadIdx.x; el < RB * T; el += NT) {
+#pragma unroll
+    for (int k = 0; k < PT; ++k) {
+      const int el = threadIdx.x + k * NT;
+      if (el >= RB * T) continue;
       const int r = el % RB, w = el / RB;  // rows fastest: pair loads coalesce over x
-      C v = {0.0, 0.0};
+      C v;
       if (!rc.pair) {
-        if (w < nbin) v = wmul(rc.wm, first_value(a, rc.idx, j0 + r, w), (double)(a.base + w) * a.fstep);
-      } else if (w < nw2) {
-        v = wmul(rc.wm, a.pair[rc.idx * fs2 + (size_t)w * a.q_len + j0 + r], (double)(2 * a.base + w) * a.fstep);
+        Powers b;
+        if constexpr (kCache) {
+          b = {b0[k], b1[k], b2[k]};
+        } else {
+          b = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+          if (w < nbin) b = first_values(a.dep, a.tw_q, a.k0, a.q_len, a.dq, j0 + r, w);
+        }
+        // arithmetic select: a branch or ?: on the runtime power is lowered to an indexed private array
+        const double s0 = rc.idx == 0, s1 = rc.idx == 1, s2 = rc.idx == 2;
+        const C u = {s0 * b.p0.x + s1 * b.p1.x + s2 * b.p2.x, s0 * b.p0.y + s1 * b.p1.y + s2 * b.p2.y};
+        v = wmul(rc.wm, u, (double)(a.base + w) * a.fstep);
+      } else {
+        v = w < nw2 ? wmul(rc.wm, a.pair[rc.idx * fs2 + (size_t)w * a.q_len + j0 + r], (double)(2 * a.base + w) * a.fstep)
+                    : C{0.0, 0.0};
       }
       buf[lds_at<T>(r, w)] = v;
     }
     __syncthreads();
-    fft_lds<double, T, RB, SS, NT, -1, 0>(buf, tw);
+    fft_lds<double, T, RB, SS, NT, -1, kTPlan>(buf, tw);
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int p = threadIdx.x + i * NT;
@@ -337,15 +416,17 @@ __global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
   NoIO none;
 #pragma unroll 1
   for (int o = 0; o < 2; ++o) {
+    const double od = o == 0, ok = o == 1;  // arithmetic select of the output (see above)
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int p = threadIdx.x + i * NT;
-      if (p < RB * T) buf[lds_at<T>(p / T, p % T)] = o == 0 ? acc_d[i] : acc_k[i];
+      if (p < RB * T) buf[lds_at<T>(p / T, p % T)] = C{od * acc_d[i].x + ok * acc_k[i].x, od * acc_d[i].y + ok * acc_k[i].y};
     }
     __syncthreads();
-    TStore<MODE == MODE_E20> st{out + (size_t)o * nw * a.q_len + j0, a.q_len, T, nw, a.span, 1.0 / T};
-    fft_io<double, T, RB, SS, NT, +1, 0>(buf, tw, none, st);
+    const int ld = out_ld(nw);
+    TStore<MODE == MODE_E20> st{out + ((size_t)o * a.q_len + j0) * ld, ld, T, nw, a.span, 1.0 / T};
+    fft_io<double, T, RB, SS, NT, +1, kTPlan>(buf, tw, none, st);
   }
 }
 
@@ -411,7 +492,7 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   constexpr int PQ = (Q + NT - 1) / NT;
   __shared__ C buf[SS];
   __shared__ double red[NT / 64 * 4];
-  const int w = blockIdx.x;
+  const int w = col_block(blockIdx.x, gridDim.x);
   const int nw = MODE == MODE_O3 ? 3 * a.span + 1 : 2 * a.span + 1;
   const C* in = MODE == MODE_O3 ? a.forcing : a.src;
   const double inv_q = 1.0 / Q;
@@ -420,8 +501,9 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   C fd[PQ];
 #pragma unroll 1
   for (int o = 0; o < 2; ++o) {
+    const double od = o == 0, ok = o == 1;  // arithmetic select of the output (see above)
     __syncthreads();
-    ColLoad ld{in + ((size_t)o * nw + w) * Q};
+    ColLoad ld{in + (size_t)o * Q * out_ld(nw) + w, out_ld(nw)};
     fft_io<double, Q, 1, SS, NT, -1, Q >= 3072 ? 1 : 0>(buf, a.tw_q, ld, none);
     if (o == 0) {
 #pragma unroll

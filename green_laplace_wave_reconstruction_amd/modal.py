@@ -60,6 +60,9 @@ def _bind(lib):
     lib.glw_modal_execute.argtypes = [POINTER(CModalDesc), c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
                                       c_void_p, c_void_p, c_void_p]
     lib.glw_modal_execute.restype = c_int
+    lib.glw_modal_stage_times.argtypes = [POINTER(CModalDesc), c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
+                                          c_void_p, c_void_p, c_void_p, POINTER(ctypes.c_float)]
+    lib.glw_modal_stage_times.restype = c_int
     lib.glw_eta20_workspace_size.argtypes = [POINTER(CEta20Desc), POINTER(c_size_t)]
     lib.glw_eta20_workspace_size.restype = c_int
     lib.glw_eta20_execute.argtypes = [POINTER(CEta20Desc), c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]
@@ -207,7 +210,9 @@ class ModalDevice:
 
         return torch.as_tensor(np.ascontiguousarray(deposit, dtype=np.float64), device=self.device)
 
-    def modal(self, desc: CModalDesc, dep, *, eta11: bool):
+    def modal(self, desc: CModalDesc, dep, *, eta11: bool, stage_ms: list | None = None):
+        """One level: (eta11, eta22, eta33, audit) device tensors.  With ``stage_ms`` (a list), the
+        call waits and appends the six stage durations in ms (glw_modal_stage_times)."""
         import torch
 
         size = c_size_t()
@@ -218,10 +223,15 @@ class ModalDevice:
         e22 = torch.empty(n, dtype=torch.complex128, device=self.device)
         e33 = torch.empty(n, dtype=torch.complex128, device=self.device) if desc.order == 3 else None
         audit = torch.empty(4, dtype=torch.float64, device=self.device)
-        _check(self.lib.glw_modal_execute(ctypes.byref(desc), c_void_p(dep.data_ptr()), c_void_p(ws.data_ptr()),
-                                          ws.numel(), None if e11 is None else c_void_p(e11.data_ptr()),
-                                          c_void_p(e22.data_ptr()), None if e33 is None else c_void_p(e33.data_ptr()),
-                                          c_void_p(audit.data_ptr()), self.stream()))
+        args = (ctypes.byref(desc), c_void_p(dep.data_ptr()), c_void_p(ws.data_ptr()), ws.numel(),
+                None if e11 is None else c_void_p(e11.data_ptr()), c_void_p(e22.data_ptr()),
+                None if e33 is None else c_void_p(e33.data_ptr()), c_void_p(audit.data_ptr()), self.stream())
+        if stage_ms is None:
+            _check(self.lib.glw_modal_execute(*args))
+        else:
+            ms = (ctypes.c_float * 6)()
+            _check(self.lib.glw_modal_stage_times(*args, ms))
+            stage_ms.append(list(ms))
         return e11, e22, e33, audit
 
     def eta20(self, desc: CEta20Desc, dep):

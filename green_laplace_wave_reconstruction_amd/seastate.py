@@ -195,3 +195,32 @@ def inverse_256(seed: int = 4) -> SeaState:
     L=1700 m (|k| < 4 kp stays cubic-alias safe below L=1787 m), parents kh > 0.5."""
     return jonswap_sea_state(nx=256, ny=256, lx=1700.0, ly=1700.0, depth=40.0, hs=5.0, tp=11.0, gamma=3.3,
                              spread_s=10.0, min_kh=0.5, seed=seed)
+
+
+def unidirectional_record(count: int = 2048, seed: int = 11, *, periods: float = 50.0, depth: float = 20.0,
+                          kp_h: float = 1.0, hs: float = 1.5, gamma: float = 3.3, gravity: float = 9.81):
+    """Single-direction JONSWAP elevation record eta1(t) at k_p h = kp_h over `periods` peak periods.
+
+    Random phases (numpy default_rng(seed)) on the record's own FFT bins, peak
+    enhancement gamma with sigma 0.07/0.09, scaled to significant height hs.
+    Returns (eta1, time, depth).  The reference's modal timing runs use
+    k_p h = 1 and 50 T_p records (python/docs/random_hos_modal_comparison.md).
+    """
+    kp = kp_h / depth
+    tp = 2 * math.pi / math.sqrt(gravity * kp * math.tanh(kp * depth))
+    dt = periods * tp / count
+    t = np.arange(count) * dt
+    f = np.fft.rfftfreq(count, dt)
+    fp = 1.0 / tp
+    s = np.zeros_like(f)
+    fk = f[1:]
+    sig = np.where(fk <= fp, 0.07, 0.09)
+    s[1:] = fk**-5.0 * np.exp(-1.25 * (fp / fk) ** 4) * gamma ** np.exp(-((fk - fp) ** 2) / (2 * sig**2 * fp**2))
+    df = f[1] - f[0]
+    s *= (hs / 4) ** 2 / np.sum(s * df)
+    rng = np.random.default_rng(seed)
+    half = 0.5 * count * np.sqrt(2 * s * df) * np.exp(2j * math.pi * rng.random(f.size))
+    half[0] = 0.0
+    if count % 2 == 0:
+        half[-1] = 0.0
+    return np.fft.irfft(half, count), t, depth

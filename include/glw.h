@@ -295,6 +295,12 @@ int glw_modal_length_supported(int32_t n);
 int glw_modal_workspace_size(const glw_modal_desc* desc, size_t* bytes);
 int glw_modal_execute(const glw_modal_desc* desc, const double* deposit, void* workspace, size_t workspace_bytes,
                       double* eta11, double* eta22, double* eta33, double* audit, void* stream);
+/* glw_modal_execute bracketed by HIP events; waits for them and returns the
+ * durations (ms[6]: twiddle tables, order-2 time pass, eta22 modal pass,
+ * order-3 time pass, eta33 modal pass, output synthesis).  Measurement only:
+ * no counterpart in the reference. */
+int glw_modal_stage_times(const glw_modal_desc* desc, const double* deposit, void* workspace, size_t workspace_bytes,
+                          double* eta11, double* eta22, double* eta33, double* audit, void* stream, float* ms);
 
 /* glw_eta20_execute replaces unidirectional_eta20
  * (python/src/green_laplace/_eta20.py:13-132): the difference-frequency

@@ -87,33 +87,8 @@ def off_grid_case():
     return eta1, time, kwargs
 
 
-def broadband_record(count: int = 2048, seed: int = 11, periods: float = 50.0, depth: float = 20.0,
-                     kp_h: float = 1.0, hs: float = 1.5, gamma: float = 3.3):
-    """Unidirectional JONSWAP record at k_p h = kp_h over `periods` peak periods (random phases).
+def broadband_record(count: int = 2048, seed: int = 11):
+    """k_p h = 1 JONSWAP record over 50 peak periods (green_laplace_wave_reconstruction_amd.seastate)."""
+    from green_laplace_wave_reconstruction_amd import seastate
 
-    Returns (eta1, time, depth); the record is synthesised on its own FFT bins.
-    """
-    g = 9.81
-    kp = kp_h / depth
-    wp = np.sqrt(g * kp * np.tanh(kp * depth))
-    tp = 2 * np.pi / wp
-    dt = periods * tp / count
-    t = np.arange(count) * dt
-    f = np.fft.rfftfreq(count, dt)
-    fp = 1.0 / tp
-    sigma = np.where(f <= fp, 0.07, 0.09)
-    s = np.zeros_like(f)
-    fk = f[1:]
-    s[1:] = fk**-5.0 * np.exp(-1.25 * (fp / fk) ** 4) * gamma ** np.exp(-((fk - fp) ** 2) / (2 * sigma[1:] ** 2 * fp**2))
-    df = f[1] - f[0]
-    s *= (hs / 4) ** 2 / np.sum(s * df)
-    rng = np.random.default_rng(seed)
-    amp = np.sqrt(2 * s * df)
-    phase = 2 * np.pi * rng.random(f.size)
-    spec = np.zeros(count, dtype=complex)
-    spec[: f.size] = 0.5 * count * amp * np.exp(1j * phase)
-    spec[0] = 0.0
-    if count % 2 == 0:
-        spec[count // 2] = 0.0
-    eta = np.fft.irfft(spec[: f.size], count)
-    return eta, t, depth
+    return seastate.unidirectional_record(count, seed)
