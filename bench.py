@@ -56,9 +56,9 @@ def workload():
     return s, c, kxg, kyg, spec, omega
 
 
-def traffic_from_profiles(kernel: str):
+def traffic_from_profiles(kernel: str, summary: str = "pmc_traffic.json"):
     """Measured HBM bytes per dispatch of `kernel` from a committed PMC summary, or None."""
-    p = ROOT / "profiles" / "pmc_traffic.json"
+    p = ROOT / "profiles" / summary
     if not p.exists():
         return None
     try:
@@ -378,13 +378,17 @@ def bench_modal(args):
     alg = _modal_alg_bytes(lp.desc)
     dominant = max(alg, key=lambda n: mean_ms[n])
     achieved = alg[dominant] / (mean_ms[dominant] * 1e-3) / 1e9
-    # end-to-end public calls (host analysis, planning, device, copies back)
-    t1 = time.perf_counter()
-    MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=MODAL_POINTS)
-    api_s = time.perf_counter() - t1
-    t2 = time.perf_counter()
-    lad = MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=(2048, 4096, 8192))
-    ladder_s = time.perf_counter() - t2
+    # end-to-end public calls (host analysis, planning, device, copies back), after one untimed call
+    api = None
+    if not args.no_api:
+        MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=MODAL_POINTS)
+        t1 = time.perf_counter()
+        MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=MODAL_POINTS)
+        api_s = time.perf_counter() - t1
+        t2 = time.perf_counter()
+        lad = MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=(2048, 4096, 8192))
+        ladder_s = time.perf_counter() - t2
+        api = {"one_level": api_s, "ladder_2048_4096_8192": ladder_s, "ladder_converged_at": lad.audit["modal_points"]}
     cpu = None
     if not args.no_cpu:
         from oracle import modal_oracle as M
@@ -410,9 +414,9 @@ def bench_modal(args):
                    "eta20_grid": {"q_len": int(d20.q_len), "t_len": int(d20.t_len)}},
         "stage_ms": mean_ms,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None, "alg_bytes_per_launch": alg[dominant]},
-        "api_seconds": {"one_level": api_s, "ladder_2048_4096_8192": ladder_s,
-                        "ladder_converged_at": lad.audit["modal_points"]},
+                     "frac": achieved / 8000.0, "traffic": traffic_from_profiles(dominant, "pmc_traffic_modal.json"),
+                     "alg_bytes_per_launch": alg[dominant], "avg_launch_ms": mean_ms[dominant]},
+        "api_seconds": api,
         "reference_published": {"seconds_8192_points": [6.78, 6.91], "ladder_median_seconds": 12.07,
                                 "source": "python/docs/random_hos_modal_comparison.md (HOS records, other machine)"},
         "cpu_baseline": cpu,
@@ -497,6 +501,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="order3: skip the FFT schedule autotuner")
+    ap.add_argument("--no-api", action="store_true", help="modal: skip the end-to-end API timings (profiling runs)")
     ap.add_argument("--chunk", type=int, default=INSTANTS,
                     help="order3: instants per plan evaluation (the 64 instants run in chunks of this size)")
     ap.add_argument("--schedule", default="", help="order3: use this FFT schedule (stage=plan/tile;...) instead of tuning")

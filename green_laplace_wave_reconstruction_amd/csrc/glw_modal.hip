@@ -139,20 +139,26 @@ __global__ __launch_bounds__(256) void k_modal_synth(SynthArgs s) {
 // eta20 record: Hermitian projection of the signed-bin coefficients
 // (_eta20.py:175-185) and the real part of their synthesis (_eta20.py:125):
 // c'_0 = Re c_0, c'_s = (c_s + conj c_-s) / 2, eta20 = c'_0 + 2 Re sum_s>0 c'_s e^{-2 pi i s n / N}.
-__global__ void k_eta20_synth(const C* coef, int span, int count, const C* tw_n, double* out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave per output sample, lanes over the bins.
+__global__ __launch_bounds__(256) void k_eta20_synth(const C* coef, int span, int count, const C* tw_n, double* out) {
+  const int n = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (n >= count) return;
   double acc = 0.0;
-  long long idx = 0;
-  for (int sb = 1; sb <= span; ++sb) {
-    idx += n;
-    if (idx >= count) idx -= count;
-    const C cp = coef[span + sb], cm = coef[span - sb];
-    const double re = 0.5 * (cp.x + cm.x), im = 0.5 * (cp.y - cm.y);
-    const C e = tw_n[idx];
-    acc += re * e.x - im * e.y;
+  if (lane < span) {
+    long long idx = ((long long)(lane + 1) * n) % count;
+    const long long step = (64LL * n) % count;
+    for (int sb = lane + 1; sb <= span; sb += 64) {
+      const C cp = coef[span + sb], cm = coef[span - sb];
+      const double re = 0.5 * (cp.x + cm.x), im = 0.5 * (cp.y - cm.y);
+      const C e = tw_n[idx];
+      acc += re * e.x - im * e.y;
+      idx += step;
+      if (idx >= count) idx -= count;
+    }
   }
-  out[n] = coef[span].x + 2.0 * acc;
+  acc = wave_sum(acc);
+  if (lane == 0) out[n] = coef[span].x + 2.0 * acc;
 }
 
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -416,7 +422,7 @@ int glw_eta20_execute(const glw_eta20_desc* d, const double* deposit, void* work
   if (int rc = twiddles((C*)(ws + L.tw_n), d->count, st)) return rc;
   if (int rc = launched(glw::modal::launch_t_eta20(a, st), "eta20 time pass")) return rc;
   if (int rc = launched(glw::modal::launch_q_eta20(a, 2 * d->span + 1, st), "eta20 modal pass")) return rc;
-  hipLaunchKernelGGL(k_eta20_synth, dim3((d->count + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+  hipLaunchKernelGGL(k_eta20_synth, dim3((d->count + 3) / 4), dim3(256), 0, st,
                      (const C*)(ws + L.coef2), d->span, d->count, (const C*)(ws + L.tw_n), eta20);
   GLW_HIP_CHECK(hipGetLastError());
   return GLW_OK;
