@@ -77,3 +77,49 @@ def test_eta20_plan_and_validation():
         MD.reconstruct_unidirectional_modal_timeseries(eta1, tt, **dict(kw, modal_points=(512, 256)))
     with pytest.raises(ValueError, match="rank 4, 6, 8, or 10"):
         MD.reconstruct_unidirectional_modal_timeseries(eta1, tt, **dict(kw, quadrature_rank=5))
+
+
+def test_capi_rejects_bad_descriptors():
+    """glw_modal_workspace_size / glw_eta20_workspace_size validate without a device."""
+    import ctypes
+
+    _lib_or_skip()
+    lib = MD._bind(_native.load_library())
+    eta, t, h = broadband_record()
+    band = MD._select_band(eta, t, depth=h, gravity=9.81, order=3, omega_max=None, energy_fraction=0.999)
+    good = MD.modal_level_plan(band, depth=h, gravity=9.81, rank=8, order=3, points=2048, modal_limit=None).desc
+    size = ctypes.c_size_t()
+    assert lib.glw_modal_workspace_size(ctypes.byref(good), ctypes.byref(size)) == 0 and size.value > 0
+    for field, value, code in (("order", 4, 1), ("q_len", 1000, 3), ("t_len", 48, 1), ("rank", 17, 1),
+                               ("q_len", 16, 1), ("dq", 0.0, 1), ("count", 300, 1)):
+        bad = MD.CModalDesc.from_buffer_copy(good)
+        setattr(bad, field, value)
+        assert lib.glw_modal_workspace_size(ctypes.byref(bad), ctypes.byref(size)) == code, field
+        assert lib.glw_last_error()
+    d20 = MD.CEta20Desc(count=256, base_bin=1, span=10, q_len=64, t_len=32, k0=3, kspan=5, rank=16, dq=0.1,
+                        fstep=0.05, depth=1.0, delta_q=0.07)
+    assert lib.glw_eta20_workspace_size(ctypes.byref(d20), ctypes.byref(size)) == 0
+    d20.t_len = 16  # below 2 span + 1
+    assert lib.glw_eta20_workspace_size(ctypes.byref(d20), ctypes.byref(size)) != 0
+
+
+def test_degenerate_eta20_and_grid_limit():
+    _lib_or_skip()
+    # a single input bin has no difference band: eta20 is exactly zero (_eta20.py:94-100)
+    n = 256
+    t = np.arange(n) * 0.25
+    one = 0.01 * np.cos(2 * np.pi * 5 * np.arange(n) / n)
+    band = MD._select_band(one, t, depth=1.0, gravity=9.81, order=2, omega_max=2 * np.pi * 5 / (n * 0.25) + 1e-12,
+                           energy_fraction=1.0)
+    ob = M.select_band(one, t, depth=1.0, gravity=9.81, order=2, omega_max=2 * np.pi * 5 / (n * 0.25) + 1e-12,
+                       energy_fraction=1.0)
+    desc, dep, audit = MD.eta20_plan(band.amplitudes, band.omega, band.q, band.count, band.dt, depth=1.0,
+                                     gravity=9.81, rank=16, nx=128, ny=16)
+    e20, a20 = M.eta20(ob.amplitudes, ob.omega, ob.q, t, depth=1.0, gravity=9.81)
+    assert desc is None and audit["status"] == a20["status"] == "degenerate-zero-difference-output"
+    assert not np.any(e20)
+    # a level whose reduced modal grid exceeds the largest supported length raises a clear error
+    eta, tt, h = broadband_record()
+    band = MD._select_band(eta, tt, depth=h, gravity=9.81, order=3, omega_max=None, energy_fraction=0.999)
+    with pytest.raises(_native.GlwError, match="no supported modal q grid length"):
+        MD.modal_level_plan(band, depth=h, gravity=9.81, rank=8, order=3, points=65536, modal_limit=None)
