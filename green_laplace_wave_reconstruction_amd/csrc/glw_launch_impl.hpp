@@ -10,7 +10,14 @@
 
 namespace glw {
 
-constexpr int kLdsBudget = 48 * 1024;  // column tiles: keep several workgroups per CU
+#ifndef GLW_COL_LDS_KB
+#define GLW_COL_LDS_KB 48
+#endif
+#ifndef GLW_ROW_LDS_KB
+#define GLW_ROW_LDS_KB 64
+#endif
+constexpr int kLdsBudget = GLW_COL_LDS_KB * 1024;  // column tiles: keep several workgroups per CU
+constexpr int kRowLds = GLW_ROW_LDS_KB * 1024;     // fused row kernel tile
 constexpr int kLdsMax = 160 * 1024;    // gfx950 LDS per workgroup
 
 // Widest power-of-two column tile (<= 32 columns) within kLdsBudget.
@@ -22,11 +29,11 @@ constexpr int col_width() {
   return p;
 }
 
-// Rows per workgroup of the fused row kernel: as many as fit 64 KB (at most 4).
+// Rows per workgroup of the fused row kernel: as many as fit kRowLds (at most 4).
 template <typename T, int NIN, int L>
 constexpr int row_ry() {
   int per = NIN * seq_stride<L>() * (int)sizeof(cpx<T>);
-  int ry = (64 * 1024) / per;
+  int ry = kRowLds / per;
   if (ry >= 4) return 4;
   if (ry >= 2) return 2;
   return 1;

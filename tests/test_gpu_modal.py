@@ -106,3 +106,17 @@ def test_homogeneity_largest_grid(MD):
     assert r1.audit["reduced_grid"]["q_len"] == 8192
     assert rel(r2.components["eta22_analytic"], 1.7**2 * r1.components["eta22_analytic"]) < 1e-13
     assert rel(r2.components["eta33_analytic"], 1.7**3 * r1.components["eta33_analytic"]) < 1e-13
+
+
+def test_bench_record_8192_vs_oracle(MD):
+    """The measured workload itself: 2048 samples, 267 bins, one 8192-point level (grids 4096 x 1024) + eta20."""
+    eta, t, h = broadband_record()
+    kw = dict(depth=h, quadrature_rank=8, order=3, modal_points=8192, include_eta20=True)
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    o = M.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    assert r.audit["reduced_grid"] == {"q_len": 4096, "t_len": 1024}
+    errs = {k: rel(r.components[k], o.components[k]) for k in ("eta11", "eta22_analytic", "eta33_analytic", "eta20")}
+    print("bench record 8192", errs, flush=True)
+    assert max(errs.values()) < 1e-12
+    for key in ("eta22_invalid_source_energy_fraction", "eta33_invalid_source_energy_fraction"):
+        assert r.audit[key] == pytest.approx(o.audit[key], rel=1e-6, abs=1e-18)
