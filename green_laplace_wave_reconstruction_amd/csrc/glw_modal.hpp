@@ -200,7 +200,11 @@ struct TStore {
   C* out;  // out + field * Q * ld + j0 * ld
   int ld, t_len, nw, span;
   double scale;
-  __device__ void operator()(int r, int w, C v) const {
+  int rb;              // rows per field: sequence r >= rb is row r - rb of the next field
+  size_t field_step;   // elements between fields (Q ld)
+  __device__ void operator()(int seq, int w, C v) const {
+    const int fld = seq >= rb ? 1 : 0;
+    const int r = seq - fld * rb;
     int c = w;
     if constexpr (SIGNED) {
       if (w <= span) c = w + span;
@@ -209,7 +213,7 @@ struct TStore {
     } else {
       if (w >= nw) return;
     }
-    out[(size_t)r * ld + c] = cmulr(v, scale);
+    out[fld * field_step + (size_t)r * ld + c] = cmulr(v, scale);
   }
 };
 
@@ -274,12 +278,23 @@ __device__ __forceinline__ int col_block(int b, int nblocks) {
 // (modal_time.py:273-310) into a.forcing.  MODE_E20: difference forcing
 // (_eta20.py:64-86, the qy = 0 plane) into a.src at signed columns.
 // Every field runs through the same staged LDS transform (one FFT instance).
+// Fields transformed per LDS FFT call in the time pass (1 or 2): two fields
+// give the radix stages twice the butterflies (all threads busy at RB = 2)
+// and half the barriers (order-3 pass at T = 1024: 0.63 ms instead of
+// 0.75 ms).  Lengths with a radix-3 stage keep one field: their butterflies
+// need the registers (two fields spill there).
+#ifndef GLW_MODAL_FIELDS_PER_FFT
+#define GLW_MODAL_FIELDS_PER_FFT 2
+#endif
+__host__ __device__ constexpr int fields_per_fft(int T) { return T % 3 == 0 ? 1 : GLW_MODAL_FIELDS_PER_FFT; }
+
 template <int T, int MODE>
 __global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
   constexpr int RB = t_rows(T), NT = t_threads(T);
   constexpr int SS = seq_stride<T>();
   constexpr int PT = (RB * T + NT - 1) / NT;
-  __shared__ C buf[RB * SS];
+  constexpr int FB = fields_per_fft(T);
+  __shared__ C buf[FB * RB * SS];
   const int j0 = row_block(blockIdx.x, gridDim.x, RB) * RB;
   const int nbin = a.span + 1;
   const int nw2 = 2 * a.span + 1;
@@ -307,17 +322,18 @@ __global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
       b2[k] = b.p2;
     }
   }
-#pragma unroll 1
-  for (int step = 0; step < Steps<MODE>::N; ++step) {
-    const Recipe rc = recipe(MODE, step);
-    __syncthreads();  // previous transform fully read
+  // stage field `step` of the sequence into LDS sequences slot * RB + r
+  auto stage = [&](int step, int slot) {
+    const Recipe rc = recipe(MODE, step < Steps<MODE>::N ? step : 0);
+    const bool live = step < Steps<MODE>::N;
 #pragma unroll
     for (int k = 0; k < PT; ++k) {
       const int el = threadIdx.x + k * NT;
       if (el >= RB * T) continue;
       const int r = el % RB, w = el / RB;  // rows fastest: pair loads coalesce over x
-      C v;
-      if (!rc.pair) {
+      C v = {0.0, 0.0};
+      if (!live) {
+      } else if (!rc.pair) {
         Powers b;
         if constexpr (kCache) {
           b = {b0[k], b1[k], b2[k]};
@@ -329,104 +345,134 @@ __global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
         const double s0 = rc.idx == 0, s1 = rc.idx == 1, s2 = rc.idx == 2;
         const C u = {s0 * b.p0.x + s1 * b.p1.x + s2 * b.p2.x, s0 * b.p0.y + s1 * b.p1.y + s2 * b.p2.y};
         v = wmul(rc.wm, u, (double)(a.base + w) * a.fstep);
+      } else if (w < nw2) {
+        v = wmul(rc.wm, a.pair[rc.idx * fs2 + (size_t)w * a.q_len + j0 + r], (double)(2 * a.base + w) * a.fstep);
+      }
+      buf[lds_at<T>(slot * RB + r, w)] = v;
+    }
+  };
+  // fold the transformed field `step` (in f) into the per-point products
+  auto update = [&](int step) {
+      if constexpr (MODE == MODE_O2) {
+        // e = v, h = hx
+  #pragma unroll
+        for (int i = 0; i < PT; ++i) {
+          switch (step) {
+            case 0: e[i] = f[i]; break;
+            case 1: acc_d[i] = cmul(f[i], f[i]); break;
+            case 2: acc_d[i] = cfma(acc_d[i], cmulr(e[i], 2.0), f[i]); break;
+            case 3: h[i] = f[i]; acc_d[i] = csub(acc_d[i], cmul(f[i], f[i])); break;
+            case 4: acc_k[i] = cmul(cmulr(h[i], 2.0), f[i]); break;
+            default: acc_k[i] = cfma(acc_k[i], cmulr(e[i], 2.0), f[i]); break;
+          }
+        }
+      } else if constexpr (MODE == MODE_O3) {
+        // fk = (eta phixz etax - eta^2 phizzz / 2) / 4 + (phix Getax + Gphix etax - eta Gphizz - Geta phizz) / 2
+        // fd = (eta^2 phitzz / 2 + eta (phix phixz + phiz phizz)) / 4
+        //      + (eta Gphitz + Geta phitz + phix Gphix + phiz Gphiz) / 2
+        // live values: e = eta; x = etax, then phiz, then Geta; h = phix, then phizz
+  #pragma unroll
+        for (int i = 0; i < PT; ++i) {
+          switch (step) {
+            case 0: e[i] = f[i]; break;
+            case 1: acc_k[i] = cmul(cmulr(cmul(e[i], e[i]), -0.125), f[i]); break;
+            case 2: acc_d[i] = cmul(cmulr(cmul(e[i], e[i]), 0.125), f[i]); break;
+            case 3: acc_k[i] = csub(acc_k[i], cmul(cmulr(e[i], 0.5), f[i])); break;
+            case 4: acc_d[i] = cfma(acc_d[i], cmulr(e[i], 0.5), f[i]); break;
+            case 5: x[i] = f[i]; break;
+            case 6: h[i] = f[i]; break;
+            case 7: {
+              const C ef = cmulr(cmul(e[i], f[i]), 0.25);
+              acc_k[i] = cfma(acc_k[i], ef, x[i]);
+              acc_d[i] = cfma(acc_d[i], ef, h[i]);
+              break;
+            }
+            case 8: acc_k[i] = cfma(acc_k[i], cmulr(h[i], 0.5), f[i]); break;
+            case 9: {
+              const C g = cmulr(f[i], 0.5);
+              acc_k[i] = cfma(acc_k[i], g, x[i]);
+              acc_d[i] = cfma(acc_d[i], g, h[i]);
+              break;
+            }
+            case 10: x[i] = f[i]; break;
+            case 11: h[i] = f[i]; acc_d[i] = cfma(acc_d[i], cmulr(cmul(e[i], x[i]), 0.25), f[i]); break;
+            case 12: acc_d[i] = cfma(acc_d[i], cmulr(x[i], 0.5), f[i]); break;
+            case 13: x[i] = f[i]; acc_k[i] = csub(acc_k[i], cmul(cmulr(f[i], 0.5), h[i])); break;
+            default: acc_d[i] = cfma(acc_d[i], cmulr(x[i], 0.5), f[i]); break;
+          }
+        }
       } else {
-        v = w < nw2 ? wmul(rc.wm, a.pair[rc.idx * fs2 + (size_t)w * a.q_len + j0 + r], (double)(2 * a.base + w) * a.fstep)
-                    : C{0.0, 0.0};
+        // fd = -dno one* - one dno* + nu nu* + bx bx*
+        // fk = i (radial one* - bx qx* + qx bx* - one radial*);   e = one, h = bx
+  #pragma unroll
+        for (int i = 0; i < PT; ++i) {
+          switch (step) {
+            case 0: e[i] = f[i]; break;
+            case 1: acc_d[i] = csub(cmul(cmulr(f[i], -1.0), conj_(e[i])), cmul(e[i], conj_(f[i]))); break;
+            case 2: acc_d[i] = cfma(acc_d[i], f[i], conj_(f[i])); break;
+            case 3: h[i] = f[i]; acc_d[i] = cfma(acc_d[i], f[i], conj_(f[i])); break;
+            case 4: acc_k[i] = csub(cmul(f[i], conj_(h[i])), cmul(h[i], conj_(f[i]))); break;
+            default: {
+              const C t = csub(cadd(acc_k[i], cmul(f[i], conj_(e[i]))), cmul(e[i], conj_(f[i])));
+              acc_k[i] = {-t.y, t.x};
+              break;
+            }
+          }
+        }
       }
-      buf[lds_at<T>(r, w)] = v;
-    }
+
+  };
+#pragma unroll 1
+  for (int s0 = 0; s0 < Steps<MODE>::N; s0 += FB) {
+    __syncthreads();  // previous transform fully read
+#pragma unroll
+    for (int slot = 0; slot < FB; ++slot) stage(s0 + slot, slot);
     __syncthreads();
-    fft_lds<double, T, RB, SS, NT, -1, kTPlan>(buf, tw);
+    fft_lds<double, T, FB * RB, SS, NT, -1, kTPlan>(buf, tw);
 #pragma unroll
-    for (int i = 0; i < PT; ++i) {
-      const int p = threadIdx.x + i * NT;
-      f[i] = p < RB * T ? buf[lds_at<T>(p / T, p % T)] : C{0.0, 0.0};
-    }
-    if constexpr (MODE == MODE_O2) {
-      // e = v, h = hx
+    for (int slot = 0; slot < FB; ++slot) {
+      if (s0 + slot >= Steps<MODE>::N) break;
 #pragma unroll
       for (int i = 0; i < PT; ++i) {
-        switch (step) {
-          case 0: e[i] = f[i]; break;
-          case 1: acc_d[i] = cmul(f[i], f[i]); break;
-          case 2: acc_d[i] = cfma(acc_d[i], cmulr(e[i], 2.0), f[i]); break;
-          case 3: h[i] = f[i]; acc_d[i] = csub(acc_d[i], cmul(f[i], f[i])); break;
-          case 4: acc_k[i] = cmul(cmulr(h[i], 2.0), f[i]); break;
-          default: acc_k[i] = cfma(acc_k[i], cmulr(e[i], 2.0), f[i]); break;
-        }
+        const int p = threadIdx.x + i * NT;
+        f[i] = p < RB * T ? buf[lds_at<T>(slot * RB + p / T, p % T)] : C{0.0, 0.0};
       }
-    } else if constexpr (MODE == MODE_O3) {
-      // fk = (eta phixz etax - eta^2 phizzz / 2) / 4 + (phix Getax + Gphix etax - eta Gphizz - Geta phizz) / 2
-      // fd = (eta^2 phitzz / 2 + eta (phix phixz + phiz phizz)) / 4
-      //      + (eta Gphitz + Geta phitz + phix Gphix + phiz Gphiz) / 2
-      // live values: e = eta; x = etax, then phiz, then Geta; h = phix, then phizz
-#pragma unroll
-      for (int i = 0; i < PT; ++i) {
-        switch (step) {
-          case 0: e[i] = f[i]; break;
-          case 1: acc_k[i] = cmul(cmulr(cmul(e[i], e[i]), -0.125), f[i]); break;
-          case 2: acc_d[i] = cmul(cmulr(cmul(e[i], e[i]), 0.125), f[i]); break;
-          case 3: acc_k[i] = csub(acc_k[i], cmul(cmulr(e[i], 0.5), f[i])); break;
-          case 4: acc_d[i] = cfma(acc_d[i], cmulr(e[i], 0.5), f[i]); break;
-          case 5: x[i] = f[i]; break;
-          case 6: h[i] = f[i]; break;
-          case 7: {
-            const C ef = cmulr(cmul(e[i], f[i]), 0.25);
-            acc_k[i] = cfma(acc_k[i], ef, x[i]);
-            acc_d[i] = cfma(acc_d[i], ef, h[i]);
-            break;
-          }
-          case 8: acc_k[i] = cfma(acc_k[i], cmulr(h[i], 0.5), f[i]); break;
-          case 9: {
-            const C g = cmulr(f[i], 0.5);
-            acc_k[i] = cfma(acc_k[i], g, x[i]);
-            acc_d[i] = cfma(acc_d[i], g, h[i]);
-            break;
-          }
-          case 10: x[i] = f[i]; break;
-          case 11: h[i] = f[i]; acc_d[i] = cfma(acc_d[i], cmulr(cmul(e[i], x[i]), 0.25), f[i]); break;
-          case 12: acc_d[i] = cfma(acc_d[i], cmulr(x[i], 0.5), f[i]); break;
-          case 13: x[i] = f[i]; acc_k[i] = csub(acc_k[i], cmul(cmulr(f[i], 0.5), h[i])); break;
-          default: acc_d[i] = cfma(acc_d[i], cmulr(x[i], 0.5), f[i]); break;
-        }
-      }
-    } else {
-      // fd = -dno one* - one dno* + nu nu* + bx bx*
-      // fk = i (radial one* - bx qx* + qx bx* - one radial*);   e = one, h = bx
-#pragma unroll
-      for (int i = 0; i < PT; ++i) {
-        switch (step) {
-          case 0: e[i] = f[i]; break;
-          case 1: acc_d[i] = csub(cmul(cmulr(f[i], -1.0), conj_(e[i])), cmul(e[i], conj_(f[i]))); break;
-          case 2: acc_d[i] = cfma(acc_d[i], f[i], conj_(f[i])); break;
-          case 3: h[i] = f[i]; acc_d[i] = cfma(acc_d[i], f[i], conj_(f[i])); break;
-          case 4: acc_k[i] = csub(cmul(f[i], conj_(h[i])), cmul(h[i], conj_(f[i]))); break;
-          default: {
-            const C t = csub(cadd(acc_k[i], cmul(f[i], conj_(e[i]))), cmul(e[i], conj_(f[i])));
-            acc_k[i] = {-t.y, t.x};
-            break;
-          }
-        }
-      }
+      update(s0 + slot);
     }
   }
   // inverse t transforms of (d, k), kept frequencies stored at 1/T
   C* out = MODE == MODE_O3 ? a.forcing : a.src;
   const int nw = MODE == MODE_O3 ? 3 * a.span + 1 : nw2;
+  const int ld = out_ld(nw);
   NoIO none;
-#pragma unroll 1
-  for (int o = 0; o < 2; ++o) {
-    const double od = o == 0, ok = o == 1;  // arithmetic select of the output (see above)
+  if constexpr (FB == 2) {
     __syncthreads();
 #pragma unroll
     for (int i = 0; i < PT; ++i) {
       const int p = threadIdx.x + i * NT;
-      if (p < RB * T) buf[lds_at<T>(p / T, p % T)] = C{od * acc_d[i].x + ok * acc_k[i].x, od * acc_d[i].y + ok * acc_k[i].y};
+      if (p < RB * T) {
+        buf[lds_at<T>(p / T, p % T)] = acc_d[i];
+        buf[lds_at<T>(RB + p / T, p % T)] = acc_k[i];
+      }
     }
     __syncthreads();
-    const int ld = out_ld(nw);
-    TStore<MODE == MODE_E20> st{out + ((size_t)o * a.q_len + j0) * ld, ld, T, nw, a.span, 1.0 / T};
-    fft_io<double, T, RB, SS, NT, +1, kTPlan>(buf, tw, none, st);
+    // sequences 0..RB-1: d rows, RB..2RB-1: k rows (field stride Q ld)
+    TStore<MODE == MODE_E20> st{out + (size_t)j0 * ld, ld, T, nw, a.span, 1.0 / T, RB, (size_t)a.q_len * ld};
+    fft_io<double, T, FB * RB, SS, NT, +1, kTPlan>(buf, tw, none, st);
+  } else {
+#pragma unroll 1
+    for (int o = 0; o < 2; ++o) {
+      const double od = o == 0, ok = o == 1;  // arithmetic select of the output (see above)
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < PT; ++i) {
+        const int p = threadIdx.x + i * NT;
+        if (p < RB * T) buf[lds_at<T>(p / T, p % T)] = C{od * acc_d[i].x + ok * acc_k[i].x, od * acc_d[i].y + ok * acc_k[i].y};
+      }
+      __syncthreads();
+      TStore<MODE == MODE_E20> st{out + ((size_t)o * a.q_len + j0) * ld, ld, T, nw, a.span, 1.0 / T, RB, 0};
+      fft_io<double, T, RB, SS, NT, +1, kTPlan>(buf, tw, none, st);
+    }
   }
 }
 
