@@ -120,3 +120,16 @@ def test_bench_record_8192_vs_oracle(MD):
     assert max(errs.values()) < 1e-12
     for key in ("eta22_invalid_source_energy_fraction", "eta33_invalid_source_energy_fraction"):
         assert r.audit[key] == pytest.approx(o.audit[key], rel=1e-6, abs=1e-18)
+
+
+def test_broadband_order2_vs_oracle(MD):
+    """Order 2 (no pair fields, 2 span + 1 time grid) on the broadband record, GL rank 6."""
+    eta, t, h = broadband_record(1536, seed=21)
+    kw = dict(depth=h, quadrature_rank=6, order=2, modal_points=(1024, 4096), relative_tolerance=1e-12,
+              include_eta20=True, eta20_rank=12)
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    o = M.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    assert "eta33" not in r.components and r.audit["eta33_invalid_source_energy_fraction"] is None
+    assert rel(r.components["eta22_analytic"], o.components["eta22_analytic"]) < 1e-12
+    assert rel(r.components["eta20"], o.components["eta20"]) < 1e-12
+    assert r.audit["levels"][1]["eta22_change"] == pytest.approx(o.audit["levels"][1]["eta22_change"], rel=1e-9)
