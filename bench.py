@@ -575,6 +575,22 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     stages = dp.stage_times()
+    # PCIe-inclusive rate (outside `value`): the step's eta and psi fields copied to pinned host memory
+    pinned = [torch.empty(eta.shape, dtype=eta.dtype, pin_memory=True) for _ in range(2)]
+    for _ in range(2):
+        pinned[0].copy_(eta, non_blocking=True)
+        pinned[1].copy_(psi, non_blocking=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    reps = 5
+    for _ in range(reps):
+        pinned[0].copy_(eta, non_blocking=True)
+        pinned[1].copy_(psi, non_blocking=True)
+    torch.cuda.synchronize()
+    d2h_s = (time.perf_counter() - t1) / reps
+    pcie = {"d2h_ms_per_step": d2h_s * 1e3, "bytes_per_step": 2 * eta.numel() * eta.element_size(),
+            "fields_per_s_with_d2h": INSTANTS / (wall / args.steps + d2h_s),
+            "note": "eta+psi of the 64 instants copied to pinned host memory after each step (serial)"}
     ms_per_step = wall / args.steps * 1e3
     value = INSTANTS * args.steps / wall
 
@@ -633,6 +649,7 @@ def main():
         "stages_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
         "rel_l2_vs_oracle": errs,
         "cpu_baseline": cpu,
+        "pcie_inclusive": pcie,
     }
     print(json.dumps(result))
 
