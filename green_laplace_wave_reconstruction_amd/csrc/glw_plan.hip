@@ -470,33 +470,59 @@ __global__ __launch_bounds__(NT) void k_adj_acc_inner(AccInnerArgs g, const cpx<
 // Adjoint of a column-inverse pass (k_colinv) after its y-DFT: x [ntask][nfield][box]
 // holds DFT_y of the column gradients; out[set][src] += sum over the set's tasks
 // and fields f of conj(multiplier_f) * damping * fscale_f * x (floored bins excluded).
+// Workgroup tile of the collection: kCollectBins consecutive bins; the
+// (task, field) terms of each bin are split over NT / kCollectBins thread
+// groups and summed through LDS in a fixed order (deterministic).
+constexpr int kCollectBins = 16;
+constexpr int kCollectGroups = NT / kCollectBins;
+
 __global__ __launch_bounds__(NT) void k_adj_collect(ColInvArgs a, const cpx<double>* x, cpx<double>* out) {
-  const int b = blockIdx.x * NT + threadIdx.x;
+  __shared__ cpx<double> part[kCollectGroups][4][kCollectBins];
+  const int lb = threadIdx.x % kCollectBins, grp = threadIdx.x / kCollectBins;
+  const int b = blockIdx.x * kCollectBins + lb;
   const int set = blockIdx.y;
   const int box = a.wx * a.wy;
-  if (b >= box) return;
-  cpx<double> acc[4] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-  bool keep[4] = {true, true, true, true};
-  for (int s = 0; s < a.nsrc && s < 4; ++s)
-    if (a.fmax && ((a.floor_src_mask >> s) & 1)) {
-      const size_t k = (size_t)set * a.fmax_stride + (a.fmax_stride == 1 ? 0 : s);
-      keep[s] = kept(a.spec[((size_t)set * a.nsrc + s) * box + b], a.floor_level * bits_to_d(a.fmax[k]));
-    }
-  for (int u = 0; u < a.sdiv; ++u) {
-    const int task = set * a.sdiv + u;
-    const double dmp = a.damp ? a.damp[(size_t)((task % a.dmod) * a.dmul + a.dadd) * box + b] : 1.0;
-    for (int f = 0; f < a.nfield; ++f) {
+  const int nsrc = a.nsrc < 4 ? a.nsrc : 4;
+  // source s of field f is summed in acc_s; selects keep the four sums in registers
+  cpx<double> acc0 = {0, 0}, acc1 = {0, 0}, acc2 = {0, 0}, acc3 = {0, 0};
+  if (b < box) {
+    const int nterm = a.sdiv * a.nfield;
+    for (int it = grp; it < nterm; it += kCollectGroups) {
+      const int u = it / a.nfield, f = it - u * a.nfield;
+      const int task = set * a.sdiv + u;
+      const double dmp = a.damp ? a.damp[(size_t)((task % a.dmod) * a.dmul + a.dadd) * box + b] : 1.0;
       const cpx<double> m = apply_mul(a.fcode[f], cpx<double>{1.0, 0.0}, a.tab, b);
       const cpx<double> v = x[((size_t)task * a.nfield + f) * box + b];
       const cpx<double> w = conj_mul(m, v);
       const double sc = dmp * a.fscale[f];
-      const int s = a.fsrc[f];
-      acc[s] = {acc[s].x + sc * w.x, acc[s].y + sc * w.y};
+      const cpx<double> t = {sc * w.x, sc * w.y};
+      const int src = a.fsrc[f];
+      if (src == 0) acc0 = cadd(acc0, t);
+      else if (src == 1) acc1 = cadd(acc1, t);
+      else if (src == 2) acc2 = cadd(acc2, t);
+      else acc3 = cadd(acc3, t);
     }
   }
-  for (int s = 0; s < a.nsrc && s < 4; ++s) {
-    cpx<double>* o = out + ((size_t)set * a.nsrc + s) * box + b;
-    if (keep[s]) *o = cadd(*o, acc[s]);
+  part[grp][0][lb] = acc0;
+  part[grp][1][lb] = acc1;
+  part[grp][2][lb] = acc2;
+  part[grp][3][lb] = acc3;
+  __syncthreads();
+  // threads (s, lb) of the first 4 x kCollectBins sum their column over the groups
+  if (threadIdx.x < 4 * kCollectBins) {
+    const int s = threadIdx.x / kCollectBins, l = threadIdx.x % kCollectBins;
+    const int bb = blockIdx.x * kCollectBins + l;
+    if (s < nsrc && bb < box) {
+      cpx<double> sum = {0, 0};
+      for (int g = 0; g < kCollectGroups; ++g) sum = cadd(sum, part[g][s][l]);
+      bool keep = true;
+      if (a.fmax && ((a.floor_src_mask >> s) & 1)) {
+        const size_t k = (size_t)set * a.fmax_stride + (a.fmax_stride == 1 ? 0 : s);
+        keep = kept(a.spec[((size_t)set * a.nsrc + s) * box + bb], a.floor_level * bits_to_d(a.fmax[k]));
+      }
+      cpx<double>* o = out + ((size_t)set * a.nsrc + s) * box + bb;
+      if (keep) *o = cadd(*o, sum);
+    }
   }
 }
 
@@ -1729,7 +1755,7 @@ ColInvArgs col_idft(const cpx<double>* spec, int nsrc, const glw_box& box, const
 
 int collect(const ColInvArgs& fwd, const cpx<double>* x, int nset, cpx<double>* out, hipStream_t st) {
   const int n = box_size(glw_box{fwd.mx0, fwd.my0, fwd.wx, fwd.wy});
-  hipLaunchKernelGGL(k_adj_collect, dim3((n + NT - 1) / NT, nset), dim3(NT), 0, st, fwd, x, out);
+  hipLaunchKernelGGL(k_adj_collect, dim3((n + kCollectBins - 1) / kCollectBins, nset), dim3(NT), 0, st, fwd, x, out);
   GLW_HIP(hipGetLastError());
   return GLW_OK;
 }
