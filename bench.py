@@ -245,6 +245,8 @@ def bench_inverse(args):
     s = seastate.inverse_256()
     m = GLElevationModel(s.kx, s.ky, s.a, s.b, lx=s.lx, ly=s.ly, nx=s.nx, ny=s.ny, depth=s.depth,
                          gravity=s.gravity, order=3)
+    m_graph = GLElevationModel(s.kx, s.ky, s.a, s.b, lx=s.lx, ly=s.ly, nx=s.nx, ny=s.ny, depth=s.depth,
+                               gravity=s.gravity, order=3, graphs=True)
     at = torch.tensor(s.a, device=m.device)
     bt = torch.tensor(s.b, device=m.device)
     target = m.elevation(at, bt)
@@ -268,6 +270,18 @@ def bench_inverse(args):
         step()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    # the same step with the forward and adjoint replayed as HIP graphs, for comparison
+    model_eager = m
+    m = m_graph
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0g = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    graph_rate = args.steps / (time.perf_counter() - t0g)
+    m = model_eager
     # the 200-step Adam fit (timed as a whole)
     a.data.copy_((at * (1 + 0.2 * torch.tensor(np.random.default_rng(14).standard_normal(s.count),
                                                device=m.device))).float())
@@ -322,6 +336,7 @@ def bench_inverse(args):
         "dtype": "f64", "data": "synthetic seeded JONSWAP sea state (numpy default_rng seed 4), hidden amplitudes",
         "config": {"workload": f"256x256, h=40 m, Hs=5 m, Tp=11 s, L=1700 m, {s.count} parents; amplitudes float32, "
                                "forward and adjoint FP64"},
+        "graph_gradients_per_s": graph_rate,
         "adam_200_steps": {"seconds": adam_s, "misfit_first": first, "misfit_final": final,
                            "amplitude_rel_error_final": err, "amplitude_rel_error_initial": 0.2},
         "cpu_baseline": cpu,

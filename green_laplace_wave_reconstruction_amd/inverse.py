@@ -4,7 +4,10 @@
 ``eta = Re(eta11+ + eta22+ + eta33+)`` of MF12-style amplitudes ``(a, b)`` on
 fixed modes (src/gl_spectral_surface.m:1-102 at one instant) and its exact
 gradient through the hand-written adjoint (``glw_adjoint_execute``), and
-``gl_elevation`` wraps both in a ``torch.autograd.Function``.
+``gl_elevation`` wraps both in a ``torch.autograd.Function``.  With ``graphs=True``
+the forward (phase, eta11, eta22, eta33) and the adjoint are captured once
+into two HIP graphs (``torch.cuda.CUDAGraph`` on the capture stream, which the
+library launches on) and replayed: the same kernels, one launch each.
 
 The plan (parent support, pair/triple supports, GL nodes, balance shifts,
 peak wavenumber) is built once from the initial amplitudes and held fixed:
@@ -32,7 +35,8 @@ class GLElevationModel:
     """Fixed-plan map (a, b) -> eta on an ``[ny, nx]`` grid, with its adjoint on the GPU."""
 
     def __init__(self, kx, ky, a0, b0, *, lx: float, ly: float, nx: int, ny: int, depth: float,
-                 gravity: float = 9.81, order: int = 3, time: float = 0.0, peak_q: float | None = None):
+                 gravity: float = 9.81, order: int = 3, time: float = 0.0, peak_q: float | None = None,
+                 graphs: bool = False):
         import torch
 
         if order not in (1, 2, 3):
@@ -64,6 +68,59 @@ class GLElevationModel:
         self.generation = 0
         shape = (1, ny, nx)
         self._out = [torch.empty(shape, dtype=torch.complex128, device=self.device) for _ in range(2)]
+        self.graphs = bool(graphs)
+        self._cap = None  # captured forward / adjoint graphs and their static buffers
+
+    # ---- HIP-graph mode: the forward (phase, eta11, eta22, eta33) and the adjoint as two graph launches
+    def _capture(self) -> None:
+        """Capture the forward and the adjoint (recompute=False) once, after one eager run of each."""
+        import torch
+
+        n = self.kx.size
+        a_in = torch.zeros(n, dtype=torch.float64, device=self.device)
+        b_in = torch.zeros(n, dtype=torch.float64, device=self.device)
+        g_in = torch.zeros((self.ny, self.nx), dtype=torch.float64, device=self.device)
+
+        def forward():
+            self._load(a_in, b_in)
+            return self._evaluate()
+
+        def backward():
+            return self.gradient(g_in, recompute=False)
+
+        forward()  # sizes the plan scratch eagerly (nothing may grow under capture)
+        backward()
+        torch.cuda.synchronize()
+        fwd_graph, bwd_graph = torch.cuda.CUDAGraph(), torch.cuda.CUDAGraph()
+        with torch.cuda.graph(fwd_graph):
+            eta = forward()
+        with torch.cuda.graph(bwd_graph):
+            ga, gb = backward()
+        self._cap = {"a": a_in, "b": b_in, "g": g_in, "fwd": fwd_graph, "bwd": bwd_graph, "eta": eta, "ga": ga,
+                     "gb": gb}
+
+    def _elevation_graph(self, a, b):
+        cap = self._cap
+        cap["a"].copy_(a.detach().to(self.device, torch_float64()))
+        cap["b"].copy_(b.detach().to(self.device, torch_float64()))
+        cap["fwd"].replay()
+        self.generation += 1
+        return cap["eta"].clone()
+
+    def _gradient_graph(self, grad_eta):
+        cap = self._cap
+        cap["g"].copy_(grad_eta.to(torch_float64()))
+        cap["bwd"].replay()
+        return cap["ga"].clone(), cap["gb"].clone()
+
+    def _evaluate(self):
+        eta = None
+        for kind, n in (("linear", 1), ("order2", 2), ("order3", 3)):
+            if n > self.order:
+                break
+            e, _ = self.plan.run(kind, self.times, out=self._out)
+            eta = e[0].real.clone() if eta is None else eta + e[0].real
+        return eta
 
     def _load(self, a, b) -> None:
         import torch
@@ -76,14 +133,12 @@ class GLElevationModel:
 
     def elevation(self, a, b):
         """Re(eta11 + eta22 + eta33) (up to the model order), float64 [ny, nx] device tensor."""
+        if self.graphs:
+            if self._cap is None:
+                self._capture()
+            return self._elevation_graph(a, b)
         self._load(a, b)
-        eta = None
-        for kind, n in (("linear", 1), ("order2", 2), ("order3", 3)):
-            if n > self.order:
-                break
-            e, _ = self.plan.run(kind, self.times, out=self._out)
-            eta = e[0].real.clone() if eta is None else eta + e[0].real
-        return eta
+        return self._evaluate()
 
     def component(self, a, b, n: int):
         """Re(eta_nn+) alone (n = 1, 2, 3)."""
@@ -98,6 +153,12 @@ class GLElevationModel:
                                recompute=recompute).reshape(-1)
         gm = g0[self.index] * self.scale
         return gm.real.contiguous(), gm.imag.contiguous()
+
+
+def torch_float64():
+    import torch
+
+    return torch.float64
 
 
 def _function():
@@ -120,7 +181,10 @@ def _function():
             fresh = model.generation == ctx.generation
             if not fresh:  # another evaluation ran since: restore this one's spectrum
                 model._load(a, b)
-            ga, gb = model.gradient(grad_eta.to(torch.float64), recompute=not fresh)
+            if model.graphs and fresh:
+                ga, gb = model._gradient_graph(grad_eta)
+            else:
+                ga, gb = model.gradient(grad_eta.to(torch.float64), recompute=not fresh)
             return ga.to(ctx.dtypes[0]), gb.to(ctx.dtypes[1]), None
 
     return GLElevation

@@ -168,3 +168,33 @@ def test_adam_recovers_hidden_amplitudes(torch):
                            / (at**2 + bt**2).sum()))
     print(f"Adam 200 steps: misfit {first:.3e} -> {final:.3e}, amplitude error {err:.3e}", flush=True)
     assert final < 1e-4 * first and err < err0 / 10
+
+
+def test_graph_mode_replays_match_eager(torch):
+    """HIP-graph forward and adjoint: every replay (with eager work of another model in between) equals
+    the eager evaluation of the same amplitudes."""
+    from green_laplace_wave_reconstruction_amd.inverse import GLElevationModel, gl_elevation
+
+    s = small_state()
+    kw = dict(lx=s.lx, ly=s.ly, nx=s.nx, ny=s.ny, depth=s.depth, gravity=s.gravity, order=3)
+    eager = GLElevationModel(s.kx, s.ky, s.a, s.b, **kw)
+    graph = GLElevationModel(s.kx, s.ky, s.a, s.b, graphs=True, **kw)
+    other = _model(small_state())
+    rng = np.random.default_rng(8)
+    target = eager.elevation(torch.tensor(s.a, device=eager.device), torch.tensor(s.b, device=eager.device))
+    for rep in range(4):
+        scale = 1 + 0.1 * rng.standard_normal(s.count)
+        res = []
+        for m in (eager, graph):
+            a = torch.tensor(s.a * scale, device=m.device, requires_grad=True)
+            b = torch.tensor(s.b / scale, device=m.device, requires_grad=True)
+            loss = 0.5 * ((gl_elevation(a, b, m) - target) ** 2).sum()
+            loss.backward()
+            res.append((float(loss), a.grad.cpu().numpy(), b.grad.cpu().numpy()))
+        # eager work on another plan between replays
+        other.elevation(torch.tensor(s.a, device=other.device), torch.tensor(s.b, device=other.device))
+        torch.cuda.synchronize()
+        (l0, ga0, gb0), (l1, ga1, gb1) = res
+        assert l1 == pytest.approx(l0, rel=1e-13), rep
+        assert rel(ga1, ga0) < 1e-13 and rel(gb1, gb0) < 1e-13, rep
+    assert graph._cap is not None
