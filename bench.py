@@ -453,8 +453,9 @@ def bench_wit(args):
     inp = formats.read_dir4in(ROOT / "tests" / "golden" / f"wit_{name}_input.bin")
     plan = wit.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
                        dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"], precision=args.precision)
+    out = None
     for _ in range(args.warmup):
-        out, cnt = plan.sector44_device(inp["amplitudes"])
+        out, cnt = plan.sector44_device(inp["amplitudes"], out)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -498,8 +499,25 @@ def bench_wit(args):
         "data": f"reference frozen input paper/order4/inputs/runtime/{name} (copied as a fixture)",
         "config": {"workload": f"WIT sector 44, M={inp['modes'].shape[0]}, {inp['nx']}x{inp['ny']}",
                    "quartets": int(c[0])},
+        "roofline": _wit_roofline(args.precision, c[1] * args.steps / wall),
         "cpu_baseline": cpu,
     }
+
+
+def _wit_roofline(precision: str, quartets_per_s: float):
+    """Vector-ALU roofline of the quartet kernel: FLOP per quartet measured with rocprofv3 (F64/F32
+    add, mul and FMA wave instructions x 64 lanes, profiles/wit_m400_valu_counters.md) times the live
+    quartet rate, against the 78.6 TFLOP/s FP64 (= non-packed FP32) vector peak."""
+    p = ROOT / "profiles" / "wit_flops.json"
+    if not p.exists():
+        return None
+    rec = json.loads(p.read_text()).get("fp64" if precision == "fp64" else "fp32 eval")
+    if not rec:
+        return None
+    achieved = rec["flop_per_quartet"] * quartets_per_s / 1e12
+    return {"bound": "fp64 vector ALU", "kernel": "k_wit_quartets", "achieved": achieved, "peak": 78.6,
+            "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None,
+            "flop_per_quartet": rec["flop_per_quartet"]}
 
 
 def main():
