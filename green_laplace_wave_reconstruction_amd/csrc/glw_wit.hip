@@ -45,6 +45,9 @@ int wfail(int code, const std::string& m) { return glw_report_error(code, m); }
   } while (0)
 
 constexpr int WNT = 256;
+// Largest per-workgroup LDS accumulator (eta and psi over the pair-sum window
+// of the modes); inputs whose window is larger accumulate with global atomics.
+constexpr size_t kWitLdsBytes = 128 * 1024;
 
 // Subset record: state of one subset (multiset of parents) in arithmetic T.
 template <typename T>
@@ -384,6 +387,10 @@ struct QuartetArgs {
   double* out;              // [5][cases][ny][nx] complex
   unsigned long long* cnt;  // kept, near, zero_out, zero_int
   unsigned long long* dmax; // bits: max residual, max zero source, min detuning
+  // Workgroup accumulator in LDS (0 bins: off).  The quartets of one leading
+  // pair (i, j) land on mode(i) + mode(j) + the pair sums of the other two,
+  // all inside the pw x ph box starting at (pxmin, pymin) = 2 min(mode).
+  int lds_bins, pw, ph, pxmin, pymin;
 };
 
 __device__ __forceinline__ void atomic_addc(double* p, cpx<double> v) {
@@ -424,6 +431,14 @@ __global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
   unsigned long long kept = 0, nearc = 0, zo = 0, zi = 0;
   double maxres = 0.0, maxz = 0.0, mindet = INFINITY;
   const size_t plane = (size_t)w.nx * w.ny * g.cases;
+  // LDS accumulator [bin][case][eta, psi] (complex), cleared before use
+  extern __shared__ double wacc[];
+  const int nacc = g.lds_bins * g.cases * 4;
+  const int sx = w.modes[2 * i] + w.modes[2 * j], sy = w.modes[2 * i + 1] + w.modes[2 * j + 1];
+  if (g.lds_bins) {
+    for (int e = threadIdx.x; e < nacc; e += WNT) wacc[e] = 0.0;
+    __syncthreads();
+  }
   for (long long f = threadIdx.x; f < tri; f += WNT) {
     // unrank f -> (k', l'), 0 <= k' <= l' < n: row k' starts at k' n - k'(k'-1)/2
     double disc = (2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * (double)f;
@@ -504,8 +519,18 @@ __global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
       cpx<double> wt = cm(cm(cm(cs(ac[i], 2.0), ac[j]), ac[k]), ac[l]);
       wt = cs(wt, 1.0 / (double)fac);
       const size_t loc = (size_t)c * w.nx * w.ny + ix + (size_t)w.nx * iy;
-      atomic_addc(g.out + 2 * loc, cm(wt, eta));
-      atomic_addc(g.out + 2 * (plane + loc), cm(wt, psi));
+      if (g.lds_bins) {
+        const int lb = (ty - sy - g.pymin) * g.pw + (tx - sx - g.pxmin);
+        double* a4 = wacc + ((size_t)lb * g.cases + c) * 4;
+        const cpx<double> ve = cm(wt, eta), vp = cm(wt, psi);
+        atomicAdd(a4 + 0, ve.x);
+        atomicAdd(a4 + 1, ve.y);
+        atomicAdd(a4 + 2, vp.x);
+        atomicAdd(a4 + 3, vp.y);
+      } else {
+        atomic_addc(g.out + 2 * loc, cm(wt, eta));
+        atomic_addc(g.out + 2 * (plane + loc), cm(wt, psi));
+      }
       if (o.near) atomic_addc(g.out + 2 * (4 * plane + loc), cm(wt, ret));
     }
     ++kept;
@@ -514,6 +539,19 @@ __global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
     maxres = fmax(maxres, res);
     mindet = fmin(mindet, (double)o.rho);
     maxz = fmax(maxz, zsrc);
+  }
+  if (g.lds_bins) {  // flush: one global add per touched bin
+    __syncthreads();
+    for (int e = threadIdx.x; e < g.lds_bins * g.cases; e += WNT) {
+      const double* a4 = wacc + (size_t)e * 4;
+      if (a4[0] == 0.0 && a4[1] == 0.0 && a4[2] == 0.0 && a4[3] == 0.0) continue;
+      const int lb = e / g.cases, c = e - lb * g.cases;
+      const int tx = sx + g.pxmin + lb % g.pw, ty = sy + g.pymin + lb / g.pw;
+      const size_t ix = (size_t)(((tx % w.nx) + w.nx) % w.nx), iy = (size_t)(((ty % w.ny) + w.ny) % w.ny);
+      const size_t loc = (size_t)c * w.nx * w.ny + ix + (size_t)w.nx * iy;
+      atomic_addc(g.out + 2 * loc, cpx<double>{a4[0], a4[1]});
+      atomic_addc(g.out + 2 * (plane + loc), cpx<double>{a4[2], a4[3]});
+    }
   }
   kept = wsum_u64(kept);
   nearc = wsum_u64(nearc);
@@ -557,8 +595,9 @@ __global__ void k_wit_counters(const unsigned long long* cnt, const unsigned lon
 }  // namespace
 
 struct glw_wit_s {
-  glw_wit_desc d{};
+  glw_wit_desc d{};  // d.modes is the caller's array: read during create only
   int device = 0;
+  int xmin = 0, xmax = 0, ymin = 0, ymax = 0;  // mode box (LDS accumulator window)
   WitDev w{};
   std::vector<void*> allocs;
   unsigned long long* cnt = nullptr;
@@ -585,6 +624,14 @@ int glw_wit_create(const glw_wit_desc* d, glw_wit** out) {
   W->d = *d;
   int rc = GLW_OK;
   const int M = d->modes_count;
+  W->xmin = W->xmax = d->modes[0];
+  W->ymin = W->ymax = d->modes[1];
+  for (int m = 1; m < M; ++m) {
+    W->xmin = std::min(W->xmin, (int)d->modes[2 * m]);
+    W->xmax = std::max(W->xmax, (int)d->modes[2 * m]);
+    W->ymin = std::min(W->ymin, (int)d->modes[2 * m + 1]);
+    W->ymax = std::max(W->ymax, (int)d->modes[2 * m + 1]);
+  }
   if (hipGetDevice(&W->device) != hipSuccess) {
     delete W;
     return wfail(GLW_EDEVICE, "no HIP device");
@@ -680,10 +727,23 @@ int glw_wit_sector44(glw_wit* W, const double* amps, int32_t cases, double* out,
   g.cnt = W->cnt;
   g.dmax = W->dmax;
   const long long M = W->d.modes_count;
-  if (W->d.precision == GLW_PREC_FP32)
-    hipLaunchKernelGGL(k_wit_quartets<float>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), 0, st, g);
-  else
-    hipLaunchKernelGGL(k_wit_quartets<double>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), 0, st, g);
+  // LDS accumulator window: pair sums of the modes, (2 W - 1) x (2 H - 1) bins
+  g.pw = 2 * (W->xmax - W->xmin) + 1;
+  g.ph = 2 * (W->ymax - W->ymin) + 1;
+  g.pxmin = 2 * W->xmin;
+  g.pymin = 2 * W->ymin;
+  const size_t lds = (size_t)g.pw * g.ph * cases * 4 * sizeof(double);
+  g.lds_bins = lds <= kWitLdsBytes ? g.pw * g.ph : 0;
+  const size_t shm = g.lds_bins ? lds : 0;
+  if (W->d.precision == GLW_PREC_FP32) {
+    if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_quartets<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+    hipLaunchKernelGGL(k_wit_quartets<float>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), shm, st, g);
+  } else {
+    if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_quartets<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+    hipLaunchKernelGGL(k_wit_quartets<double>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), shm, st, g);
+  }
   const double total = (double)M * (M + 1) * (M + 2) * (M + 3) / 24.0;
   hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, total, counters);
   W_HIP(hipGetLastError());
