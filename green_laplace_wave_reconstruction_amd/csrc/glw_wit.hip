@@ -48,6 +48,7 @@ constexpr int WNT = 256;
 // Largest per-workgroup LDS accumulator (eta and psi over the pair-sum window
 // of the modes); inputs whose window is larger accumulate with global atomics.
 constexpr size_t kWitLdsBytes = 128 * 1024;
+constexpr size_t kLdsPerCu = 160 * 1024;
 
 // Subset record: state of one subset (multiset of parents) in arithmetic T.
 template <typename T>
@@ -416,8 +417,11 @@ __device__ __forceinline__ double wmin_d(double v) {
 // FP64; the FP32 path reads FP32 copies).  Weights and accumulation are FP64
 // in both: every FP32 term is exact in FP64 and the FP64 sums bound the
 // summation error far below a compensated FP32 sum.
-template <typename T>
-__global__ __launch_bounds__(WNT) void k_wit_quartets(QuartetArgs g) {
+// OCC: minimum waves per SIMD asked of the register allocator (2 fits the FP32
+// evaluation with a small spill; used when the LDS window leaves room for two
+// workgroups per CU).
+template <typename T, int OCC = 1>
+__global__ __launch_bounds__(WNT, OCC) void k_wit_quartets(QuartetArgs g) {
   const WitDev& w = g.w;
   const long long p = blockIdx.x;
   int lo = 0, hi = w.M - 1;
@@ -735,7 +739,11 @@ int glw_wit_sector44(glw_wit* W, const double* amps, int32_t cases, double* out,
   const size_t lds = (size_t)g.pw * g.ph * cases * 4 * sizeof(double);
   g.lds_bins = lds <= kWitLdsBytes ? g.pw * g.ph : 0;
   const size_t shm = g.lds_bins ? lds : 0;
-  if (W->d.precision == GLW_PREC_FP32) {
+  if (W->d.precision == GLW_PREC_FP32 && shm <= kLdsPerCu / 2) {  // two workgroups per CU fit: occupancy 2
+    if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_quartets<float, 2>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL((k_wit_quartets<float, 2>), dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), shm, st, g);
+  } else if (W->d.precision == GLW_PREC_FP32) {
     if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_quartets<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)shm));
     hipLaunchKernelGGL(k_wit_quartets<float>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), shm, st, g);
