@@ -161,7 +161,7 @@ def test_adam_recovers_hidden_amplitudes(torch):
         loss.backward()
         opt.step()
         sched.step()
-        first = first if first is not None else float(loss)
+        first = first if first is not None else float(loss.detach())
     final = float(0.5 * ((m.elevation(a.detach().double(), b.detach().double()) - target) ** 2).sum())
     err0 = 0.2
     err = float(torch.sqrt(((a.detach().double() - at) ** 2 + (b.detach().double() - bt) ** 2).sum()
