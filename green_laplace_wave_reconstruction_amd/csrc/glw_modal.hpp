@@ -28,7 +28,8 @@
 //   k_modal_t<MODE>  one workgroup per RB x rows: builds each first-order
 //                    field's row from the deposit (two nodes per frequency,
 //                    exp(2 pi i m j / Q) from the Q-point table), or loads a
-//                    pair field's row, transforms it over t in LDS, folds it
+//                    pair field's row, transforms it over t in LDS (two
+//                    fields per transform on power-of-two T), folds it
 //                    into the pointwise products held in registers, and
 //                    finally inverse-transforms the products and stores the
 //                    kept frequencies.
