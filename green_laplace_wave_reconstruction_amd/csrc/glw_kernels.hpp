@@ -66,6 +66,7 @@ enum MulCode : int {
   M_I_QYA = 33,     // i qy A
   M_Q2A = 34,       // q^2 A
 };
+constexpr int kNumMulCodes = 35;
 
 struct BoxTables {
   const double* qx;   // [wy][wx]
@@ -74,6 +75,10 @@ struct BoxTables {
   const double* inu;  // 1 / max(nu, tiny)
   const double* q2;
   const double* a;    // q tanh q
+  // [kNumMulCodes][wy*wx]: every multiplier code evaluated on the box
+  // (apply_mul(code, 1)), so the column loads read one value instead of
+  // branching on the code per element
+  const cpx<double>* mul;
 };
 
 __device__ __forceinline__ cpx<double> apply_mul(int code, cpx<double> s, const BoxTables& t, int b) {
@@ -154,18 +159,20 @@ struct ColInvLoad {
   static constexpr bool SEQ_FAST = true;  // neighbouring columns are contiguous
   const cpx<double>* spec;
   const double* damp;
-  BoxTables tab;
+  const cpx<double>* mul;  // this field's row of the multiplier table
   double level, scale;
-  int code, col0, wx, wy, my0, gy;
+  int col0, wx, wy, my0, gy;
   __device__ __forceinline__ cpx<T> operator()(int c, int e) const {
     const int col = col0 + c;
     const int r = wrap(e - my0, gy);
     if (col >= wx || r >= wy) return {T(0), T(0)};
     const int b = r * wx + col;
     cpx<double> s = spec[b];
+    const cpx<double> m = mul[b];
+    const double d = damp ? damp[b] : 1.0;
     if (level >= 0.0 && hypot(s.x, s.y) < level) s = {0.0, 0.0};
-    if (damp) s = {s.x * damp[b], s.y * damp[b]};
-    s = apply_mul(code, s, tab, b);
+    if (damp) s = {s.x * d, s.y * d};
+    s = {m.x * s.x - m.y * s.y, m.x * s.y + m.y * s.x};  // the product apply_mul forms
     return {T(s.x * scale), T(s.y * scale)};
   }
 };
@@ -197,14 +204,13 @@ __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __re
   ColInvLoad<T> ld;
   ld.spec = a.spec + ((size_t)set * a.nsrc + src) * box;
   ld.damp = a.damp ? a.damp + (size_t)((task % a.dmod) * a.dmul + a.dadd) * box : nullptr;
-  ld.tab = a.tab;
   ld.level = -1.0;
   if (a.fmax && ((a.floor_src_mask >> src) & 1)) {
     const size_t k = (size_t)set * a.fmax_stride + (a.fmax_stride == 1 ? 0 : src);
     ld.level = a.floor_level * __longlong_as_double((long long)a.fmax[k]);
   }
   ld.scale = a.fscale[f];
-  ld.code = a.fcode[f];
+  ld.mul = a.tab.mul + (size_t)a.fcode[f] * box;
   ld.col0 = blockIdx.x * CW;
   ld.wx = a.wx;
   ld.wy = a.wy;

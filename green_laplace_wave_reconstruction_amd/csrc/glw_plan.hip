@@ -491,7 +491,7 @@ __global__ __launch_bounds__(NT) void k_adj_collect(ColInvArgs a, const cpx<doub
       const int u = it / a.nfield, f = it - u * a.nfield;
       const int task = set * a.sdiv + u;
       const double dmp = a.damp ? a.damp[(size_t)((task % a.dmod) * a.dmul + a.dadd) * box + b] : 1.0;
-      const cpx<double> m = apply_mul(a.fcode[f], cpx<double>{1.0, 0.0}, a.tab, b);
+      const cpx<double> m = a.tab.mul[(size_t)a.fcode[f] * box + b];
       const cpx<double> v = x[((size_t)task * a.nfield + f) * box + b];
       const cpx<double> w = conj_mul(m, v);
       const double sc = dmp * a.fscale[f];
@@ -551,8 +551,16 @@ __global__ void k_fill_zero_c(cpx<double>* p, size_t n) {
 // ----------------------------------------------------------------------------
 struct DevTables {
   double *qx = nullptr, *qy = nullptr, *nu = nullptr, *inu = nullptr, *q2 = nullptr, *a = nullptr;
-  BoxTables view() const { return BoxTables{qx, qy, nu, inu, q2, a}; }
+  cpx<double>* mul = nullptr;
+  BoxTables view() const { return BoxTables{qx, qy, nu, inu, q2, a, mul}; }
 };
+
+// mul[code][b] = apply_mul(code, 1) on every bin of a box.
+__global__ void k_mul_table(BoxTables t, int n, cpx<double>* mul) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int code = blockIdx.y;
+  if (b < n) mul[(size_t)code * n + b] = apply_mul(code, cpx<double>{1.0, 0.0}, t, b);
+}
 
 int box_size(const glw_box& b) { return b.wx * b.wy; }
 
@@ -670,7 +678,16 @@ struct glw_plan_s {
     if ((rc = upload(&dt.nu, h.nu, n))) return rc;
     if ((rc = upload(&dt.inu, h.inu, n))) return rc;
     if ((rc = upload(&dt.q2, h.q2, n))) return rc;
-    return upload(&dt.a, h.a, n);
+    if ((rc = upload(&dt.a, h.a, n))) return rc;
+    void* p = nullptr;
+    GLW_HIP(hipMalloc(&p, n * kNumMulCodes * sizeof(cpx<double>)));
+    allocs.push_back(p);
+    dt.mul = (cpx<double>*)p;
+    hipLaunchKernelGGL(k_mul_table, dim3((unsigned)((n + 255) / 256), kNumMulCodes), dim3(256), 0, 0, dt.view(),
+                       (int)n, dt.mul);
+    GLW_HIP(hipGetLastError());
+    GLW_HIP(hipDeviceSynchronize());
+    return GLW_OK;
   }
 
   int twiddles(int n) {
