@@ -180,6 +180,25 @@ def _evaluate(host: _plan.HostPlan, s0_box: np.ndarray, times: np.ndarray, order
     return comps
 
 
+def _all_finite(x) -> bool:
+    if isinstance(x, np.ndarray):
+        return bool(np.all(np.isfinite(x)))
+    import torch
+
+    return bool(torch.isfinite(x).all())
+
+
+def _check_order3_finite(comps: dict) -> None:
+    """Non-finite order-3 outputs are an error (src/internal/gl_no_stokes_eta33.m:223-228).
+
+    The reference raises these without an identifier, so ``identifier`` is empty.
+    """
+    if "eta33_plus" in comps and not _all_finite(comps["eta33_plus"]):
+        raise GreenLaplaceError("", "Nested Green--Laplace eta33 output is not finite.")
+    if "psi33_plus" in comps and not _all_finite(comps["psi33_plus"]):
+        raise GreenLaplaceError("", "Direct Green--Laplace Psi33 output is not finite.")
+
+
 def _finish(comps: dict, order: int, single: bool) -> dict:
     keys = ["eta11", "psi11"] + (["eta22", "psi22"] if order >= 2 else []) + (["eta33", "psi33"] if order >= 3 else [])
     out: dict = {}
@@ -232,6 +251,7 @@ def gl_spectral_surface(coeffs: dict, Lx: float, Ly: float, Nx: int, Ny: int, t,
                             precision=_precision(precision))
     host.omega = host.box1.take(omega)
     comps = _evaluate(host, host.box1.take(spec), times, order, return_device=return_device)
+    _check_order3_finite(comps)
     out = _finish(comps, order, single)
     X, Y = np.meshgrid(np.arange(Nx) * Lx / Nx, np.arange(Ny) * Ly / Ny)
     included = ["eta11", "psi11"] + (["eta22", "psi22"] if order >= 2 else []) + (["eta33", "psi33"] if order >= 3 else [])

@@ -134,6 +134,25 @@ def test_matlab_api_option_errors():
     assert e.value.identifier == "green_laplace:Coefficients"
 
 
+def test_matlab_api_nonfinite_order3_output(monkeypatch):
+    """Non-finite eta33/psi33 raise (src/internal/gl_no_stokes_eta33.m:223-228); host logic only:
+    the device evaluation is replaced by fields with one NaN."""
+    c = api.gl_spectral_coefficients(3, 1.0, 1.0, [.01, .006], [0, -.002], [2, 3], [0, 1], 0, 0)
+
+    def fake(host, s0, times, order, *, return_device, max_instants=16):
+        z = np.zeros((len(times), 32, 32), complex)
+        out = {k: z.copy() for k in ("eta11_plus", "psi11_plus", "eta22_plus", "psi22_plus",
+                                     "eta33_plus", "psi33_plus")}
+        out[bad][0, 3, 4] = np.nan
+        return out
+
+    monkeypatch.setattr(api, "_evaluate", fake)
+    for bad, msg in (("eta33_plus", "eta33 output is not finite"), ("psi33_plus", "Psi33 output is not finite")):
+        with pytest.raises(GreenLaplaceError, match=msg) as e:
+            api.gl_spectral_surface(c, 2 * np.pi, 2 * np.pi, 32, 32, 0.0)
+        assert e.value.identifier == ""
+
+
 def test_coefficients_pack_like_reference():
     c = api.gl_spectral_coefficients(3, 9.81, 2.0, [.01, .02], [0, .01], [2, 3], [0, 1], 0, 0)
     assert c["api"] == "green-laplace-spectral-v1"
