@@ -30,7 +30,8 @@ bool t_length_ok(int n) {
     default: return false;
   }
 }
-bool q_length_ok(int n) {
+// Modal lengths one workgroup transforms in LDS.
+bool q_length_direct(int n) {
   switch (n) {
 #define GLW_CASE(L) case L:
     GLW_MODAL_Q_LENGTHS(GLW_CASE)
@@ -38,6 +39,23 @@ bool q_length_ok(int n) {
 #undef GLW_CASE
     default: return false;
   }
+}
+bool q_length_ok(int n) {
+  switch (n) {
+#define GLW_CASE(L) case L:
+    GLW_MODAL_Q_SPLIT_LENGTHS(GLW_CASE)
+    return true;
+#undef GLW_CASE
+    default: return q_length_direct(n);
+  }
+}
+// Sub-transforms of the modal pass at length n: 1 for an LDS length, else
+// R0 = 2, 4 or 8 with n / R0 = 8192 or 6144 (0: none).
+int q_split(int n) {
+  if (q_length_direct(n)) return 1;
+  for (int r0 = 2; r0 <= 8; r0 *= 2)
+    if (n == 8192 * r0 || n == 6144 * r0) return r0;
+  return 0;
 }
 
 // tw[m] = exp(-2 pi i m / n), m < n
@@ -164,7 +182,7 @@ __global__ __launch_bounds__(256) void k_eta20_synth(const C* coef, int span, in
 size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 struct Layout {
-  size_t tw_t, tw_q, tw_n, src, pair, forcing, coef2, coef3, energy2, energy3, total;
+  size_t tw_t, tw_q, tw_n, src, pair, forcing, coef2, coef3, energy2, energy3, tw_qs, partial, total;
 };
 
 Layout modal_layout(const glw_modal_desc& d) {
@@ -187,6 +205,11 @@ Layout modal_layout(const glw_modal_desc& d) {
   L.coef3 = d.order == 3 ? take(nw3 * cs) : 0;
   L.energy2 = take(2 * nw2 * sizeof(double));
   L.energy3 = d.order == 3 ? take(2 * nw3 * sizeof(double)) : 0;
+  const int r0 = q_split(d.q_len);
+  if (r0 > 1) {
+    L.tw_qs = take((size_t)(d.q_len / r0) * cs);
+    L.partial = take((d.order == 3 ? nw3 : nw2) * (size_t)r0 * 4 * sizeof(double));
+  }
   L.total = off;
   return L;
 }
@@ -215,6 +238,49 @@ int twiddles(C* tw, int n, hipStream_t st) {
   return GLW_OK;
 }
 
+// Sums of the R0 sub-transforms of each column, in a fixed order.
+__global__ __launch_bounds__(256) void k_modal_qsum(const double* partial, int r0, int ncol, C* coef, double* energy) {
+  const int w = blockIdx.x * 256 + threadIdx.x;
+  if (w >= ncol) return;
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int r = 0; r < r0; ++r)
+    for (int k = 0; k < 4; ++k) s[k] += partial[((size_t)w * r0 + r) * 4 + k];
+  coef[w] = {s[0], s[1]};
+  energy[2 * w] = s[2];
+  energy[2 * w + 1] = s[3];
+}
+
+// Forward half of a split modal transform on the two columns of `x` ([2][Q][ld]).
+int split_pre(C* x, int q_len, int ld, int ncol, const C* tw_q, hipStream_t st) {
+  const int r0 = q_split(q_len);
+  const dim3 grid((ncol + 63) / 64, q_len / r0, 2);
+  if (r0 == 2) hipLaunchKernelGGL(glw::modal::k_modal_pre<2>, grid, dim3(64), 0, st, x, q_len, ld, ncol, tw_q);
+  else if (r0 == 4) hipLaunchKernelGGL(glw::modal::k_modal_pre<4>, grid, dim3(64), 0, st, x, q_len, ld, ncol, tw_q);
+  else if (r0 == 8) hipLaunchKernelGGL(glw::modal::k_modal_pre<8>, grid, dim3(64), 0, st, x, q_len, ld, ncol, tw_q);
+  else return mfail(GLW_EUNSUPPORTED, "split modal length " + std::to_string(q_len));
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+// Inverse half on the five pair fields ([5][ncol][Q]).
+int split_post(C* pair, int q_len, int ncol, const C* tw_q, hipStream_t st) {
+  const int r0 = q_split(q_len);
+  const dim3 grid((q_len / r0 + 255) / 256, ncol, glw::modal::kPairFields);
+  if (r0 == 2) hipLaunchKernelGGL(glw::modal::k_modal_post<2>, grid, dim3(256), 0, st, pair, q_len, tw_q);
+  else if (r0 == 4) hipLaunchKernelGGL(glw::modal::k_modal_post<4>, grid, dim3(256), 0, st, pair, q_len, tw_q);
+  else if (r0 == 8) hipLaunchKernelGGL(glw::modal::k_modal_post<8>, grid, dim3(256), 0, st, pair, q_len, tw_q);
+  else return mfail(GLW_EUNSUPPORTED, "split modal length " + std::to_string(q_len));
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
+int split_sum(const Args& a, int ncol, hipStream_t st) {
+  hipLaunchKernelGGL(k_modal_qsum, dim3((ncol + 255) / 256), dim3(256), 0, st, a.partial,
+                     q_split(a.q_len), ncol, a.coef, a.energy);
+  GLW_HIP_CHECK(hipGetLastError());
+  return GLW_OK;
+}
+
 int launched(int rc, const char* what) {
   if (rc == 1) return mfail(GLW_EUNSUPPORTED, std::string(what) + ": unsupported length or launch shape");
   if (rc != 0) return mfail(GLW_EDEVICE, std::string(what) + ": launch failed");
@@ -225,7 +291,9 @@ int launched(int rc, const char* what) {
 
 extern "C" {
 
-int glw_modal_length_supported(int32_t n) { return (t_length_ok(n) ? 1 : 0) | (q_length_ok(n) ? 2 : 0); }
+int glw_modal_length_supported(int32_t n) {
+  return (t_length_ok(n) ? 1 : 0) | (q_length_ok(n) ? 2 : 0) | (q_length_direct(n) ? 4 : 0);
+}
 
 int glw_modal_workspace_size(const glw_modal_desc* d, size_t* bytes) {
   if (!bytes) return mfail(GLW_EINVAL, "null argument");
@@ -266,6 +334,9 @@ static int modal_run(const glw_modal_desc* d, const double* deposit, void* works
   a.dep = (const Dep*)deposit;
   a.tw_t = (const C*)(ws + L.tw_t);
   a.tw_q = (const C*)(ws + L.tw_q);
+  const int r0 = q_split(d->q_len);
+  a.tw_qs = r0 > 1 ? (const C*)(ws + L.tw_qs) : a.tw_q;
+  a.partial = r0 > 1 ? (double*)(ws + L.partial) : nullptr;
   a.src = (C*)(ws + L.src);
   a.pair = d->order == 3 ? (C*)(ws + L.pair) : nullptr;
   a.forcing = d->order == 3 ? (C*)(ws + L.forcing) : nullptr;
@@ -281,6 +352,8 @@ static int modal_run(const glw_modal_desc* d, const double* deposit, void* works
   if (int rc = twiddles((C*)(ws + L.tw_t), d->t_len, st)) return rc;
   if (int rc = twiddles((C*)(ws + L.tw_q), d->q_len, st)) return rc;
   if (int rc = twiddles((C*)(ws + L.tw_n), d->count, st)) return rc;
+  if (r0 > 1)
+    if (int rc = twiddles((C*)(ws + L.tw_qs), d->q_len / r0, st)) return rc;
   if (int rc = mark(1)) return rc;
   const int nw2 = 2 * d->span + 1, nw3 = 3 * d->span + 1;
   // order-2 sources, eta22 coefficients (+ pair fields)
@@ -288,14 +361,25 @@ static int modal_run(const glw_modal_desc* d, const double* deposit, void* works
   if (int rc = mark(2)) return rc;
   a.coef = (C*)(ws + L.coef2);
   a.energy = (double*)(ws + L.energy2);
-  if (int rc = launched(glw::modal::launch_q_order2(a, nw2, st), "modal pass (eta22)")) return rc;
+  if (r0 > 1)
+    if (int rc = split_pre(a.src, d->q_len, glw::modal::out_ld(nw2), nw2, a.tw_q, st)) return rc;
+  if (int rc = launched(glw::modal::launch_q_order2(a, nw2, r0, st), "modal pass (eta22)")) return rc;
+  if (r0 > 1) {
+    if (int rc = split_sum(a, nw2, st)) return rc;
+    if (d->order == 3)
+      if (int rc = split_post(a.pair, d->q_len, nw2, a.tw_q, st)) return rc;
+  }
   if (int rc = mark(3)) return rc;
   if (d->order == 3) {
     if (int rc = launched(glw::modal::launch_t_order3(a, st), "modal time pass (order 3)")) return rc;
     if (int rc = mark(4)) return rc;
     a.coef = (C*)(ws + L.coef3);
     a.energy = (double*)(ws + L.energy3);
-    if (int rc = launched(glw::modal::launch_q_order3(a, nw3, st), "modal pass (eta33)")) return rc;
+    if (r0 > 1)
+      if (int rc = split_pre(a.forcing, d->q_len, glw::modal::out_ld(nw3), nw3, a.tw_q, st)) return rc;
+    if (int rc = launched(glw::modal::launch_q_order3(a, nw3, r0, st), "modal pass (eta33)")) return rc;
+    if (r0 > 1)
+      if (int rc = split_sum(a, nw3, st)) return rc;
   } else {
     if (int rc = mark(4)) return rc;
   }
@@ -351,7 +435,7 @@ static int check_eta20(const glw_eta20_desc* d) {
     return mfail(GLW_EINVAL, "eta20 record or band is invalid");
   if (!t_length_ok(d->t_len) || d->t_len < 2 * d->span + 1)
     return mfail(GLW_EUNSUPPORTED, "eta20 t_len unsupported or below 2 span + 1");
-  if (!q_length_ok(d->q_len) || d->kspan < 1 || d->q_len < 2 * d->kspan + 1)
+  if (!q_length_direct(d->q_len) || d->kspan < 1 || d->q_len < 2 * d->kspan + 1)
     return mfail(GLW_EUNSUPPORTED, "eta20 q_len unsupported or below 2 kspan + 1");
   if (d->rank < 1 || d->rank > glw::modal::kMaxRank) return mfail(GLW_EINVAL, "GL rank must be 1..16");
   if (!(d->dq > 0) || !(d->fstep > 0) || !(d->depth > 0) || !(d->delta_q > 0))
@@ -408,6 +492,7 @@ int glw_eta20_execute(const glw_eta20_desc* d, const double* deposit, void* work
   a.dep = (const Dep*)deposit;
   a.tw_t = (const C*)(ws + L.tw_t);
   a.tw_q = (const C*)(ws + L.tw_q);
+  a.tw_qs = a.tw_q;
   a.src = (C*)(ws + L.src);
   a.coef = (C*)(ws + L.coef2);
   a.energy = (double*)(ws + L.energy2);

@@ -40,6 +40,11 @@
 //                    source-energy audit); for order 3 it also forms the
 //                    pair spectra e2, p2 and inverse-transforms the five
 //                    modal multiples the time pass needs.
+//   k_modal_pre/post modal axes longer than one workgroup's LDS (Q = R0 Q',
+//                    R0 = 2, 4, 8): radix-R0 fold of the forcing columns
+//                    before k_modal_q runs its R0 sub-transforms of Q'
+//                    points per column, and the matching unfold of the pair
+//                    fields after it.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -70,6 +75,10 @@ struct Args {
   const Dep* dep;        // [span + 1]
   const C* tw_t;         // exp(-2 pi i m / T), m < T
   const C* tw_q;         // exp(-2 pi i m / Q), m < Q
+  // Modal lengths beyond one workgroup's LDS run as R0 sub-transforms of
+  // length Q / R0 (k_modal_pre, k_modal_q over blockIdx.y, k_modal_post):
+  const C* tw_qs;        // exp(-2 pi i m / (Q / R0)), m < Q / R0 (tw_q when R0 = 1)
+  double* partial;       // [ncol][R0][4] per-sub-transform sums (R0 > 1)
   C* src;                // [2][Q][ldw2]  order-2 sources (d, k); eta20: forcing (d, k) at signed columns
   C* pair;               // [5][2 span + 1][Q]  pair fields in x
   C* forcing;            // [2][Q][ldw3]  order-3 forcing (d, k)
@@ -532,6 +541,11 @@ __device__ __forceinline__ void block_sum4(double (&v)[4], double* red) {
 //            to x as e2, e2 q, p2 q, p2 A, p2 q^2.
 //   MODE_O3: eta33 coefficient of bin 3 base + w (modal_time.py:316-327).
 //   MODE_E20: eta20 coefficient of signed bin w - span (_eta20.py:103-123).
+// Q here is the length of the transform in LDS.  With R0 = gridDim.y > 1 the
+// modal length is a.q_len = R0 Q: workgroup r = blockIdx.y transforms the
+// r-th sequence k_modal_pre left in rows r Q .. r Q + Q - 1 (the modes
+// m = R0 m' + r), writes its sums to a.partial, and leaves the inverse
+// transforms of its pair spectra for k_modal_post to combine.
 template <int Q, int MODE>
 __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   constexpr int NT = q_threads(Q);
@@ -540,9 +554,10 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   __shared__ C buf[SS];
   __shared__ double red[NT / 64 * 4];
   const int w = col_block(blockIdx.x, gridDim.x);
+  const int r0 = gridDim.y, rs = blockIdx.y;  // R0 sub-transforms; this one's index
   const int nw = MODE == MODE_O3 ? 3 * a.span + 1 : 2 * a.span + 1;
   const C* in = MODE == MODE_O3 ? a.forcing : a.src;
-  const double inv_q = 1.0 / Q;
+  const double inv_q = 1.0 / a.q_len;
   NoIO none;
   // forward modal FFT of the two forcing columns: d kept in registers, k left in LDS
   C fd[PQ];
@@ -550,8 +565,8 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   for (int o = 0; o < 2; ++o) {
     const double od = o == 0, ok = o == 1;  // arithmetic select of the output (see above)
     __syncthreads();
-    ColLoad ld{in + (size_t)o * Q * out_ld(nw) + w, out_ld(nw)};
-    fft_io<double, Q, 1, SS, NT, -1, Q >= 3072 ? 1 : 0>(buf, a.tw_q, ld, none);
+    ColLoad ld{in + ((size_t)o * a.q_len + (size_t)rs * Q) * out_ld(nw) + w, out_ld(nw)};
+    fft_io<double, Q, 1, SS, NT, -1, Q >= 3072 ? 1 : 0>(buf, a.tw_qs, ld, none);
     if (o == 0) {
 #pragma unroll
       for (int i = 0; i < PQ; ++i) {
@@ -569,17 +584,18 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   const bool pairs = MODE == MODE_O2 && a.order == 3;
 #pragma unroll
   for (int i = 0; i < PQ; ++i) {
-    const int m = threadIdx.x + i * NT;
-    if (m >= Q) continue;
-    const C sd = fd[i], sk = cmulr(buf[lds_pad(m)], inv_q);
+    const int ml = threadIdx.x + i * NT;  // element of this transform
+    if (ml >= Q) continue;
+    const int m = ml * r0 + rs;           // mode index on the a.q_len grid
+    const C sd = fd[i], sk = cmulr(buf[lds_pad(ml)], inv_q);
     if (pairs) {  // fd[i] becomes e2, the LDS element p2 (zero off the valid band)
       fd[i] = C{0.0, 0.0};
-      buf[lds_pad(m)] = C{0.0, 0.0};
+      buf[lds_pad(ml)] = C{0.0, 0.0};
     }
     int k;
     if constexpr (MODE == MODE_E20) {
       if (m <= a.kspan) k = m;
-      else if (m >= Q - a.kspan) k = m - Q;
+      else if (m >= a.q_len - a.kspan) k = m - a.q_len;
       else continue;
     } else {
       if (m > mmax) continue;
@@ -607,7 +623,7 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
           // e2 = (-A S sd + C sk) / 4, p2 = i (C sd - S sk) / 4
           fd[i] = {(-aq * Si * sd.x + Ci * sk.x) / 4.0, (-aq * Si * sd.y + Ci * sk.y) / 4.0};
           const C t = {Ci * sd.x - Si * sk.x, Ci * sd.y - Si * sk.y};
-          buf[lds_pad(m)] = {-t.y / 4.0, t.x / 4.0};
+          buf[lds_pad(ml)] = {-t.y / 4.0, t.x / 4.0};
         }
       } else {
         // (A S fd - i C fk) / h^2
@@ -634,40 +650,95 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   }
   block_sum4<NT>(acc, red);
   if (threadIdx.x == 0) {
-    a.coef[w] = {acc[0], acc[1]};
-    a.energy[2 * w] = acc[2];
-    a.energy[2 * w + 1] = acc[3];
+    if (r0 == 1) {
+      a.coef[w] = {acc[0], acc[1]};
+      a.energy[2 * w] = acc[2];
+      a.energy[2 * w + 1] = acc[3];
+    } else {
+      double* pt = a.partial + ((size_t)w * r0 + rs) * 4;
+      for (int k = 0; k < 4; ++k) pt[k] = acc[k];
+    }
   }
   if constexpr (MODE == MODE_O2) {
     if (!pairs) return;
     // pair fields in x (modal_time.py:285-292, _spectral_field = fft_t(ifft_q(.) P)):
     // e2, e2 q, p2 q, p2 A, p2 q^2, each inverse-transformed over the modal axis
-    const size_t fs = (size_t)nw * Q;
+    const size_t fs = (size_t)nw * a.q_len;
     C p2[PQ];
 #pragma unroll
     for (int i = 0; i < PQ; ++i) {
-      const int m = threadIdx.x + i * NT;
-      p2[i] = m < Q ? buf[lds_pad(m)] : C{0.0, 0.0};  // own elements (block_sum4 synchronised)
+      const int ml = threadIdx.x + i * NT;
+      p2[i] = ml < Q ? buf[lds_pad(ml)] : C{0.0, 0.0};  // own elements (block_sum4 synchronised)
     }
 #pragma unroll 1
     for (int f = 0; f < kPairFields; ++f) {
       __syncthreads();
 #pragma unroll
       for (int i = 0; i < PQ; ++i) {
-        const int m = threadIdx.x + i * NT;
-        if (m >= Q) continue;
-        const double q = (double)(2 * a.k0 + m) * a.dq;
+        const int ml = threadIdx.x + i * NT;
+        if (ml >= Q) continue;
+        const double q = (double)(2 * a.k0 + ml * r0 + rs) * a.dq;
         double mul = 1.0;
         if (f == 1 || f == 2) mul = q;
         else if (f == 3) mul = q * tanh(q);
         else if (f == 4) mul = q * q;
-        buf[lds_pad(m)] = cmulr(f == 0 || f == 1 ? fd[i] : p2[i], mul);
+        buf[lds_pad(ml)] = cmulr(f == 0 || f == 1 ? fd[i] : p2[i], mul);
       }
       __syncthreads();
-      ColStore st{a.pair + f * fs + (size_t)w * Q};
-      fft_io<double, Q, 1, SS, NT, +1, Q >= 3072 ? 1 : 0>(buf, a.tw_q, none, st);
+      ColStore st{a.pair + f * fs + (size_t)w * a.q_len + (size_t)rs * Q};
+      fft_io<double, Q, 1, SS, NT, +1, Q >= 3072 ? 1 : 0>(buf, a.tw_qs, none, st);
     }
   }
+}
+
+// Split modal transform, forward half (R0 sub-transforms of length QS = Q / R0):
+// in place on the two columns [2][Q][ld] of `x`, row n' + p QS becomes
+//   y_r[n'] = exp(-2 pi i n' r / Q) sum_p x[n' + p QS] exp(-2 pi i p r / R0)
+// at row r QS + n', so that the QS-point transform of y_r holds modes R0 m' + r.
+// Threads run along the columns (whole lines); each owns the R0 rows it reads.
+template <int R0>
+__global__ __launch_bounds__(64) void k_modal_pre(C* x, int q_len, int ld, int ncol, const C* tw_q) {
+  const int w = blockIdx.x * 64 + threadIdx.x;
+  const int np = blockIdx.y;  // n'
+  if (w >= ncol) return;
+  const int qs = q_len / R0;
+  C* col = x + (size_t)blockIdx.z * q_len * ld + w;
+  C v[R0], y[R0];
+#pragma unroll
+  for (int p = 0; p < R0; ++p) v[p] = col[(size_t)(np + p * qs) * ld];
+#pragma unroll
+  for (int r = 0; r < R0; ++r) {
+    C s = v[0];
+#pragma unroll
+    for (int p = 1; p < R0; ++p) s = cfma(s, v[p], tw_q[((p * r) % R0) * qs]);
+    y[r] = r == 0 ? s : glw::cmul(s, tw_q[np * r]);
+  }
+#pragma unroll
+  for (int r = 0; r < R0; ++r) col[(size_t)(r * qs + np) * ld] = y[r];
+}
+
+// Split modal transform, inverse half: in place on each pair column
+// [5][ncol][Q] (k_modal_q left z_r = the QS-point inverse transform of modes
+// R0 m' + r at r QS .. r QS + QS - 1):
+//   x[n' + p QS] = sum_r exp(2 pi i p r / R0) exp(2 pi i n' r / Q) z_r[n'].
+template <int R0>
+__global__ __launch_bounds__(256) void k_modal_post(C* pair, int q_len, const C* tw_q) {
+  const int qs = q_len / R0;
+  const int np = blockIdx.x * 256 + threadIdx.x;
+  if (np >= qs) return;
+  C* col = pair + ((size_t)blockIdx.z * gridDim.y + blockIdx.y) * q_len;
+  C z[R0], x[R0];
+#pragma unroll
+  for (int r = 0; r < R0; ++r) z[r] = r == 0 ? col[np] : glw::cmul(col[r * qs + np], conj_(tw_q[np * r]));
+#pragma unroll
+  for (int p = 0; p < R0; ++p) {
+    C s = z[0];
+#pragma unroll
+    for (int r = 1; r < R0; ++r) s = cfma(s, z[r], conj_(tw_q[((p * r) % R0) * qs]));
+    x[p] = s;
+  }
+#pragma unroll
+  for (int p = 0; p < R0; ++p) col[p * qs + np] = x[p];
 }
 
 }  // namespace modal

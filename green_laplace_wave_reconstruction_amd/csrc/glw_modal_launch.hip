@@ -21,8 +21,9 @@ int t_go(const Args& a, hipStream_t st) {
 }
 
 template <int Q, int MODE>
-int q_go(const Args& a, int ncol, hipStream_t st) {
-  hipLaunchKernelGGL((k_modal_q<Q, MODE>), dim3(ncol), dim3(q_threads(Q)), 0, st, a);
+int q_go(const Args& a, int ncol, int r0, hipStream_t st) {
+  if (a.q_len != Q * r0) return 1;
+  hipLaunchKernelGGL((k_modal_q<Q, MODE>), dim3(ncol, r0), dim3(q_threads(Q)), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
@@ -33,7 +34,7 @@ int q_go(const Args& a, int ncol, hipStream_t st) {
     return t_go<L, MODE>(a, st);
 #define GLW_Q_CASE(L) \
   case L:             \
-    return q_go<L, MODE>(a, ncol, st);
+    return q_go<L, MODE>(a, ncol, r0, st);
 
 #if GLW_MODAL_SET == 0 || GLW_MODAL_SET == 1
 template <int MODE>
@@ -46,8 +47,9 @@ static int t_switch(const Args& a, hipStream_t st) {
 #endif
 #if GLW_MODAL_SET >= 2
 template <int MODE>
-static int q_switch(const Args& a, int ncol, hipStream_t st) {
-  switch (a.q_len) {
+static int q_switch(const Args& a, int ncol, int r0, hipStream_t st) {
+  if (r0 < 1 || a.q_len % r0) return 1;
+  switch (a.q_len / r0) {
     GLW_MODAL_Q_LENGTHS(GLW_Q_CASE)
     default: return 1;
   }
@@ -60,10 +62,10 @@ int launch_t_eta20(const Args& a, hipStream_t st) { return t_switch<MODE_E20>(a,
 #elif GLW_MODAL_SET == 1
 int launch_t_order3(const Args& a, hipStream_t st) { return t_switch<MODE_O3>(a, st); }
 #elif GLW_MODAL_SET == 2
-int launch_q_order2(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_O2>(a, ncol, st); }
+int launch_q_order2(const Args& a, int ncol, int r0, hipStream_t st) { return q_switch<MODE_O2>(a, ncol, r0, st); }
 #elif GLW_MODAL_SET == 3
-int launch_q_order3(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_O3>(a, ncol, st); }
-int launch_q_eta20(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_E20>(a, ncol, st); }
+int launch_q_order3(const Args& a, int ncol, int r0, hipStream_t st) { return q_switch<MODE_O3>(a, ncol, r0, st); }
+int launch_q_eta20(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_E20>(a, ncol, 1, st); }
 #endif
 
 }  // namespace modal

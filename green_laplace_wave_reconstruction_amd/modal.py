@@ -74,11 +74,12 @@ _LENGTHS: dict[int, list[int]] = {}
 
 
 def grid_length(n: int, which: str) -> int:
-    """Smallest supported t (``which='t'``) or modal (``'q'``) grid length >= n."""
+    """Smallest supported t (``which='t'``), modal (``'q'``) or eta20 modal (``'q20'``: one
+    workgroup's LDS transform) grid length >= n."""
     lib = _bind(load_library())
-    bit = 1 if which == "t" else 2
+    bit = {"t": 1, "q": 2, "q20": 4}[which]
     if bit not in _LENGTHS:
-        _LENGTHS[bit] = [m for m in range(16, 8193, 16) if lib.glw_modal_length_supported(m) & bit]
+        _LENGTHS[bit] = [m for m in range(16, 65537, 16) if lib.glw_modal_length_supported(m) & bit]
     for m in _LENGTHS[bit]:
         if m >= n:
             return m
@@ -340,7 +341,7 @@ def eta20_plan(amplitudes, omega, q, count: int, dt: float, *, depth, gravity, r
     k0 = int(klo.min())
     kspan = int(klo.max()) + 1 - k0
     nodes, wts = _rule(rank)
-    desc = CEta20Desc(count=count, base_bin=base, span=span, q_len=grid_length(2 * kspan + 1, "q"),
+    desc = CEta20Desc(count=count, base_bin=base, span=span, q_len=grid_length(2 * kspan + 1, "q20"),
                       t_len=grid_length(2 * span + 1, "t"), k0=k0, kspan=kspan, rank=rank, dq=dqx,
                       fstep=2.0 * np.pi / (count * dt) * np.sqrt(depth / gravity), depth=depth, delta_q=delta_q,
                       nodes=nodes, weights=wts)

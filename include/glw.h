@@ -265,7 +265,10 @@ int glw_fft_length_supported(int32_t n);
  * axis (modal_time.py:344-368) and picks the reduced grids:
  *   q_len >= order * kspan + 1  (x points of the band-shifted modal axis),
  *   t_len >= order * span + 1   (t points of the band-shifted frequency axis),
- * lengths that glw_modal_length_supported accepts (bit 1: t_len, bit 2: q_len).
+ * lengths that glw_modal_length_supported accepts (bit 1: t_len, bit 2: q_len,
+ * bit 4: a q_len transformed in one workgroup's LDS, up to 8192; the longer
+ * ones, 12288 .. 65536, run as 2, 4 or 8 sub-transforms with a global
+ * pre- and post-pass).
  * Input bins are base_bin ..
  * base_bin + span (work index w); deposit[w] = {Re A, Im A, w_lo, w_hi, k_lo}
  * (device, FP64, k_hi = k_lo + 1, k0 = min k_lo, kspan = max k_lo + 1 - k0).
@@ -307,7 +310,8 @@ int glw_modal_stage_times(const glw_modal_desc* desc, const double* deposit, voi
  * elevation on the reference's nx x ny modal grid, evaluated on its qy = 0
  * plane (the unidirectional input deposits nothing else).  The deposit
  * brackets each q on the eta20 qx axis (nx points, _eta20.py:44,156-172);
- * q_len >= 2 kspan + 1, t_len >= 2 span + 1.  Output eta20 (device,
+ * q_len >= 2 kspan + 1 (an LDS length: bit 4 of glw_modal_length_supported),
+ * t_len >= 2 span + 1.  Output eta20 (device,
  * double[count]) after the Hermitian projection (_eta20.py:175-185).
  * delta_q is the GL scale (_eta20.py:90-93); the caller returns zeros for
  * the degenerate case. */

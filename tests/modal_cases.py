@@ -92,3 +92,11 @@ def broadband_record(count: int = 2048, seed: int = 11):
     from green_laplace_wave_reconstruction_amd import seastate
 
     return seastate.unidirectional_record(count, seed)
+
+
+def short_record(count: int = 128, seed: int = 5):
+    """A 128-sample k_p h = 1 record over 12 peak periods: its whole 3 x band fits the record and
+    the oracle's P-point grids stay cheap up to P = 131072 (modal lengths up to 65536)."""
+    from green_laplace_wave_reconstruction_amd import seastate
+
+    return seastate.unidirectional_record(count, seed, periods=12.0)

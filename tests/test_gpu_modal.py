@@ -5,7 +5,8 @@ eta20 8e-11, python/tests/test_temporal.py) and for the on-grid identity with
 the spatial graph (rtol 3e-14, python/tests/test_modal_time.py:57-63); 1e-12
 relative against the FP64 oracle on the same level (the device grids are the
 band-shifted halves of the reference's, so the rounding differs: measured
-~1e-14); degree-2/3 homogeneity at 1e-13 on the largest modal grid.
+~1e-14), also for modal lengths split into 2, 4 or 8 sub-transforms;
+degree-2/3 homogeneity at 1e-13 on the largest LDS modal grid.
 """
 
 import numpy as np
@@ -13,7 +14,7 @@ import pytest
 
 from cases import rel
 from modal_cases import (broadband_record, eta20_fixture_kwargs, off_grid_case, on_grid_case, order3_fixture_kwargs,
-                         unidirectional)
+                         short_record, unidirectional)
 from oracle import gl_oracle as O
 from oracle import modal_oracle as M
 
@@ -117,6 +118,32 @@ def test_bench_record_8192_vs_oracle(MD):
     assert r.audit["reduced_grid"] == {"q_len": 4096, "t_len": 1024}
     errs = {k: rel(r.components[k], o.components[k]) for k in ("eta11", "eta22_analytic", "eta33_analytic", "eta20")}
     print("bench record 8192", errs, flush=True)
+    assert max(errs.values()) < 1e-12
+    for key in ("eta22_invalid_source_energy_fraction", "eta33_invalid_source_energy_fraction"):
+        assert r.audit[key] == pytest.approx(o.audit[key], rel=1e-6, abs=1e-18)
+
+
+@pytest.mark.parametrize("points,limit,q_len", [
+    (32768, None, 16384),    # 2 x 8192
+    (32768, 4.4, 12288),     # 2 x 6144
+    (65536, None, 32768),    # 4 x 8192
+    (131072, 4.4, 49152),    # 8 x 6144
+    (131072, None, 65536),   # 8 x 8192
+])
+def test_split_modal_lengths_vs_oracle(MD, points, limit, q_len):
+    """Modal lengths past one workgroup's LDS: R0 sub-transforms with the global pre/post passes,
+    against the oracle on the reference's full P-point grids (a short record keeps the oracle to
+    seconds; `limit` scales the modal axis, in units of the band's largest q, to reach the
+    6144-based lengths)."""
+    eta, t, h = short_record()
+    q = M.select_band(eta, t, depth=h, gravity=9.81, order=3, omega_max=None, energy_fraction=0.999).q
+    kw = dict(depth=h, quadrature_rank=8, order=3, modal_points=points, include_eta20=False,
+              modal_limit=None if limit is None else limit * float(np.max(q)))
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    assert r.audit["reduced_grid"]["q_len"] == q_len
+    o = M.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    errs = {k: rel(r.components[k], o.components[k]) for k in ("eta11", "eta22_analytic", "eta33_analytic")}
+    print("split", q_len, errs, flush=True)
     assert max(errs.values()) < 1e-12
     for key in ("eta22_invalid_source_energy_fraction", "eta33_invalid_source_energy_fraction"):
         assert r.audit[key] == pytest.approx(o.audit[key], rel=1e-6, abs=1e-18)

@@ -5,6 +5,8 @@ its reduced grids must hold every product without wrap (q_len >= order K + 1,
 t_len >= order span + 1), which the kernels rely on.
 """
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -43,7 +45,7 @@ def test_band_and_deposit_match_oracle():
     assert np.array_equal(re, joint)
 
 
-@pytest.mark.parametrize("points", [256, 1024, 4096, 16384])
+@pytest.mark.parametrize("points", [256, 1024, 4096, 16384, 65536])
 def test_reduced_grids_hold_every_product(points):
     _lib_or_skip()
     eta, t, h = broadband_record()
@@ -122,4 +124,27 @@ def test_degenerate_eta20_and_grid_limit():
     eta, tt, h = broadband_record()
     band = MD._select_band(eta, tt, depth=h, gravity=9.81, order=3, omega_max=None, energy_fraction=0.999)
     with pytest.raises(_native.GlwError, match="no supported modal q grid length"):
-        MD.modal_level_plan(band, depth=h, gravity=9.81, rank=8, order=3, points=65536, modal_limit=None)
+        MD.modal_level_plan(band, depth=h, gravity=9.81, rank=8, order=3, points=262144, modal_limit=None)
+
+
+def test_split_modal_lengths():
+    """Modal lengths past one workgroup's LDS (8192) run as 2, 4 or 8 sub-transforms: the
+    reference's default ladder (up to 65536 modal points) plans on supported grids."""
+    _lib_or_skip()
+    lib = _native.load_library()
+    for n in (16, 6144, 8192):
+        assert lib.glw_modal_length_supported(n) & 6 == 6
+    for n in (12288, 16384, 24576, 32768, 49152, 65536):
+        assert lib.glw_modal_length_supported(n) & 6 == 2  # modal pass only, not eta20
+    for n in (10240, 20480, 98304):
+        assert not lib.glw_modal_length_supported(n) & 2
+    assert MD.grid_length(8193, "q") == 12288 and MD.grid_length(8192, "q20") == 8192
+    with pytest.raises(_native.GlwError):
+        MD.grid_length(8193, "q20")
+    eta, tt, h = broadband_record()
+    band = MD._select_band(eta, tt, depth=h, gravity=9.81, order=3, omega_max=None, energy_fraction=0.999)
+    for points in (32768, 65536):
+        d = MD.modal_level_plan(band, depth=h, gravity=9.81, rank=8, order=3, points=points, modal_limit=None).desc
+        assert d.q_len > 8192 and d.q_len >= 3 * d.kspan + 1
+        size = ctypes.c_size_t()
+        assert lib.glw_modal_workspace_size(ctypes.byref(d), ctypes.byref(size)) == 0
