@@ -45,7 +45,7 @@ class COrder4Desc(ctypes.Structure):
                 ("mask4", _U8P), ("damp_quartic", _DP), ("damp_third", _DP), ("damp_pair3", _DP),
                 ("damp_pair4", _DP), ("pair_cosh", _DP), ("pair_sinhr", _DP), ("pair_coef", _DP),
                 ("third_cosh", _DP), ("third_sinhr", _DP), ("third_coef", _DP), ("quartic_cosh", _DP),
-                ("quartic_sinhr", _DP), ("quartic_coef", _DP)]
+                ("quartic_sinhr", _DP), ("quartic_coef", _DP), ("forcing_terms", c_int32)]
 
 
 class CPlanDesc(ctypes.Structure):
@@ -195,7 +195,7 @@ def describe(plan: HostPlan, max_instants: int) -> tuple[CPlanDesc, list]:
                            _u8p(plan.mask4, keep), *(_dp(o[k], keep) for k in (
                                "damp_quartic", "damp_third", "damp_pair3", "damp_pair4", "pair_cosh", "pair_sinhr",
                                "pair_coef", "third_cosh", "third_sinhr", "third_coef", "quartic_cosh",
-                               "quartic_sinhr", "quartic_coef")))
+                               "quartic_sinhr", "quartic_coef")), int(o.get("forcing_terms", 0)))
     return d, keep
 
 

@@ -337,10 +337,31 @@ def gl_pure_sum_order4(s, qx, qy, qp: float, rank: int = 4, *, precision: str = 
     C++ runtime (paper/order4/cpp/gl_eta44_fftw.cpp) computes eta44 only; so
     does this build: ``psi44`` is ``None``.
     """
+    eta, audit = _order4_field(s, qx, qy, qp, int(rank), precision)
+    audit["release_status"] = "experimental no-Stokes; eta44 only"
+    return eta, None, audit
+
+
+def _order4_field(s, qx, qy, qp, rank: int, precision: str, outer: int = -1, part: int = -2):
+    """eta44/h on the GPU; ``outer >= 0`` keeps one outer (quartic) GL node and ``part`` selects,
+    within it, the 2+2/2+1+1/1+1+1+1 partitions (-1) or the 3+1 partition of one third-order
+    node (>= 0), as the reference's split runs do (paper/order4/cpp/gl_eta44_fftw.cpp:694-716)."""
     import torch
 
     host = _plan.build_order4_plan(np.asarray(s), np.asarray(qx, float), np.asarray(qy, float), peak_q=float(qp),
-                                   rank=int(rank), precision=_precision(precision))
+                                   rank=rank, precision=_precision(precision))
+    o4 = host.o4
+    if outer >= 0:  # the other outer nodes weigh zero in the quartic sum
+        keep = np.zeros(rank)
+        keep[outer] = 1.0
+        o4["quartic_coef"] = o4["quartic_coef"] * keep
+    if part == -1:  # no 3+1 partition: zero third-order state
+        o4["third_coef"] = np.zeros_like(o4["third_coef"])
+    elif part >= 0:  # the 3+1 partition of third-order node `part` only
+        keep = np.zeros(rank)
+        keep[part] = 1.0
+        o4["third_coef"] = o4["third_coef"] * keep
+        o4["forcing_terms"] = 1
     dp = DevicePlan(host, host.box1.take(np.asarray(s, dtype=np.complex128)), max_instants=1)
     try:
         e, _ = dp.run("order4", np.array([0.0]))
@@ -349,23 +370,38 @@ def gl_pure_sum_order4(s, qx, qy, qp: float, rank: int = 4, *, precision: str = 
     finally:
         dp.close()
     audit = dict(host.audits["eta44"])
-    audit["release_status"] = "experimental no-Stokes; eta44 only"
-    return eta, None, audit
+    if outer >= 0:
+        audit["outer_index"], audit["part_index"] = outer, part
+    return eta, audit
 
 
-def run_glrti(input_path, output_path=None, rank: int = 6, *, precision: str = "fp64") -> dict:
-    """Drop-in for ``gl_eta44 INPUT OUTPUT RANK THREADS`` (paper/order4/cpp/gl_eta44_fftw.cpp:639-764).
+def run_glrti(input_path, output_path=None, rank: int = 6, *, outer: int | None = None, part: int | None = None,
+              precision: str = "fp64") -> dict:
+    """Drop-in for ``gl_eta44 INPUT OUTPUT RANK THREADS [OUTER4_INDEX [PART_INDEX]]``
+    (paper/order4/cpp/gl_eta44_fftw.cpp:639-764).
 
     Reads a GLRTI001 file (spectrum, dk, depth, peak kh, GL rules), evaluates
     the pure-sum GL eta44 on the GPU and optionally writes GLE44O01 (the
     dimensionless analytic eta44 on the n x n grid).  The GL rule of the
     requested rank must be present in the file, as the reference requires;
     this build regenerates it (Golub--Welsch) and checks it matches.
+    ``outer`` (one outer GL node; negative: all) and ``part`` (-1: the
+    partitions without the third-order state, p >= 0: the 3+1 partition of
+    third-order node p; needs one outer node) give the reference's split-run
+    partial fields, which ``formats.merge_gle44`` sums.
     """
     import time
 
     from . import formats
 
+    o = -1 if outer is None else int(outer)
+    pt = -2 if part is None else int(part)
+    if o >= int(rank):
+        raise ValueError("outer index")  # reference messages (gl_eta44_fftw.cpp:643-646)
+    if pt >= int(rank) or pt < -2:
+        raise ValueError("part index")
+    if part is not None and o < 0:
+        raise ValueError("split mode requires one outer4 index")
     inp = formats.read_glrti(input_path)
     if int(rank) not in inp["rules"]:
         raise ValueError("rank")  # reference message (gl_eta44_fftw.cpp:690)
@@ -378,8 +414,8 @@ def run_glrti(input_path, output_path=None, rank: int = 6, *, precision: str = "
     mx, my = np.meshgrid(mode, mode)
     h, dk = inp["depth"], inp["dk"]
     t0 = time.perf_counter()
-    eta, _, audit = gl_pure_sum_order4(inp["spectrum"], h * dk * mx, h * dk * my, inp["peak_kh"], int(rank),
-                                       precision=precision)
+    eta, audit = _order4_field(inp["spectrum"], h * dk * mx, h * dk * my, inp["peak_kh"], int(rank), precision,
+                               outer=o, part=pt if o >= 0 else -2)
     elapsed = time.perf_counter() - t0
     if output_path is not None:
         formats.write_gle44(output_path, eta, rank=int(rank), threads=1, elapsed=elapsed)

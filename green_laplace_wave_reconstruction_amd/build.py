@@ -11,7 +11,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
-ROW_SETS = 7  # operator sets of glw_launch_rows.hip (see GLW_ROW_SET there)
+ROW_SETS = 8  # operator sets of glw_launch_rows.hip (see GLW_ROW_SET there)
 # (source, extra defines, object name): one hipcc process per unit, run in parallel
 UNITS = [
     (CSRC / "glw_plan.hip", [], "glw_plan.o"),

@@ -25,6 +25,8 @@
 #define GLW_ROW_OPS(X) X(31, 4, Third44Op)
 #elif GLW_ROW_SET == 6
 #define GLW_ROW_OPS(X) X(45, 2, Quartic44Op)
+#elif GLW_ROW_SET == 7
+#define GLW_ROW_OPS(X) X(45, 2, Quartic31Op)
 #else
 #error "unknown GLW_ROW_SET"
 #endif

@@ -301,4 +301,22 @@ struct Quartic44Op {
   }
 };
 
+// The 3+1 partition of Quartic44Op alone (same inputs): the forcing of one
+// third-order node in the reference's split runs
+// (paper/order4/cpp/gl_eta44_fftw.cpp:704-717, PART_INDEX >= 0).
+struct Quartic31Op {
+  template <typename T>
+  __device__ __forceinline__ static void apply(const cpx<T>* f, cpx<T>* o) {
+    const cpx<T> e = f[0], ex = f[1], ey = f[2], px = f[3], py = f[4], pz = f[5];
+    const cpx<T> pzz = f[8], ptz = f[10];
+    const cpx<T> Te = f[37], Tex = f[38], Tey = f[39], Tpx = f[40], Tpy = f[41], Tpz = f[42], Tpzz = f[43];
+    const cpx<T> Tpzt = f[44];
+    const cpx<T> k31 = sub(sub(add(add(add(mul(px, Tex), mul(py, Tey)), mul(Tpx, ex)), mul(Tpy, ey)), mul(e, Tpzz)),
+                           mul(Te, pzz));
+    const cpx<T> d31 = add(add(add(add(mul(e, Tpzt), mul(Te, ptz)), mul(px, Tpx)), mul(py, Tpy)), mul(pz, Tpz));
+    o[0] = scl(k31, T(0.5));
+    o[1] = scl(d31, T(0.5));
+  }
+};
+
 }  // namespace glw

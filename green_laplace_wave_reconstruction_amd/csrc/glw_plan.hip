@@ -761,6 +761,7 @@ int validate(const glw_plan_desc* d) {
   if (d->with_order4) {
     const glw_order4_desc& o = d->o4;
     if (o.rank < 1 || o.rank > 12) return fail(GLW_EINVAL, "order-4 GL rank must be 1..12");
+    if (o.forcing_terms != 0 && o.forcing_terms != 1) return fail(GLW_EINVAL, "order-4 forcing_terms must be 0 or 1");
     if (!length_supported(d->g2x) || !length_supported(d->g2y) || !length_supported(d->g3x) ||
         !length_supported(d->g3y) || !length_supported(o.g4x) || !length_supported(o.g4y))
       return fail(GLW_EUNSUPPORTED, "order-4 working grid length not supported");
@@ -1584,9 +1585,14 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     r.omx0 = o.p4.mx0;
     r.owx = o.p4.wx;
     r.gy = o.g4y;
-    if ((rc = GLW_STAGE(P, "o4_quartic_row", st,
-                        (launch_row<T, 45, 2, Quartic44Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st)))))
+    if (o.forcing_terms == 1) {  // the 3+1 partition only (split runs)
+      if ((rc = GLW_STAGE(P, "o4_quartic_row", st,
+                          (launch_row<T, 45, 2, Quartic31Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st)))))
+        return rc;
+    } else if ((rc = GLW_STAGE(P, "o4_quartic_row", st,
+                               (launch_row<T, 45, 2, Quartic44Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st))))) {
       return rc;
+    }
     if ((rc = fill_zero(P->buf<uint64_t>("o4_qm"), (size_t)ntask * 2, st))) return rc;
     ColFwdArgs c;
     c.in = P->ws.at("o4_qu").first;

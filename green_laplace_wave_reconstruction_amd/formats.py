@@ -123,6 +123,26 @@ def read_gle44(path) -> dict:
     return {"n": n, "rank": rank, "threads": threads, "elapsed": elapsed, "field": f}
 
 
+def merge_gle44(output_path, input_paths) -> dict:
+    """Sum partial GLE44O01 fields (paper/order4/cpp/merge_eta44_outer.cpp): same n and rank
+    required; the header records the partial count as threads and the largest elapsed time."""
+    paths = list(input_paths)
+    if len(paths) < 2:
+        raise ValueError("usage OUTPUT INPUT1 INPUT2 ...")
+    total = None
+    for p in paths:
+        r = read_gle44(p)
+        if total is None:
+            n, rank, wall, total = r["n"], r["rank"], r["elapsed"], r["field"].copy()
+        else:
+            if r["n"] != n or r["rank"] != rank:
+                raise ValueError("incompatible partial outputs")
+            wall = max(wall, r["elapsed"])
+            total += r["field"]
+    write_gle44(output_path, total, rank=rank, threads=len(paths), elapsed=wall)
+    return {"n": n, "rank": rank, "partials": len(paths), "elapsed": wall, "field": total}
+
+
 def write_gle44(path, field, *, rank: int, threads: int = 1, elapsed: float = 0.0) -> None:
     f = np.asarray(field, dtype=np.complex128)
     n = f.shape[0]

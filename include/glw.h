@@ -97,6 +97,11 @@ typedef struct glw_order4_desc {
   const double *pair_cosh, *pair_sinhr, *pair_coef;          /* [J][p2], [J][p2], [J] */
   const double *third_cosh, *third_sinhr, *third_coef;       /* [J][p3], [J][p3], [J] */
   const double *quartic_cosh, *quartic_sinhr, *quartic_coef; /* [J][p4], [J][p4], [J] */
+  /* 0: every forcing partition (1+1+1+1, 2+1+1, 2+2, 3+1); 1: the 3+1 partition
+   * only.  With quartic_coef / third_coef zero outside one outer / third-order
+   * node this gives the partial fields of the reference's split runs
+   * (gl_eta44 ... OUTER4_INDEX [PART_INDEX], paper/order4/cpp/gl_eta44_fftw.cpp:639-716). */
+  int32_t forcing_terms;
 } glw_order4_desc;
 
 typedef struct glw_plan_desc {

@@ -63,6 +63,35 @@ def test_glrti_gle44_drop_in(glw, tmp_path):
     assert out["rank"] == 8 and abs(l2 - N0382_METRICS[8][0]) < 1e-8 * N0382_METRICS[8][0]
 
 
+def test_glrti_split_runs_sum_to_the_single_run(glw, tmp_path):
+    """The reference's split mode (gl_eta44 ... OUTER4_INDEX [PART_INDEX], merged by
+    merge_eta44_outer.cpp) on its 32^2 smoke input at GL6: the outer-node partials, and the
+    base + per-third-node partials of every outer node, sum to the single run (the reference
+    records 5.97e-16, paper/order4/validation/cpp_matlab_smoke.json)."""
+    from green_laplace_wave_reconstruction_amd import api, formats, plan as PL
+    from order4_cases import smoke32
+
+    s, qx, qy, _ = smoke32()
+    rank = 6
+    formats.write_glrti(tmp_path / "in.bin", s, dk=0.5, depth=1.0, peak_kh=1.0, rules={rank: PL.gauss_laguerre_rule(rank)})
+    full = api.run_glrti(tmp_path / "in.bin", None, rank)["field"]
+    outer = []
+    for o in range(rank):
+        outer.append(tmp_path / f"o{o}.bin")
+        api.run_glrti(tmp_path / "in.bin", outer[-1], rank, outer=o)
+    by_outer = formats.merge_gle44(tmp_path / "merged_outer.bin", outer)["field"]
+    parts = []
+    for o in range(rank):
+        for p in range(-1, rank):
+            parts.append(tmp_path / f"o{o}p{p}.bin")
+            api.run_glrti(tmp_path / "in.bin", parts[-1], rank, outer=o, part=p)
+    by_part = formats.merge_gle44(tmp_path / "merged_parts.bin", parts)["field"]
+    e1, e2 = rel(by_outer, full), rel(by_part, full)
+    print(f"split GL6: outer {e1:.2e}, outer x part {e2:.2e}", flush=True)
+    assert e1 < 1e-13 and e2 < 1e-13
+    assert rel(formats.read_gle44(tmp_path / "o0p-1.bin")["field"], full) > 1e-3  # a partial is not the whole
+
+
 @pytest.mark.parametrize("rank", [6, 8, 10])
 def test_eta44_and_wit_reproduce_reference_smoke(glw, rank):
     """GPU eta44 and GPU WIT reproduce the GL-vs-WIT metric of the reference's 32^2 three-mode smoke

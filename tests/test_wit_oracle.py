@@ -68,3 +68,37 @@ def test_glrti_gle44_roundtrip(tmp_path):
     formats.write_gle44(tmp_path / "o.bin", s, rank=8)
     o = formats.read_gle44(tmp_path / "o.bin")
     assert np.array_equal(o["field"], s) and o["rank"] == 8
+
+
+def test_gle44_merge_sums_partials(tmp_path):
+    """Partial GLE44O01 fields sum like paper/order4/cpp/merge_eta44_outer.cpp: same n and rank
+    required, partial count as threads, largest elapsed time."""
+    rng = np.random.default_rng(1)
+    parts = [rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8)) for _ in range(3)]
+    for k, f in enumerate(parts):
+        formats.write_gle44(tmp_path / f"p{k}.bin", f, rank=6, threads=1, elapsed=0.5 + k)
+    r = formats.merge_gle44(tmp_path / "m.bin", [tmp_path / f"p{k}.bin" for k in range(3)])
+    m = formats.read_gle44(tmp_path / "m.bin")
+    assert np.array_equal(m["field"], parts[0] + parts[1] + parts[2])
+    assert (m["rank"], m["threads"], m["elapsed"], r["partials"]) == (6, 3, 2.5, 3)
+    formats.write_gle44(tmp_path / "bad.bin", parts[0], rank=8)
+    with pytest.raises(ValueError, match="incompatible partial outputs"):
+        formats.merge_gle44(tmp_path / "m2.bin", [tmp_path / "p0.bin", tmp_path / "bad.bin"])
+    with pytest.raises(ValueError, match="usage"):
+        formats.merge_gle44(tmp_path / "m3.bin", [tmp_path / "p0.bin"])
+
+
+def test_glrti_split_arguments_like_reference(tmp_path):
+    """OUTER4_INDEX / PART_INDEX checks of gl_eta44_fftw.cpp:640-646, raised before the input is read."""
+    from green_laplace_wave_reconstruction_amd import api
+
+    missing = tmp_path / "none.bin"
+    with pytest.raises(ValueError, match="outer index"):
+        api.run_glrti(missing, None, 6, outer=6)
+    with pytest.raises(ValueError, match="part index"):
+        api.run_glrti(missing, None, 6, outer=0, part=6)
+    with pytest.raises(ValueError, match="part index"):
+        api.run_glrti(missing, None, 6, outer=0, part=-3)
+    with pytest.raises(ValueError, match="split mode requires one outer4 index"):
+        api.run_glrti(missing, None, 6, part=0)
+
