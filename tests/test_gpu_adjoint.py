@@ -190,7 +190,7 @@ def test_graph_mode_replays_match_eager(torch):
             b = torch.tensor(s.b / scale, device=m.device, requires_grad=True)
             loss = 0.5 * ((gl_elevation(a, b, m) - target) ** 2).sum()
             loss.backward()
-            res.append((float(loss), a.grad.cpu().numpy(), b.grad.cpu().numpy()))
+            res.append((float(loss.detach()), a.grad.cpu().numpy(), b.grad.cpu().numpy()))
         # eager work on another plan between replays
         other.elevation(torch.tensor(s.a, device=other.device), torch.tensor(s.b, device=other.device))
         torch.cuda.synchronize()
