@@ -56,16 +56,42 @@ def workload():
     return s, c, kxg, kyg, spec, omega
 
 
-def traffic_from_profiles(kernel: str, summary: str = "pmc_traffic.json"):
-    """Measured HBM bytes per dispatch of `kernel` from a committed PMC summary, or None."""
+def stored_schedule(summary: str = "pmc_traffic.json"):
+    """The FFT schedule a committed PMC summary was taken with, or None."""
+    p = ROOT / "profiles" / summary
+    try:
+        return json.loads(p.read_text()).get("_schedule") or None
+    except Exception:
+        return None
+
+
+def traffic_from_profiles(kernel: str, summary: str = "pmc_traffic.json", schedule: str | None = None):
+    """Measured HBM bytes per dispatch of `kernel` from a committed PMC summary, or None.
+
+    With `schedule` (stage=plan/tile;...), None unless the summary was taken with the same
+    plan/tile for that stage (another tile moves other bytes)."""
     p = ROOT / "profiles" / summary
     if not p.exists():
         return None
     try:
         data = json.loads(p.read_text())
+        if schedule is not None:
+            def entry(text):
+                return dict(e.split("=") for e in text.split(";") if "=" in e).get(kernel, "0/0")
+            if entry(data.get("_schedule", "")) != entry(schedule):
+                return None
         return data.get(kernel, {}).get("hbm_bytes_per_dispatch")
     except Exception:
         return None
+
+
+def measured_hbm(traffic, avg_launch_ms):
+    """Measured HBM rate of the dominant kernel (PMC bytes per dispatch / launch time) and its
+    fraction of the 8 TB/s peak, next to the algorithmic figure; Nones without a PMC summary."""
+    if not traffic or not avg_launch_ms:
+        return {"traffic_GBps": None, "traffic_frac": None}
+    gbps = traffic / (avg_launch_ms * 1e-3) / 1e9
+    return {"traffic_GBps": gbps, "traffic_frac": gbps / HBM_PEAK_GBS}
 
 
 def cpu_baseline(s, c, kxg, kyg, times_sample, gpu_fields=None):
@@ -430,7 +456,8 @@ def bench_modal(args):
         "stage_ms": mean_ms,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": traffic_from_profiles(dominant, "pmc_traffic_modal.json"),
-                     "alg_bytes_per_launch": alg[dominant], "avg_launch_ms": mean_ms[dominant]},
+                     "alg_bytes_per_launch": alg[dominant], "avg_launch_ms": mean_ms[dominant],
+                     **measured_hbm(traffic_from_profiles(dominant, "pmc_traffic_modal.json"), mean_ms[dominant])},
         "api_seconds": api,
         "reference_published": {"seconds_8192_points": [6.78, 6.91], "ladder_median_seconds": 12.07,
                                 "source": "python/docs/random_hos_modal_comparison.md (HOS records, other machine)"},
@@ -534,6 +561,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="order3: skip the FFT schedule autotuner")
+    ap.add_argument("--tune", action="store_true",
+                    help="order3: autotune the FFT schedule now instead of using the stored (profiled) one")
     ap.add_argument("--no-api", action="store_true", help="modal: skip the end-to-end API timings (profiling runs)")
     ap.add_argument("--chunk", type=int, default=INSTANTS,
                     help="order3: instants per plan evaluation (the 64 instants run in chunks of this size)")
@@ -591,11 +620,22 @@ def main():
     eta = torch.empty(shape, dtype=torch.complex128, device=dp.device)
     psi = torch.empty(shape, dtype=torch.complex128, device=dp.device)
     schedule = "default (plan 0, tile 0)"
+    stored = stored_schedule() if args.precision == "fp64" else None
     if args.schedule:  # a schedule from an earlier autotuned run (profiling runs replay it)
         dp.schedule = args.schedule
         schedule = dp.schedule
+        source = "given"
+    elif not args.no_tune and stored and not args.tune:
+        # the autotuned schedule the committed HBM-traffic profile was taken with (FFT "wisdom"):
+        # the measured traffic then belongs to the kernels timed here
+        dp.schedule = stored
+        schedule = dp.schedule
+        source = "stored (profiles/pmc_traffic.json, autotuned on MI355X)"
     elif not args.no_tune:  # setup, outside the timed region
         schedule = dp.autotune("order3", times, out=(eta, psi), reps=3)
+        source = "autotuned"
+    else:
+        source = "default"
     for _ in range(args.warmup):
         dp.run("order3", times, out=(eta, psi))
     torch.cuda.synchronize()
@@ -659,7 +699,8 @@ def main():
     launches_per_step = dom_launches / args.steps
     alg_bytes = sweeps * units[unit_kind] * sweep_bytes / launches_per_step
     achieved = alg_bytes / dom_avg_s / 1e9
-    traffic = traffic_from_profiles(dom)
+    # the PMC summary is of the FP64 path: no measured traffic for the FP32 one
+    traffic = traffic_from_profiles(dom, schedule=schedule) if args.precision == "fp64" else None
 
     sample_idx = np.linspace(0, INSTANTS - 1, max(1, args.cpu_sample)).round().astype(int)
     cpu = None
@@ -690,12 +731,13 @@ def main():
             "instants_per_step": INSTANTS, "instants_per_chunk": args.chunk, "grid": [s.ny, s.nx],
             "precision": args.precision,
             "working_grids": {"quadratic": list(host.g2[::-1]), "cubic": list(host.g3[::-1])},
-            "fft_schedule": schedule,
+            "fft_schedule": schedule, "fft_schedule_source": source,
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": dom_avg_s * 1e3,
+            **measured_hbm(traffic, dom_avg_s * 1e3),
         },
         "pipeline_alg_GBps": 716 * sweep_bytes * value / 1e9,
         "stages_ms_per_step": {k: v[0] / args.steps for k, v in stages.items()},
