@@ -2316,4 +2316,19 @@ const char* glw_last_error(void) { return g_err.c_str(); }
 
 const char* glw_version(void) { return "glw 0.1.0 gfx950"; }
 
+size_t glw_abi_sizeof(int32_t which) {
+  switch (which) {
+    case GLW_ABI_BOX: return sizeof(glw_box);
+    case GLW_ABI_BOX_TABLES: return sizeof(glw_box_tables);
+    case GLW_ABI_ORDER2_DESC: return sizeof(glw_order2_desc);
+    case GLW_ABI_ORDER4_DESC: return sizeof(glw_order4_desc);
+    case GLW_ABI_PLAN_DESC: return sizeof(glw_plan_desc);
+    case GLW_ABI_WIT_DESC: return sizeof(glw_wit_desc);
+    case GLW_ABI_BATCH_ITEM: return sizeof(glw_batch_item);
+    case GLW_ABI_MODAL_DESC: return sizeof(glw_modal_desc);
+    case GLW_ABI_ETA20_DESC: return sizeof(glw_eta20_desc);
+    default: return 0;
+  }
+}
+
 }  // extern "C"

@@ -333,6 +333,16 @@ int glw_eta20_execute(const glw_eta20_desc* desc, const double* deposit, void* w
 const char* glw_last_error(void);
 const char* glw_version(void);
 
+/* ABI self-check (no reference counterpart: the reference has no C ABI).
+ * sizeof the public descriptor struct `which` as this library was compiled,
+ * so a binding can compare its own layout before passing one (0: unknown). */
+enum {
+  GLW_ABI_BOX = 1, GLW_ABI_BOX_TABLES = 2, GLW_ABI_ORDER2_DESC = 3, GLW_ABI_ORDER4_DESC = 4,
+  GLW_ABI_PLAN_DESC = 5, GLW_ABI_WIT_DESC = 6, GLW_ABI_BATCH_ITEM = 7, GLW_ABI_MODAL_DESC = 8,
+  GLW_ABI_ETA20_DESC = 9
+};
+size_t glw_abi_sizeof(int32_t which);
+
 #ifdef __cplusplus
 }
 #endif

@@ -27,3 +27,26 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert lib.glw_version().decode().startswith("glw ")
     assert lib.glw_fft_length_supported(512) == 1 and lib.glw_fft_length_supported(100) == 0
+
+
+def test_ctypes_layouts_match_the_compiled_structs():
+    """Every ctypes mirror has the size of its C struct (glw_abi_sizeof): a field added on one
+    side only would shift every later field of a descriptor."""
+    import ctypes
+
+    from green_laplace_wave_reconstruction_amd import modal, wit
+
+    if not _native.LIB_PATH.exists():
+        from green_laplace_wave_reconstruction_amd import build
+
+        build.build()
+    lib = _native.load_library(_native.LIB_PATH)
+    lib.glw_abi_sizeof.argtypes = [ctypes.c_int32]
+    lib.glw_abi_sizeof.restype = ctypes.c_size_t
+    mirrors = {1: _native.CBox, 2: _native.CBoxTables, 3: _native.COrder2Desc, 4: _native.COrder4Desc,
+               5: _native.CPlanDesc, 6: wit.CWitDesc, 7: _native.CBatchItem, 8: modal.CModalDesc,
+               9: modal.CEta20Desc}
+    for which, cls in mirrors.items():
+        assert lib.glw_abi_sizeof(which) == ctypes.sizeof(cls), cls.__name__
+    assert lib.glw_abi_sizeof(0) == 0
+
