@@ -39,7 +39,8 @@ struct Exec {
 };
 
 #define GLW_LENGTHS(X) \
-  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768) X(1024)
+  X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(144) X(192) X(256) X(288) X(384) X(512) X(576) X(768) X(1024) \
+  X(1152) X(1536) X(2048)
 
 bool length_supported(int n);
 

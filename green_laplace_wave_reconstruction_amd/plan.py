@@ -30,7 +30,7 @@ TINY = np.finfo(float).tiny
 EPS = np.finfo(float).eps
 
 #: FFT lengths the HIP kernels are instantiated for (2^a 3^b).
-FFT_LENGTHS = (16, 24, 32, 48, 64, 96, 128, 144, 192, 256, 288, 384, 512, 576, 768, 1024)
+FFT_LENGTHS = (16, 24, 32, 48, 64, 96, 128, 144, 192, 256, 288, 384, 512, 576, 768, 1024, 1152, 1536, 2048)
 
 PREC_FP64 = 0
 PREC_FP32 = 1

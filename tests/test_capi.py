@@ -50,3 +50,16 @@ def test_ctypes_layouts_match_the_compiled_structs():
         assert lib.glw_abi_sizeof(which) == ctypes.sizeof(cls), cls.__name__
     assert lib.glw_abi_sizeof(0) == 0
 
+
+def test_host_fft_lengths_match_the_library():
+    """plan.FFT_LENGTHS (the host's choice of working grids) is exactly what the kernels instantiate."""
+    from green_laplace_wave_reconstruction_amd import plan as PL
+
+    if not _native.LIB_PATH.exists():
+        from green_laplace_wave_reconstruction_amd import build
+
+        build.build()
+    lib = _native.load_library(_native.LIB_PATH)
+    supported = [n for n in range(8, 4097) if lib.glw_fft_length_supported(n)]
+    assert supported == list(PL.FFT_LENGTHS)
+

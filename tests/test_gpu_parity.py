@@ -61,6 +61,38 @@ def test_matlab_api_order3_matches_oracle(glw):
     assert eta.shape == (64, 64) and X.shape == (64, 64) and audit["included_components"][-1] == "psi33"
 
 
+def _tall_state(target, seed=7, wide=False):
+    """Modes with |my| up to `target` and mx <= 40 (wide: mx up to `target`, |my| <= 40) on a
+    2 pi 100 square domain, depth 20 (kh >= 1)."""
+    rng = np.random.default_rng(seed)
+    r = 100.0
+    if wide:
+        mx = np.concatenate([[target, target - 7], rng.integers(1, target + 1, 10)])
+        my = np.concatenate([[3, -5], rng.integers(-40, 41, 10)])
+    else:
+        mx = np.concatenate([[3, 5], rng.integers(1, 41, 10)])
+        my = np.concatenate([[target, -target], rng.integers(-target, target + 1, 10)])
+    keep = np.hypot(mx, my) >= 5
+    mx, my = mx[keep], my[keep]
+    a, b = 0.004 * rng.standard_normal(mx.size), 0.004 * rng.standard_normal(mx.size)
+    return a, b, mx / r, my / r, 2 * np.pi * r
+
+
+@pytest.mark.parametrize("target,nx,ny", [(150, 256, 1152), (200, 256, 1536), (300, 256, 2048), (300, 2048, 256)])
+def test_tall_grids_vs_oracle(glw, target, nx, ny):
+    """Grids whose y synthesis and cubic (and, at 2048, quadratic) working grids need the
+    1152/1536/2048-point column transforms, and a 2048-point x synthesis (x working grids stay
+    within 1024: the strict-forward box is half as wide): every component against the oracle."""
+    a, b, kx, ky, L = _tall_state(target, wide=nx > ny)
+    c = glw.gl_spectral_coefficients(3, 9.81, 20.0, a, b, kx, ky, 0, 0)
+    print(f"grid {nx} x {ny}", flush=True)
+    _, _, _, _, comps, audit = glw.gl_spectral_surface(c, L, L, nx, ny, 0.0)
+    ref = O.gl_spectral_surface(3, 9.81, 20.0, a, b, kx, ky, L, L, nx, ny, 0.0)
+    errs = {k: rel(comps[k], ref["components"][k]) for k in ("eta11", "psi11", "eta22", "psi22", "eta33", "psi33")}
+    print(errs, flush=True)
+    assert max(errs.values()) < 1e-7
+
+
 def _headline_inputs(times):
     s = headline()
     n = s.nx
