@@ -149,6 +149,17 @@ def test_split_modal_lengths_vs_oracle(MD, points, limit, q_len):
         assert r.audit[key] == pytest.approx(o.audit[key], rel=1e-6, abs=1e-18)
 
 
+def test_split_modal_length_order2_vs_oracle(MD):
+    """Order 2 on a split modal axis (no pair fields: the fold and the partial sums only)."""
+    eta, t, h = short_record()
+    kw = dict(depth=h, quadrature_rank=8, order=2, modal_points=65536, include_eta20=True)
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    assert r.audit["reduced_grid"]["q_len"] > 8192
+    o = M.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    assert rel(r.components["eta22_analytic"], o.components["eta22_analytic"]) < 1e-12
+    assert rel(r.components["eta20"], o.components["eta20"]) < 1e-12
+
+
 def test_broadband_order2_vs_oracle(MD):
     """Order 2 (no pair fields, 2 span + 1 time grid) on the broadband record, GL rank 6."""
     eta, t, h = broadband_record(1536, seed=21)
