@@ -62,6 +62,7 @@ class CPlanDesc(ctypes.Structure):
 #: every symbol include/glw.h declares
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
            "glw_order3_execute", "glw_order4_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
+           "glw_wit_sector",
            "glw_adjoint_execute", "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_batch_create",
            "glw_batch_launch", "glw_batch_destroy", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
            "glw_fft_length_supported", "glw_modal_length_supported", "glw_modal_workspace_size",

@@ -24,6 +24,7 @@
 // degree-4 near-resonant branch (:393-411) retains a V3 term.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -575,6 +576,376 @@ __global__ __launch_bounds__(WNT, OCC) void k_wit_quartets(QuartetArgs g) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// Sectors 40 and 42: quartets with signed parents (directional_eta4_openmp.cpp
+// sector loops :603-688).  A subset's state now depends on its members' signs,
+// so nothing is tabulated: every candidate evaluates all 15 subset states
+// (reference evaluate, :157-453) in registers and private memory, including
+// the two terms that vanish in sector 44: the retained-V3 state of a cubic
+// subset whose charge is not +-3 (and its push on the quartic source,
+// :282-298) and the y31 coordinate of the quartic output (:300-336).
+// ----------------------------------------------------------------------------
+struct Sgn {
+  double g, h, thr;
+  __device__ double q(double kx, double ky) const {
+    const double r = hypot(kx, ky);
+    return r * tanh(h * r);
+  }
+  __device__ double om(double kx, double ky) const { return sqrt(g * q(kx, ky)); }
+  // vertical-derivative factor |k|^n -> q, k^2, k^2 q, k^4 (n = 0..4)
+  __device__ double zd(double kx, double ky, int n) const {
+    const double k2 = kx * kx + ky * ky;
+    switch (n) {
+      case 0: return 1.0;
+      case 1: return q(kx, ky);
+      case 2: return k2;
+      case 3: return k2 * q(kx, ky);
+      default: return k2 * k2;
+    }
+  }
+  // 2x2 forced linear response (:111-121)
+  __device__ void solve(cpx<double> s0, cpx<double> s1, double kx, double ky, double w, cpx<double>& eta,
+                        cpx<double>& psi) const {
+    const double qq = q(kx, ky), den = w * w - g * qq;
+    eta = {(-w * s0.y - qq * s1.x) / den, (w * s0.x - qq * s1.y) / den};
+    psi = {(g * s0.x - w * s1.y) / den, (g * s0.y + w * s1.x) / den};
+  }
+  __device__ double g1(double ax, double ay, double bx, double by) const {
+    return -q(ax + bx, ay + by) * q(bx, by) + (bx * bx + by * by) + (ax * bx + ay * by);
+  }
+  // second-order response of two physical states (:136-146)
+  __device__ void cross(double ax, double ay, double wa, cpx<double> ea, cpx<double> pa, double bx, double by,
+                        double wb, cpx<double> eb, cpx<double> pb, cpx<double>& eta, cpx<double>& psi) const {
+    eta = psi = {0.0, 0.0};
+    if (hypot(ax + bx, ay + by) == 0.0) return;
+    const double gab = g1(ax, ay, bx, by), gba = g1(bx, by, ax, ay);
+    const cpx<double> s0 = glw::cadd(cs(glw::cmul(ea, pb), gab), cs(glw::cmul(eb, pa), gba));
+    const cpx<double> s1 = cs(glw::cmul(pa, pb), ax * bx + ay * by + q(ax, ay) * q(bx, by));
+    solve(s0, s1, ax + bx, ay + by, wa + wb, eta, psi);
+  }
+  // normal-form map of a cubic mode (unit normal amplitude) and a parent (:122-135)
+  __device__ void normal(double ax, double ay, int sa, double bx, double by, int sb, cpx<double>& eta,
+                         cpx<double>& psi) const {
+    const double wa = om(ax, ay), wb = om(bx, by);
+    const double ma = sqrt(wa / (2.0 * g)), na = sqrt(g / (2.0 * wa));
+    const cpx<double> ea = {ma, 0.0}, pa = {0.0, -sa * na};
+    const cpx<double> eb = {0.5, 0.0}, pb = {0.0, -sb * wb / (2.0 * q(bx, by))};
+    const double gab = g1(ax, ay, bx, by), gba = g1(bx, by, ax, ay);
+    const cpx<double> s0 = glw::cadd(cs(glw::cmul(ea, pb), gab), cs(glw::cmul(eb, pa), gba));
+    const cpx<double> s1 = cs(glw::cmul(pa, pb), ax * bx + ay * by + q(ax, ay) * q(bx, by));
+    solve(s0, s1, ax + bx, ay + by, sa * wa + sb * wb, eta, psi);
+  }
+};
+
+struct SignedOut {
+  cpx<double> eta, psi, ecoord, pcoord, retained;
+  double rho, residual, zsrc;
+  int zcount;
+  bool near;
+};
+
+__device__ __forceinline__ cpx<double> mul_i(double s, cpx<double> a) { return {-s * a.y, s * a.x}; }  // (i s) a
+
+// State of the four-parent set for parents `pk` (physical wavevectors) with signs `sg`.
+__device__ SignedOut evaluate_signed(const Sgn& m, const double (&pkx)[4], const double (&pky)[4], const int (&sg)[4]) {
+  const double inv_fact[4] = {1.0, 1.0, 0.5, 1.0 / 6.0};
+  double kx[16], ky[16], w[16];
+  int deg[16], chg[16];
+  double skx[4], sky[4], sw[4];
+  for (int s = 0; s < 4; ++s) {
+    skx[s] = sg[s] * pkx[s];
+    sky[s] = sg[s] * pky[s];
+    sw[s] = sg[s] * m.om(pkx[s], pky[s]);
+  }
+  for (int mask = 0; mask < 16; ++mask) {
+    kx[mask] = ky[mask] = w[mask] = 0.0;
+    deg[mask] = chg[mask] = 0;
+    for (int s = 0; s < 4; ++s)
+      if (mask >> s & 1) {
+        kx[mask] += skx[s];
+        ky[mask] += sky[s];
+        w[mask] += sw[s];
+        deg[mask] += 1;
+        chg[mask] += sg[s];
+      }
+  }
+  cpx<double> eta[16], phi[16], psi[16], ret[16], shx[16], shy[16], sv[16];
+  for (int mask = 0; mask < 16; ++mask) eta[mask] = phi[mask] = psi[mask] = ret[mask] = shx[mask] = shy[mask] = sv[mask] = {0.0, 0.0};
+  SignedOut o{};
+  cpx<double> eff0 = {0.0, 0.0}, eff1 = {0.0, 0.0};
+  // Partitions of `mask` into nb labelled non-empty blocks: the members are
+  // assigned base-nb digits of `code`; block 0 carries the potential, blocks
+  // 1.. the elevations (product `ep`).  f(block0, order, ep) is applied to each.
+  auto partitions = [&](int mask, int nb, auto&& f) {
+    int slots[4], n = 0;
+    for (int s = 0; s < 4; ++s)
+      if (mask >> s & 1) slots[n++] = s;
+    int count = 1;
+    for (int i = 0; i < n; ++i) count *= nb;
+    for (int code = 0; code < count; ++code) {
+      int blocks[4] = {0, 0, 0, 0}, seen = 0, c = code;
+      for (int i = 0; i < n; ++i) {
+        const int a = c % nb;
+        c /= nb;
+        seen |= 1 << a;
+        blocks[a] |= 1 << slots[i];
+      }
+      if (seen != (1 << nb) - 1) continue;
+      cpx<double> ep = {1.0, 0.0};
+      for (int b = 1; b < nb; ++b) ep = glw::cmul(ep, eta[blocks[b]]);
+      f(blocks, ep);
+    }
+  };
+  for (int d = 1; d <= 4; ++d) {
+    for (int mask = 1; mask < 16; ++mask) {
+      if (deg[mask] != d) continue;
+      cpx<double> taylor = {0.0, 0.0};
+      if (d == 1) {
+        eta[mask] = {0.5, 0.0};
+        phi[mask] = {0.0, -w[mask] / (2.0 * m.q(kx[mask], ky[mask]))};
+        psi[mask] = phi[mask];
+      } else {
+        // ordinary two-block source (:264-282) and the Taylor terms of orders 1..d-1
+        cpx<double> hor = {0.0, 0.0}, kin = {0.0, 0.0}, lv = {0.0, 0.0}, lt = {0.0, 0.0};
+        partitions(mask, 2, [&](const int* b, cpx<double>) {
+          const int l = b[0], r = b[1];
+          const cpx<double> hk = glw::cadd(glw::cmul(shx[l], {0.0, kx[r]}), glw::cmul(shy[l], {0.0, ky[r]}));
+          hor = glw::cadd(hor, glw::cmul(hk, eta[r]));
+          const cpx<double> kk = glw::cadd(glw::cadd(glw::cmul(shx[l], shx[r]), glw::cmul(shy[l], shy[r])),
+                                           glw::cmul(sv[l], sv[r]));
+          kin = glw::cadd(kin, cs(kk, 0.5));
+        });
+        for (int order = 1; order < d; ++order)
+          partitions(mask, order + 1, [&](const int* b, cpx<double> ep) {
+            const int pm = b[0];
+            const cpx<double> base = cs(glw::cmul(phi[pm], ep), inv_fact[order]);
+            const double z0 = m.zd(kx[pm], ky[pm], order), z1 = m.zd(kx[pm], ky[pm], order + 1);
+            taylor = glw::cadd(taylor, cs(base, z0));
+            lv = glw::cadd(lv, cs(base, z1));
+            lt = glw::cadd(lt, mul_i(-w[pm] * z0, base));
+          });
+        const double qm = m.q(kx[mask], ky[mask]);
+        const cpx<double> src0 = {-(hor.x - lv.x) - qm * taylor.x, -(hor.y - lv.y) - qm * taylor.y};
+        const cpx<double> it = mul_i(w[mask], taylor);
+        const cpx<double> src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
+        const double mag = hypot(kx[mask], ky[mask]);
+        if (mag <= 100.0 * 2.220446049250313e-16 * fmax(mag, 1.0)) {  // strict zero (:356-364)
+          eta[mask] = psi[mask] = {0.0, 0.0};
+          o.zcount += 1;
+          o.zsrc = fmax(o.zsrc, hypot(hypot(src0.x, src0.y), hypot(src1.x, src1.y)));
+        } else if (d == 2 || (d == 3 && abs(chg[mask]) == 3)) {
+          m.solve(src0, src1, kx[mask], ky[mask], w[mask], eta[mask], psi[mask]);
+        } else {
+          cpx<double> f0 = src0, f1 = src1;
+          bool resonant = d == 3;  // a cubic subset of charge +-1 keeps its free-wave part (:370-386)
+          int ts = chg[mask] > 0 ? 1 : -1;
+          if (d == 4) {
+            // push of the retained cubic states on the quartic source (:282-298)
+            for (int s = 0; s < 4; ++s) {
+              const int cm = mask & ~(1 << s);
+              if (ret[cm].x == 0.0 && ret[cm].y == 0.0) continue;
+              cpx<double> pe, pp;
+              m.normal(kx[cm], ky[cm], chg[cm] > 0 ? 1 : -1, skx[s], sky[s], sg[s], pe, pp);
+              f0 = glw::csub(f0, glw::cmul(ret[cm], pe));
+              f1 = glw::csub(f1, glw::cmul(ret[cm], pp));
+            }
+            eff0 = f0;
+            eff1 = f1;
+            // y31 coordinate (:300-336)
+            for (int s = 0; s < 4; ++s) {
+              const int cm = mask & ~(1 << s), cc = chg[cm];
+              if (abs(cc) != 1) continue;
+              if (hypot(eta[cm].x, eta[cm].y) + hypot(psi[cm].x, psi[cm].y) == 0.0) continue;
+              const double cf = m.om(kx[cm], ky[cm]);
+              const cpx<double> ps = mul_i(cc * cf / m.g, psi[cm]);
+              const cpx<double> ae = cs(glw::cadd(eta[cm], ps), 0.5), oe = cs(glw::csub(eta[cm], ps), 0.5);
+              const cpx<double> ap = mul_i(-m.g * cc / cf, ae), op = mul_i(m.g * cc / cf, oe);
+              const double sq = m.q(skx[s], sky[s]);
+              const cpx<double> se = {0.5, 0.0}, spsi = {0.0, -sg[s] * m.om(pkx[s], pky[s]) / (2.0 * sq)};
+              cpx<double> e1, p1, e2, p2;
+              m.cross(kx[cm], ky[cm], cc * cf, ae, ap, skx[s], sky[s], sw[s], se, spsi, e1, p1);
+              m.cross(kx[cm], ky[cm], -cc * cf, oe, op, skx[s], sky[s], sw[s], se, spsi, e2, p2);
+              o.ecoord = glw::cadd(o.ecoord, glw::cadd(e1, e2));
+              o.pcoord = glw::cadd(o.pcoord, glw::cadd(p1, p2));
+            }
+            const double qo = m.q(kx[mask], ky[mask]), lin = m.g * qo, tmp = w[mask] * w[mask];
+            const double rho = fabs(lin - tmp) / fmax(lin + tmp, 2.2250738585072014e-308);
+            resonant = rho < m.thr;
+            ts = w[mask] >= 0.0 ? 1 : -1;
+          }
+          if (resonant) {
+            const double wo = m.om(kx[mask], ky[mask]);
+            const double ms = sqrt(wo / (2.0 * m.g)), ns = sqrt(m.g / (2.0 * wo));
+            const cpx<double> a0 = cs(f0, 1.0 / (2.0 * ms)), a1 = mul_i(ts, cs(f1, 1.0 / (2.0 * ns)));
+            ret[mask] = glw::cadd(a0, a1);
+            const cpx<double> comp = glw::csub(a0, a1);
+            const double den = w[mask] + ts * wo;
+            eta[mask] = cs(mul_i(ms, comp), 1.0 / den);
+            psi[mask] = cs(comp, -ts * ns / den);
+          } else {
+            m.solve(f0, f1, kx[mask], ky[mask], w[mask], eta[mask], psi[mask]);
+          }
+        }
+        phi[mask] = glw::csub(psi[mask], taylor);
+      }
+      if (d < 4) {  // surface derivatives of this subset, orders 0..d-1 (:185-225)
+        cpx<double> hx = {0.0, 0.0}, hy = {0.0, 0.0}, v = {0.0, 0.0};
+        for (int order = 0; order < d; ++order)
+          partitions(mask, order + 1, [&](const int* b, cpx<double> ep) {
+            const int pm = b[0];
+            const cpx<double> base = cs(glw::cmul(phi[pm], ep), inv_fact[order]);
+            const double z0 = m.zd(kx[pm], ky[pm], order), z1 = m.zd(kx[pm], ky[pm], order + 1);
+            hx = glw::cadd(hx, mul_i(kx[pm] * z0, base));
+            hy = glw::cadd(hy, mul_i(ky[pm] * z0, base));
+            v = glw::cadd(v, cs(base, z1));
+          });
+        shx[mask] = hx;
+        shy[mask] = hy;
+        sv[mask] = v;
+      }
+    }
+  }
+  // residual of the quartic output against its effective source (:413-433)
+  const double qo = m.q(kx[15], ky[15]), lin = m.g * qo, tmp = w[15] * w[15];
+  o.rho = fabs(lin - tmp) / fmax(lin + tmp, 2.2250738585072014e-308);
+  cpx<double> er = glw::csub(glw::csub(mul_i(-w[15], eta[15]), cs(psi[15], qo)), eff0);
+  cpx<double> pr = glw::csub(glw::cadd(mul_i(-w[15], psi[15]), cs(eta[15], m.g)), eff1);
+  o.near = o.rho < m.thr;
+  if (o.near) {
+    const double wo = m.om(kx[15], ky[15]);
+    const double ms = sqrt(wo / (2.0 * m.g)), ns = sqrt(m.g / (2.0 * wo));
+    const int ts = w[15] >= 0.0 ? 1 : -1;
+    er = glw::cadd(er, cs(ret[15], ms));
+    pr = glw::cadd(pr, mul_i(-ts * ns, ret[15]));
+  }
+  o.eta = eta[15];
+  o.psi = psi[15];
+  o.retained = ret[15];
+  o.residual = hypot(hypot(er.x, er.y), hypot(pr.x, pr.y));
+  return o;
+}
+
+struct SignedArgs {
+  WitDev w;
+  double dkx, dky;
+  const cpx<double>* amps;  // [cases][M]
+  int cases;
+  double* out;              // [5][cases][ny][nx] complex
+  unsigned long long* cnt;
+  unsigned long long* dmax;
+  long long total;          // candidates
+};
+
+// pair index p (i <= j, row i starting at offP[i]) -> (i, j)
+__device__ __forceinline__ void unrank_pair(const WitDev& w, long long p, int& i, int& j) {
+  int lo = 0, hi = w.M - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (w.offP[mid] <= p) lo = mid; else hi = mid - 1;
+  }
+  i = lo;
+  j = i + (int)(p - w.offP[i]);
+}
+
+// triple index t (i <= j <= k, tidx order) -> (i, j, k)
+__device__ __forceinline__ void unrank_triple(const WitDev& w, long long t, int& i, int& j, int& k) {
+  int lo = 0, hi = w.M - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (w.offT[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  i = lo;
+  int a, b;
+  unrank_pair(w, w.offP[i] + (t - w.offT[i]), a, b);
+  j = a;
+  k = b;
+}
+
+// SECTOR 40: candidate c = minus pair * P + plus pair (signs + + - -);
+// SECTOR 42: c = minus mode * T + plus triple (signs + + + -).
+template <int SECTOR>
+__global__ __launch_bounds__(WNT) void k_wit_signed(SignedArgs g) {
+  const WitDev& w = g.w;
+  const Sgn m{w.md.g, w.md.h, w.md.thr};
+  unsigned long long kept = 0, nearc = 0, zo = 0, zi = 0;
+  double maxres = 0.0, maxz = 0.0, mindet = INFINITY;
+  const size_t plane = (size_t)w.nx * w.ny * g.cases;
+  for (long long c = blockIdx.x * (long long)WNT + threadIdx.x; c < g.total; c += (long long)gridDim.x * WNT) {
+    int id[4], sg[4];
+    int repeat;
+    if constexpr (SECTOR == 40) {
+      const long long P = w.offP[w.M];
+      unrank_pair(w, c % P, id[0], id[1]);
+      unrank_pair(w, c / P, id[2], id[3]);
+      sg[0] = sg[1] = 1;
+      sg[2] = sg[3] = -1;
+      repeat = (1 + (id[0] == id[1])) * (1 + (id[2] == id[3]));
+    } else {
+      const long long T = w.offT[w.M];
+      unrank_triple(w, c % T, id[0], id[1], id[2]);
+      id[3] = (int)(c / T);
+      sg[0] = sg[1] = sg[2] = 1;
+      sg[3] = -1;
+      repeat = id[0] == id[2] ? 6 : ((id[0] == id[1] || id[1] == id[2]) ? 2 : 1);
+    }
+    int tx = 0, ty = 0;
+    for (int s = 0; s < 4; ++s) {
+      tx += sg[s] * w.modes[2 * id[s]];
+      ty += sg[s] * w.modes[2 * id[s] + 1];
+    }
+    if (tx == 0 && ty == 0) {
+      ++zo;
+      continue;
+    }
+    if (SECTOR == 40 && !(tx > 0 || (tx == 0 && ty > 0))) continue;  // analytic half plane (:566-568)
+    if (abs(tx) >= w.nx / 2 || abs(ty) >= w.ny / 2) continue;
+    double pkx[4], pky[4];
+    for (int s = 0; s < 4; ++s) {
+      pkx[s] = w.modes[2 * id[s]] * g.dkx;
+      pky[s] = w.modes[2 * id[s] + 1] * g.dky;
+    }
+    const SignedOut o = evaluate_signed(m, pkx, pky, sg);
+    const size_t ix = (size_t)(((tx % w.nx) + w.nx) % w.nx), iy = (size_t)(((ty % w.ny) + w.ny) % w.ny);
+    for (int cs_ = 0; cs_ < g.cases; ++cs_) {
+      const cpx<double>* ac = g.amps + (size_t)cs_ * w.M;
+      cpx<double> wt = {2.0, 0.0};
+      for (int s = 0; s < 4; ++s) {
+        const cpx<double> a = ac[id[s]];
+        wt = glw::cmul(wt, sg[s] > 0 ? a : cpx<double>{a.x, -a.y});
+      }
+      wt = {wt.x / repeat, wt.y / repeat};
+      const size_t loc = (size_t)cs_ * w.nx * w.ny + ix + (size_t)w.nx * iy;
+      atomic_addc(g.out + 2 * loc, glw::cmul(wt, o.eta));
+      atomic_addc(g.out + 2 * (plane + loc), glw::cmul(wt, o.psi));
+      atomic_addc(g.out + 2 * (2 * plane + loc), glw::cmul(wt, o.ecoord));
+      atomic_addc(g.out + 2 * (3 * plane + loc), glw::cmul(wt, o.pcoord));
+      if (o.near) atomic_addc(g.out + 2 * (4 * plane + loc), glw::cmul(wt, o.retained));
+    }
+    ++kept;
+    nearc += o.near ? 1 : 0;
+    zi += (unsigned long long)o.zcount;
+    maxres = fmax(maxres, o.residual);
+    mindet = fmin(mindet, o.rho);
+    maxz = fmax(maxz, o.zsrc);
+  }
+  kept = wsum_u64(kept);
+  nearc = wsum_u64(nearc);
+  zo = wsum_u64(zo);
+  zi = wsum_u64(zi);
+  maxres = wmax_d(maxres);
+  maxz = wmax_d(maxz);
+  mindet = wmin_d(mindet);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(g.cnt + 0, kept);
+    atomicAdd(g.cnt + 1, nearc);
+    atomicAdd(g.cnt + 2, zo);
+    atomicAdd(g.cnt + 3, zi);
+    atomicMax(g.dmax + 0, (unsigned long long)__double_as_longlong(maxres));
+    atomicMax(g.dmax + 1, (unsigned long long)__double_as_longlong(maxz));
+    if (mindet < INFINITY) atomicMin(g.dmax + 2, (unsigned long long)__double_as_longlong(mindet));
+  }
+}
+
 __global__ void k_wit_clear(double* out, size_t n, unsigned long long* cnt, unsigned long long* dmax) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i < n) out[i] = 0.0;
@@ -754,6 +1125,35 @@ int glw_wit_sector44(glw_wit* W, const double* amps, int32_t cases, double* out,
   }
   const double total = (double)M * (M + 1) * (M + 2) * (M + 3) / 24.0;
   hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, total, counters);
+  W_HIP(hipGetLastError());
+  return GLW_OK;
+}
+
+int glw_wit_sector(glw_wit* W, int32_t sector, const double* amps, int32_t cases, double* out, double* counters,
+                   void* stream) {
+  if (sector == 44) return glw_wit_sector44(W, amps, cases, out, counters, stream);
+  if (sector != 40 && sector != 42) return wfail(GLW_EINVAL, "sector must be 40, 42, or 44");
+  if (!W || !amps || !out || !counters || cases < 1) return wfail(GLW_EINVAL, "null argument");
+  if (W->d.precision != GLW_PREC_FP64) return wfail(GLW_EUNSUPPORTED, "sectors 40 and 42 evaluate in FP64 only");
+  W_HIP(hipSetDevice(W->device));
+  hipStream_t st = (hipStream_t)stream;
+  const size_t n = 5ull * cases * W->d.nx * W->d.ny * 2;
+  hipLaunchKernelGGL(k_wit_clear, dim3((n + 255) / 256), dim3(256), 0, st, out, n, W->cnt, W->dmax);
+  SignedArgs g;
+  g.w = W->w;
+  g.dkx = W->d.dkx;
+  g.dky = W->d.dky;
+  g.amps = (const cpx<double>*)amps;
+  g.cases = cases;
+  g.out = out;
+  g.cnt = W->cnt;
+  g.dmax = W->dmax;
+  const long long M = W->d.modes_count, P = M * (M + 1) / 2, T = M * (M + 1) * (M + 2) / 6;
+  g.total = sector == 40 ? P * P : T * M;
+  const long long blocks = std::min<long long>((g.total + WNT - 1) / WNT, 1 << 20);
+  if (sector == 40) hipLaunchKernelGGL(k_wit_signed<40>, dim3((unsigned)blocks), dim3(WNT), 0, st, g);
+  else hipLaunchKernelGGL(k_wit_signed<42>, dim3((unsigned)blocks), dim3(WNT), 0, st, g);
+  hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, (double)g.total, counters);
   W_HIP(hipGetLastError());
   return GLW_OK;
 }

@@ -1,9 +1,12 @@
-"""WIT sector-44 quartet sum on the GPU (reference paper/order4/cpp/directional_eta4_openmp.cpp).
+"""WIT quartet sums on the GPU (reference paper/order4/cpp/directional_eta4_openmp.cpp).
 
 ``WitPlan(modes, ...)`` tabulates the single/pair/triple subset states of a
 mode set once; ``sector44(amplitudes)`` sums every unordered quartet
 i<=j<=k<=l into the five DIR44O01 output arrays and the reference's counters.
-``run_dir4in(input, output)`` is the drop-in for ``wit_eta4 INPUT OUTPUT --sector 44``.
+``sector(40 | 42, amplitudes)`` runs the signed-parent sectors (plus pair x
+minus pair; plus triple x minus mode), evaluated per candidate in FP64.
+``run_dir4in(input, output, sector=...)`` is the drop-in for
+``wit_eta4 INPUT OUTPUT --sector 40|42|44``.
 """
 
 from __future__ import annotations
@@ -35,6 +38,8 @@ def _bind(lib):
     lib.glw_wit_destroy.restype = c_int
     lib.glw_wit_sector44.argtypes = [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
     lib.glw_wit_sector44.restype = c_int
+    lib.glw_wit_sector.argtypes = [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
+    lib.glw_wit_sector.restype = c_int
     return lib
 
 
@@ -74,7 +79,14 @@ class WitPlan:
 
     def sector44_device(self, amplitudes, out=None, counters=None):
         """Device run: amplitudes complex128 [cases][M] (numpy or torch); returns (out, counters) tensors."""
+        return self.sector_device(44, amplitudes, out, counters)
+
+    def sector_device(self, sector: int, amplitudes, out=None, counters=None):
+        """Device run of sector 40, 42 or 44 (see sector44_device)."""
         import torch
+
+        if sector not in (40, 42, 44):
+            raise ValueError("sector must be 40, 42, or 44")  # reference message (:511-513)
 
         a = torch.as_tensor(np.atleast_2d(amplitudes) if isinstance(amplitudes, np.ndarray) else amplitudes)
         a = a.to(device=self.device, dtype=torch.complex128).contiguous()
@@ -86,15 +98,19 @@ class WitPlan:
         if counters is None:
             counters = torch.empty(8, dtype=torch.float64, device=self.device)
         stream = torch.cuda.current_stream(self.device).cuda_stream
-        _check(self.lib.glw_wit_sector44(self._h, c_void_p(a.data_ptr()), cases, c_void_p(out.data_ptr()),
-                                         c_void_p(counters.data_ptr()), c_void_p(stream)))
+        _check(self.lib.glw_wit_sector(self._h, int(sector), c_void_p(a.data_ptr()), cases, c_void_p(out.data_ptr()),
+                                       c_void_p(counters.data_ptr()), c_void_p(stream)))
         return out, counters
 
     def sector44(self, amplitudes) -> dict:
+        return self.sector(44, amplitudes)
+
+    def sector(self, sector: int, amplitudes) -> dict:
+        """Host run of one sector: the five output arrays and the counters, plus the wall time."""
         import torch
 
         t0 = time.perf_counter()
-        out, cnt = self.sector44_device(amplitudes)
+        out, cnt = self.sector_device(sector, amplitudes)
         torch.cuda.synchronize()
         secs = time.perf_counter() - t0
         o = out.cpu().numpy()
@@ -108,16 +124,16 @@ class WitPlan:
 
 
 def run_dir4in(input_path, output_path=None, *, sector: int = 44) -> dict:
-    """Drop-in for ``wit_eta4 INPUT OUTPUT --sector 44``: read DIR4IN01, sum, optionally write DIR44O01."""
-    if sector != 44:
-        raise ValueError("only sector 44 is implemented on the GPU")
+    """Drop-in for ``wit_eta4 INPUT OUTPUT --sector 40|42|44``: read DIR4IN01, sum, optionally write DIR4xO01."""
+    if sector not in (40, 42, 44):
+        raise ValueError("sector must be 40, 42, or 44")  # reference message (:511-513)
     inp = formats.read_dir4in(input_path)
     plan = WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
                    dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
     try:
-        res = plan.sector44(inp["amplitudes"])
+        res = plan.sector(sector, inp["amplitudes"])
     finally:
         plan.close()
     if output_path is not None:
-        formats.write_dir4out(output_path, 44, res, res)
+        formats.write_dir4out(output_path, sector, res, res)
     return res

@@ -184,6 +184,14 @@ int glw_wit_destroy(glw_wit* wit);
  * schedule-dependent order (relative differences at the 1e-16 level). */
 int glw_wit_sector44(glw_wit* wit, const double* amplitudes, int32_t cases, double* out, double* counters,
                      void* stream);
+/* Any sector of the reference runner (`--sector 40|42|44`,
+ * directional_eta4_openmp.cpp:603-734): 44 is glw_wit_sector44; 40 (plus
+ * pair x minus pair, analytic half plane) and 42 (plus triple x minus mode)
+ * have signed parents, evaluated per candidate with the retained-V3 push and
+ * the y31 coordinate (:282-336) in FP64.  Same arrays and counters as above
+ * (total = the sector's candidate count). */
+int glw_wit_sector(glw_wit* wit, int32_t sector, const double* amplitudes, int32_t cases, double* out,
+                   double* counters, void* stream);
 
 /* Stage timing (instrumentation; no reference counterpart).  enable != 0
  * creates a pool of HIP events; every kernel launch of later execute calls is

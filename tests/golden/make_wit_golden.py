@@ -8,7 +8,10 @@
 * converts those outputs and the reference's shipped n0382 output
   (paper/order4/data/wit_field_n0382.bin) to npz: eta and psi on their
   nonzero bins plus the header counters (the other three arrays are zero in
-  these runs: near = 0).
+  these runs: near = 0);
+* runs it on n0050 in sectors 40 and 42 (signed parents) and keeps all five
+  arrays on their nonzero bins (eta and psi coordinates, and in sector 42
+  two near-resonant retained bins).
 """
 
 import shutil
@@ -50,3 +53,20 @@ for name in ("n0050", "n0100"):
         subprocess.run([str(EXE), str(HERE / f"wit_{name}_input.bin"), str(out_path), "--sector", "44"], check=True)
         save(name, formats.read_dir4out(out_path))
 save("n0382", formats.read_dir4out(REF / "paper" / "order4" / "data" / "wit_field_n0382.bin"))
+
+
+ARRAYS = ("eta", "psi", "eta_coordinate", "psi_coordinate", "retained")
+COUNTERS = ("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate", "max_residual",
+            "min_detuning", "max_strict_zero_source")
+for sector in (40, 42):
+    with tempfile.TemporaryDirectory() as td:
+        out_path = Path(td) / "out.bin"
+        subprocess.run([str(EXE), str(HERE / "wit_n0050_input.bin"), str(out_path), "--sector", str(sector)],
+                       check=True)
+        out = formats.read_dir4out(out_path)
+    idx = np.flatnonzero(np.any([out[k] != 0 for k in ARRAYS], axis=0))
+    np.savez_compressed(HERE / f"wit_n0050_sector{sector}.npz", index=idx, shape=np.array(out["eta"].shape),
+                        counters=np.array([out[k] for k in COUNTERS], float),
+                        **{k: out[k].ravel()[idx] for k in ARRAYS})
+    print("n0050 sector", sector, "nonzero bins", idx.size, "kept", out["kept"], "near", out["near"])
+

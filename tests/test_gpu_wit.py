@@ -1,4 +1,4 @@
-"""GPU WIT sector-44 sum vs the reference executable's outputs and the oracle restatement.
+"""GPU WIT sums (sector 44; signed sectors 40, 42) vs the reference executable's outputs and the oracle restatement.
 
 Tolerance 1e-12 relative L2 (measured ~1e-16 for the restatement): the WIT sum
 is a well-conditioned sum of positive-definite-denominator terms; the GPU adds
@@ -39,6 +39,39 @@ def test_wit_matches_reference_outputs(witmod, name):
     assert r["max_residual"] < 10 * c["max_residual"] + 1e-15
     assert r["min_detuning"] == pytest.approx(c["min_detuning"], rel=1e-12)
     assert not np.any(r["eta_coordinate"]) and not np.any(r["retained"])
+
+
+SIGNED_ARRAYS = ("eta", "psi", "eta_coordinate", "psi_coordinate", "retained")
+
+
+@pytest.mark.parametrize("sector", [40, 42])
+def test_wit_signed_sectors_match_reference_outputs(witmod, sector):
+    """Sectors 40 and 42 (signed parents: retained-V3 push, y31 coordinate) on n0050 against the
+    reference executable's own outputs (tests/golden/make_wit_golden.py): all five arrays, the
+    counters; sector 42 has two near-resonant quartets (retained bins)."""
+    g = np.load(GOLDEN / f"wit_n0050_sector{sector}.npz")
+    c = dict(zip(("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate", "max_residual",
+                  "min_detuning", "max_strict_zero_source"), g["counters"]))
+    print(f"WIT n0050 sector {sector}", flush=True)
+    r = witmod.run_dir4in(GOLDEN / "wit_n0050_input.bin", sector=sector)
+    print(f"WIT n0050 sector {sector}: {r['kept']} candidates in {r['seconds']:.3f} s", flush=True)
+    for k in ("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate"):
+        assert r[k] == int(c[k]), k
+    # a term with normalized detuning rho is solved through a denominator ~rho: rounding moves it by
+    # ~eps / rho (sector 42 reaches rho = 6.6e-7, sector 40 8e-5)
+    tol = max(1e-12, 1e-15 / c["min_detuning"])
+    for k in SIGNED_ARRAYS:
+        ref = np.zeros(int(np.prod(g["shape"])), complex)
+        ref[g["index"]] = g[k]
+        if not np.any(ref):  # no near-resonant quartet: nothing retained
+            assert not np.any(r[k]), k
+            continue
+        err = rel(r[k], ref.reshape(tuple(g["shape"])))
+        print(f"  {k}: {err:.2e} (tolerance {tol:.1e})", flush=True)
+        assert err < tol, k
+    assert r["max_residual"] < 10 * c["max_residual"] + 1e-15
+    assert r["min_detuning"] == pytest.approx(c["min_detuning"], rel=1e-12)
+    assert r["max_strict_zero_source"] == pytest.approx(c["max_strict_zero_source"], rel=1e-12)
 
 
 def test_wit_near_resonant_branch_vs_restatement(witmod):
