@@ -252,7 +252,8 @@ constexpr int kRowWork = GLW_MODAL_ROW_WORK;
 constexpr int kPoints = GLW_MODAL_POINTS;
 __host__ __device__ constexpr int t_rows(int T) { return T >= kRowWork ? 1 : (kRowWork / T > 4 ? 4 : kRowWork / T); }
 __host__ __device__ constexpr int t_threads(int T) {
-  const int n = (t_rows(T) * T / kPoints) / 64 * 64;
+  // past 2048 points twice the points per thread: 512 threads keep the 256-VGPR budget
+  const int n = (t_rows(T) * T / (T > 2048 ? 2 * kPoints : kPoints)) / 64 * 64;
   return n < 64 ? 64 : (n > 1024 ? 1024 : n);
 }
 // Radix plan of the time-pass transforms (1: greedy radix 8/4/2/3, fewer

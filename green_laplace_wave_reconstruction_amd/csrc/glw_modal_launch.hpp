@@ -8,7 +8,7 @@
 
 // Time-grid and modal-grid lengths with kernels (2^a 3^b, multiples of 16).
 #define GLW_MODAL_T_LENGTHS(X) \
-  X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048)
+  X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) X(3072) X(4096)
 #define GLW_MODAL_Q_LENGTHS(X)                                                                          \
   X(16) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) \
   X(3072) X(4096) X(6144) X(8192)

@@ -160,6 +160,22 @@ def test_split_modal_length_order2_vs_oracle(MD):
     assert rel(r.components["eta20"], o.components["eta20"]) < 1e-12
 
 
+@pytest.mark.parametrize("count,periods,t_len", [(6144, 150.0, 3072), (8192, 200.0, 4096)])
+def test_long_record_time_grid_vs_oracle(MD, count, periods, t_len):
+    """Records whose kept band needs a time grid past 2048 points (time pass with twice the points
+    per thread), against the oracle."""
+    from green_laplace_wave_reconstruction_amd import seastate
+
+    eta, t, h = seastate.unidirectional_record(count, 5, periods=periods)
+    kw = dict(depth=h, quadrature_rank=8, order=3, modal_points=2048, include_eta20=False)
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    assert r.audit["reduced_grid"]["t_len"] == t_len
+    o = M.reconstruct_unidirectional_modal_timeseries(eta, t, **kw)
+    errs = {k: rel(r.components[k], o.components[k]) for k in ("eta11", "eta22_analytic", "eta33_analytic")}
+    print("long record", count, r.audit["reduced_grid"], errs, flush=True)
+    assert max(errs.values()) < 1e-12
+
+
 def test_broadband_order2_vs_oracle(MD):
     """Order 2 (no pair fields, 2 span + 1 time grid) on the broadband record, GL rank 6."""
     eta, t, h = broadband_record(1536, seed=21)
