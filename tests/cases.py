@@ -2,6 +2,7 @@
 
 from __future__ import annotations
 
+import math
 from pathlib import Path
 
 import numpy as np
@@ -32,3 +33,38 @@ def headline():
 
 def order2_plain():
     return seastate.order2_plain_128()
+
+
+# One oblique wave a cos(k.x - w t) on a 32^2 periodic box (L = 20 pi m, h = 3 m): the sum-frequency
+# harmonics of a single wave are the finite-depth Stokes harmonics (S = sinh kh, C = cosh kh; Fenton
+# 1985, J. Waterway Port Coastal Ocean Eng. 111(2), table 1: B22, B33, A22, A33 written there with
+# sech 2kh): eta22 = a^2 k B22 cos 2th, eta33 = a^3 k^2 B33 cos 3th and the surface potential
+# psi = phi(eta) = phi + eta phi_z + eta^2 phi_zz / 2 + ... of phi_j = w a^j k^(j-2) A_jj cosh jk(z+h) sin jth:
+#   B22 = C (2 C^2 + 1) / (4 S^3), B33 = 3 (8 C^6 + 1) / (64 S^6), A22 = 3 / (8 S^4), A33 = (13 - 4 C^2) / (64 S^7),
+#   psi22 = a^2 w (A22 cosh 2kh + 1/2) sin 2th,
+#   psi33 = a^3 w k (A33 cosh 3kh + A22 sinh 2kh + B22 / 2 + C / (8 S)) sin 3th.
+# The slow GL scale is the single wave's own exponent, so the two-node psi22 rule is exact; eta22 and
+# eta33 converge to their limits as the GL ranks grow.
+STOKES_WAVES = [(2, 1, 0.0), (3, -2, 1.7), (5, 2, 0.4)]   # mode (mx, my), time; kh 0.67, 1.08, 1.62
+
+
+def stokes_wave(mx, my, t, n=32, lx=20 * np.pi, depth=3.0, gravity=9.81, amp=0.01):
+    x = np.arange(n) * (lx / n)
+    xx, yy = np.meshgrid(x, x)
+    kx, ky = 2 * np.pi * mx / lx, 2 * np.pi * my / lx
+    k = math.hypot(kx, ky)
+    s, c = math.sinh(k * depth), math.cosh(k * depth)
+    om = math.sqrt(gravity * k * math.tanh(k * depth))
+    th = kx * xx + ky * yy - om * t
+    b22, b33 = c * (2 * c * c + 1) / (4 * s**3), 3 * (8 * c**6 + 1) / (64 * s**6)
+    a22, a33 = 3 / (8 * s**4), (13 - 4 * c * c) / (64 * s**7)
+    kh = k * depth
+    closed = {
+        "eta22": amp**2 * k * b22 * np.cos(2 * th),
+        "psi22": amp**2 * om * (a22 * math.cosh(2 * kh) + 0.5) * np.sin(2 * th),
+        "eta33": amp**3 * k * k * b33 * np.cos(3 * th),
+        "psi33": amp**3 * om * k * (a33 * math.cosh(3 * kh) + a22 * math.sinh(2 * kh) + b22 / 2 + c / (8 * s))
+        * np.sin(3 * th),
+    }
+    return dict(eta1=amp * np.cos(kx * xx + ky * yy), lx=lx, depth=depth, gravity=gravity, n=n, kh=k * depth,
+                closed=closed)

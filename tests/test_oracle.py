@@ -6,7 +6,7 @@ import math
 import numpy as np
 import pytest
 
-from cases import GOLDEN, fixture, rel
+from cases import GOLDEN, STOKES_WAVES, fixture, rel, stokes_wave
 from oracle import gl_oracle as O
 
 
@@ -65,6 +65,41 @@ def test_psi22_two_node_rule_is_gl2():
     np.testing.assert_allclose(x, xg, rtol=1e-14)
     np.testing.assert_allclose(w, wg, rtol=1e-14)
     np.testing.assert_allclose(w, [(2 + math.sqrt(2)) / 4, (2 - math.sqrt(2)) / 4], rtol=1e-14)
+
+
+@pytest.mark.parametrize("mx,my,t", STOKES_WAVES)
+def test_single_wave_second_order_is_finite_depth_stokes(mx, my, t):
+    w = stokes_wave(mx, my, t)
+    errs = []
+    for rank in (4, 8, 12, 16):
+        c = O.reconstruct_directional_field(w["eta1"], lx=w["lx"], ly=w["lx"], depth=w["depth"],
+                                            gravity=w["gravity"], time=t, order=2, eta22_rank=rank)["components"]
+        assert rel(c["psi22"], w["closed"]["psi22"]) < 1e-13, rank
+        errs.append(rel(c["eta22"], w["closed"]["eta22"]))
+    # geometric convergence of the GL eta22 rule: 2.4e-2 .. 6e-8 at kh 0.67, 4e-3 .. 2e-11 at kh 1.62
+    assert all(b < 0.1 * a for a, b in zip(errs, errs[1:])), errs
+    assert errs[-1] < (1e-7 if w["kh"] < 1 else 1e-9), errs
+
+
+@pytest.mark.parametrize("mx,my,t", STOKES_WAVES[1:])
+def test_single_wave_third_harmonic_converges_to_stokes(mx, my, t):
+    """Order-3 eta33+ and psi33+ of the nested GL graph (analytic fields in units of h and
+    h sqrt(g h)) at inner x outer ranks 4x4, 8x8, 12x10 against the Stokes third harmonics
+    (time 0: the order-3 graph is stationary)."""
+    w = stokes_wave(mx, my, 0.0)
+    h, g = w["depth"], w["gravity"]
+    kx, ky = O.wavenumber_grid(w["lx"], w["lx"], w["n"], w["n"])
+    spec, _ = O.analytic_spectrum(w["eta1"], h, kx)   # dimensionless (eta / h)
+    errs = {"eta33": [], "psi33": []}
+    for ri, ro in ((4, 4), (8, 8), (12, 10)):
+        e, p, _ = O.order3(spec, h * kx, h * ky, inner_rank=ri, outer_rank=ro)
+        errs["eta33"].append(rel(np.real(e) * h, w["closed"]["eta33"]))
+        errs["psi33"].append(rel(np.real(p) * h * math.sqrt(g * h), w["closed"]["psi33"]))
+    # eta33: 7.9e-2, 5.8e-3, 1.1e-3 at kh 1.08; 6.7e-2, 3.8e-3, 4.4e-4 at kh 1.62
+    # psi33: 2.6e-2, 2.2e-3, 4.0e-4 at kh 1.08; 3.3e-2, 2.6e-3, 2.8e-4 at kh 1.62
+    for name, e in errs.items():
+        assert e[1] < 0.1 * e[0] and e[2] < 0.3 * e[1], (name, e)
+        assert e[2] < 2e-3, (name, e)
 
 
 def test_linear_closed_form():
