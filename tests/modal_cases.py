@@ -100,3 +100,24 @@ def short_record(count: int = 128, seed: int = 5):
     from green_laplace_wave_reconstruction_amd import seastate
 
     return seastate.unidirectional_record(count, seed, periods=12.0)
+
+
+def stokes_record(period: float, depth: float = 10.0, gravity: float = 9.81, amp: float = 0.05,
+                  n: int = 1024, cycles: int = 16):
+    """One-frequency record a cos(w t) over a whole number of periods and the finite-depth Stokes
+    harmonics it must reduce to (Fenton 1985, J. Waterway Port Coastal Ocean Eng. 111(2), table 1,
+    with S = sinh kh, C = cosh kh): eta22 = a^2 k B22 cos 2wt, B22 = C (2 C^2 + 1) / (4 S^3);
+    eta33 = a^3 k^2 B33 cos 3wt, B33 = 3 (8 C^6 + 1) / (64 S^6)."""
+    om = 2 * np.pi / period
+    t = np.arange(n) * (cycles * period / n)
+    target = om * om * depth / gravity   # q tanh q = w^2 h / g by bisection
+    lo, hi = 1e-9, max(1.0, 2 * target)
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if mid * np.tanh(mid) < target else (lo, mid)
+    kh = 0.5 * (lo + hi)
+    k = kh / depth
+    s, c = np.sinh(kh), np.cosh(kh)
+    b22, b33 = c * (2 * c * c + 1) / (4 * s**3), 3 * (8 * c**6 + 1) / (64 * s**6)
+    closed = {"eta22": amp**2 * k * b22 * np.cos(2 * om * t), "eta33": amp**3 * k * k * b33 * np.cos(3 * om * t)}
+    return amp * np.cos(om * t), t, dict(depth=depth, gravity=gravity, kh=kh, closed=closed)

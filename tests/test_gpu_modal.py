@@ -14,7 +14,7 @@ import pytest
 
 from cases import rel
 from modal_cases import (broadband_record, eta20_fixture_kwargs, off_grid_case, on_grid_case, order3_fixture_kwargs,
-                         short_record, unidirectional)
+                         short_record, stokes_record, unidirectional)
 from oracle import gl_oracle as O
 from oracle import modal_oracle as M
 
@@ -187,3 +187,19 @@ def test_broadband_order2_vs_oracle(MD):
     assert rel(r.components["eta22_analytic"], o.components["eta22_analytic"]) < 1e-12
     assert rel(r.components["eta20"], o.components["eta20"]) < 1e-12
     assert r.audit["levels"][1]["eta22_change"] == pytest.approx(o.audit["levels"][1]["eta22_change"], rel=1e-9)
+
+
+@pytest.mark.parametrize("period", [6.0, 10.0])
+def test_single_frequency_record_matches_stokes(MD, period):
+    """GL rank 10 (its largest order-3 rank): the Stokes limits the oracle approaches
+    (eta22 < 1e-4, eta33 < 5e-3) and the oracle itself at 1e-12."""
+    eta1, t, w = stokes_record(period)
+    kw = dict(depth=w["depth"], gravity=w["gravity"], quadrature_rank=10, order=3, include_eta20=False)
+    print("stokes record period", period, flush=True)
+    r = MD.reconstruct_unidirectional_modal_timeseries(eta1, t, **kw)
+    o = M.reconstruct_unidirectional_modal_timeseries(eta1, t, **kw)
+    assert rel(r.components["eta22"], w["closed"]["eta22"]) < 1e-4
+    assert rel(r.components["eta33"], w["closed"]["eta33"]) < 5e-3
+    for name in ("eta22_analytic", "eta33_analytic"):
+        assert rel(r.components[name], o.components[name]) < 1e-12, name
+    assert r.audit["modal_points"] == o.audit["modal_points"]

@@ -11,7 +11,8 @@ import numpy as np
 import pytest
 
 from cases import rel
-from modal_cases import eta20_fixture_kwargs, off_grid_case, on_grid_case, order3_fixture_kwargs, unidirectional
+from modal_cases import (eta20_fixture_kwargs, off_grid_case, on_grid_case, order3_fixture_kwargs, stokes_record,
+                         unidirectional)
 from oracle import gl_oracle as O
 from oracle import modal_oracle as M
 
@@ -89,3 +90,21 @@ def test_homogeneity():
     r2 = M.reconstruct_unidirectional_modal_timeseries(2.5 * d["eta1"], d["time"], **kw)
     assert rel(r2.components["eta22_analytic"], 2.5**2 * r1.components["eta22_analytic"]) < 1e-14
     assert rel(r2.components["eta33_analytic"], 2.5**3 * r1.components["eta33_analytic"]) < 1e-14
+
+
+@pytest.mark.parametrize("period", [6.0, 10.0])
+def test_single_frequency_record_converges_to_stokes(period):
+    """A one-frequency record's eta22 and eta33 approach the finite-depth Stokes harmonics as the
+    GL rank grows (kh 1.30 and 0.68; the ladder stops at 4096 modal points)."""
+    eta1, t, w = stokes_record(period)
+    errs = {"eta22": [], "eta33": []}
+    for rank in (4, 8, 10):
+        r = M.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=w["depth"], gravity=w["gravity"],
+                                                          quadrature_rank=rank, order=3, include_eta20=False)
+        for k in errs:
+            errs[k].append(rel(r.components[k], w["closed"][k]))
+    # eta22: 5.6e-3, 1.3e-5, 5.8e-7 at kh 1.30; 2.3e-2, 3.3e-4, 3.4e-5 at kh 0.68
+    # eta33: 7.4e-2, 4.7e-3, 1.4e-3 at kh 1.30; 8.3e-2, 1.1e-2, 3.5e-3 at kh 0.68
+    e22, e33 = errs["eta22"], errs["eta33"]
+    assert e22[1] < 0.05 * e22[0] and e22[2] < 0.2 * e22[1] and e22[2] < 1e-4, e22
+    assert e33[1] < 0.2 * e33[0] and e33[2] < 0.5 * e33[1] and e33[2] < 5e-3, e33
