@@ -369,7 +369,7 @@ def bench_inverse(args):
     }
 
 
-MODAL_POINTS = 8192
+MODAL_POINTS = 8192   # default modal grid of the bench (--modal-points)
 
 
 def _modal_alg_bytes(d) -> dict:
@@ -381,18 +381,19 @@ def _modal_alg_bytes(d) -> dict:
 
 
 def bench_modal(args):
-    """Single-direction modal time path: one step = one 8192-point modal level (eta22 + eta33, GL8) and
+    """Single-direction modal time path: one step = one modal level (8192 points by default) (eta22 + eta33, GL8) and
     eta20 (GL16, 128 x 16) of a 2048-sample k_p h = 1 JONSWAP record over 50 T_p, deposits resident."""
     import torch
 
     from green_laplace_wave_reconstruction_amd import modal as MD
     from green_laplace_wave_reconstruction_amd import seastate
 
+    points = int(args.modal_points)
     eta1, t, h = seastate.unidirectional_record()
     g = 9.81
     band = MD._select_band(eta1, t, depth=h, gravity=g, order=3, omega_max=None, energy_fraction=0.999)
     dev = MD.ModalDevice(band.count)
-    lp = MD.modal_level_plan(band, depth=h, gravity=g, rank=8, order=3, points=MODAL_POINTS, modal_limit=None)
+    lp = MD.modal_level_plan(band, depth=h, gravity=g, rank=8, order=3, points=points, modal_limit=None)
     dep = dev.upload(lp.deposit)
     d20, p20, _ = MD.eta20_plan(band.amplitudes, band.omega, band.q, band.count, band.dt, depth=h, gravity=g,
                                 rank=16, nx=128, ny=16)
@@ -419,12 +420,14 @@ def bench_modal(args):
     alg = _modal_alg_bytes(lp.desc)
     dominant = max(alg, key=lambda n: mean_ms[n])
     achieved = alg[dominant] / (mean_ms[dominant] * 1e-3) / 1e9
+    # counter-measured bytes exist for the default grid only
+    traffic = traffic_from_profiles(dominant, "pmc_traffic_modal.json") if points == MODAL_POINTS else None
     # end-to-end public calls (host analysis, planning, device, copies back), after one untimed call
     api = None
     if not args.no_api:
-        MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=MODAL_POINTS)
+        MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=points)
         t1 = time.perf_counter()
-        MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=MODAL_POINTS)
+        MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=points)
         api_s = time.perf_counter() - t1
         t2 = time.perf_counter()
         lad = MD.reconstruct_unidirectional_modal_timeseries(eta1, t, depth=h, modal_points=(2048, 4096, 8192))
@@ -436,31 +439,32 @@ def bench_modal(args):
 
         ob = M.select_band(eta1, t, depth=h, gravity=g, order=3, omega_max=None, energy_fraction=0.999)
         t3 = time.perf_counter()
-        M.modal_level(ob, depth=h, gravity=g, rank=8, order=3, points=MODAL_POINTS, limit=None)
+        M.modal_level(ob, depth=h, gravity=g, rank=8, order=3, points=points, limit=None)
         M.eta20(ob.amplitudes, ob.omega, ob.q, t, depth=h, gravity=g, rank=16, nx=128, ny=16)
         el = time.perf_counter() - t3
         cpu = {"value": 1.0 / el, "unit": "records/s", "cores": 1, "kind": "port",
-               "sample": f"one {MODAL_POINTS}-point level + eta20 of the same record by the NumPy oracle "
+               "sample": f"one {points}-point level + eta20 of the same record by the NumPy oracle "
                          "(oracle/modal_oracle.py, pocketfft, one thread)"}
     return {
-        "metric": f"single-direction modal time path: records/s at {MODAL_POINTS} modal points "
+        "metric": f"single-direction modal time path: records/s at {points} modal points "
                   "(eta22 + eta33 GL8, eta20 GL16)",
         "value": args.steps / wall, "unit": "records/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic seeded unidirectional JONSWAP record (numpy default_rng seed 11)",
         "config": {"workload": f"{band.count} samples over 50 T_p, k_p h = 1, h = {h} m, Hs = 1.5 m, gamma = 3.3, "
-                               f"{band.bins.size} bins (99.9% energy), order 3, {MODAL_POINTS} modal points",
+                               f"{band.bins.size} bins (99.9% energy), order 3, {points} modal points",
                    "reduced_grid": {"q_len": int(lp.desc.q_len), "t_len": int(lp.desc.t_len),
-                                    "reference_grid": [MODAL_POINTS, lp.work_count]},
+                                    "reference_grid": [points, lp.work_count]},
                    "eta20_grid": {"q_len": int(d20.q_len), "t_len": int(d20.t_len)}},
         "stage_ms": mean_ms,
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": traffic_from_profiles(dominant, "pmc_traffic_modal.json"),
+                     "frac": achieved / 8000.0, "traffic": traffic,
                      "alg_bytes_per_launch": alg[dominant], "avg_launch_ms": mean_ms[dominant],
-                     **measured_hbm(traffic_from_profiles(dominant, "pmc_traffic_modal.json"), mean_ms[dominant])},
+                     **measured_hbm(traffic, mean_ms[dominant])},
         "api_seconds": api,
         "reference_published": {"seconds_8192_points": [6.78, 6.91], "ladder_median_seconds": 12.07,
-                                "source": "python/docs/random_hos_modal_comparison.md (HOS records, other machine)"},
+                                "source": "python/docs/random_hos_modal_comparison.md (HOS records, other machine)"}
+        if points == MODAL_POINTS else None,
         "cpu_baseline": cpu,
     }
 
@@ -563,6 +567,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="order3: skip the FFT schedule autotuner")
     ap.add_argument("--tune", action="store_true",
                     help="order3: autotune the FFT schedule now instead of using the stored (profiled) one")
+    ap.add_argument("--modal-points", type=int, default=MODAL_POINTS,
+                    help="modal: modal grid points of the level (8192; up to 65536, split into sub-transforms)")
     ap.add_argument("--no-api", action="store_true", help="modal: skip the end-to-end API timings (profiling runs)")
     ap.add_argument("--chunk", type=int, default=INSTANTS,
                     help="order3: instants per plan evaluation (the 64 instants run in chunks of this size)")
