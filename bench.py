@@ -535,18 +535,74 @@ def bench_wit(args):
     }
 
 
-def _wit_roofline(precision: str, quartets_per_s: float):
+def bench_wit_signed(args):
+    """WIT sector 40 (signs + + - -) or 42 (+ + + -) on the reference's frozen DIR4IN01 inputs
+    (512^2): every candidate of the sector's index space (P^2 pairs of pairs or T x M triples and
+    modes), FP64. The CPU leg runs the reference executable built in oracle/_ref on the 50-mode
+    input, whose own timer covers the sector loop only."""
+    import subprocess
+    import tempfile
+
+    import torch
+
+    from green_laplace_wave_reconstruction_amd import formats, wit
+
+    if args.precision != "fp64":
+        raise SystemExit("sectors 40 and 42 run in FP64 only")
+    name = f"n{args.modes:04d}"
+    inp = formats.read_dir4in(ROOT / "tests" / "golden" / f"wit_{name}_input.bin")
+    plan = wit.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                       dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
+    out = cnt = None
+    for _ in range(args.warmup):
+        out, cnt = plan.sector_device(args.sector, inp["amplitudes"], out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, cnt = plan.sector_device(args.sector, inp["amplitudes"], out)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c = cnt.cpu().numpy()
+    rate = c[1] * args.steps / wall
+    cpu = None
+    if not args.no_cpu:
+        from oracle import wit as ow
+
+        with tempfile.TemporaryDirectory() as tmp:
+            r = subprocess.run([str(ow.REF_EXE), str(ROOT / "tests" / "golden" / "wit_n0050_input.bin"),
+                                str(Path(tmp) / "out.bin"), "--sector", str(args.sector)],
+                               check=True, capture_output=True, text=True)
+        done = [ln for ln in (r.stdout + r.stderr).splitlines() if ln.startswith("DIRECTIONAL_CPP_DONE")][-1]
+        f = dict(kv.split("=", 1) for kv in done.split()[1:])
+        cpu = {"value": float(f["kept"]) / float(f["seconds"]), "unit": "candidates/s", "cores": int(f["threads"]),
+               "kind": "reference",
+               "sample": f"the whole sector-{args.sector} loop of the 50-mode input ({int(f['kept'])} kept "
+                         f"candidates in {float(f['seconds']):.2f} s by the loop's own timer): the reference's "
+                         "directional_eta4_openmp.cpp built from its source in oracle/_ref"}
+    return {
+        "metric": f"WIT sector-{args.sector} candidates/s, M={inp['modes'].shape[0]}", "value": rate,
+        "unit": "candidates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": f"reference frozen input paper/order4/inputs/runtime/{name} (copied as a fixture)",
+        "config": {"workload": f"WIT sector {args.sector}, M={inp['modes'].shape[0]}, {inp['nx']}x{inp['ny']}",
+                   "candidates": int(c[0]), "kept": int(c[1]), "near_resonant": int(c[2])},
+        "roofline": _wit_roofline("fp64", rate, key=f"sector{args.sector} fp64", kernel=f"k_wit_signed<{args.sector}>"),
+        "cpu_baseline": cpu,
+    }
+
+
+def _wit_roofline(precision: str, quartets_per_s: float, key=None, kernel="k_wit_quartets"):
     """Vector-ALU roofline of the quartet kernel: FLOP per quartet measured with rocprofv3 (F64/F32
     add, mul and FMA wave instructions x 64 lanes, profiles/wit_m400_valu_counters.md) times the live
     quartet rate, against the 78.6 TFLOP/s FP64 (= non-packed FP32) vector peak."""
     p = ROOT / "profiles" / "wit_flops.json"
     if not p.exists():
         return None
-    rec = json.loads(p.read_text()).get("fp64" if precision == "fp64" else "fp32 eval")
+    rec = json.loads(p.read_text()).get(key or ("fp64" if precision == "fp64" else "fp32 eval"))
     if not rec:
         return None
     achieved = rec["flop_per_quartet"] * quartets_per_s / 1e12
-    return {"bound": "fp64 vector ALU", "kernel": "k_wit_quartets", "achieved": achieved, "peak": 78.6,
+    return {"bound": "fp64 vector ALU", "kernel": kernel, "achieved": achieved, "peak": 78.6,
             "unit": "TFLOP/s", "frac": achieved / 78.6, "traffic": None,
             "flop_per_quartet": rec["flop_per_quartet"]}
 
@@ -562,6 +618,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"))
+    ap.add_argument("--sector", type=int, default=44, choices=(40, 42, 44), help="wit: WIT sector")
     ap.add_argument("--cpu-sample", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="order3: skip the FFT schedule autotuner")
@@ -604,7 +661,7 @@ def main():
         print(json.dumps(bench_eta44(args)))
         return
     if args.workload == "wit":
-        print(json.dumps(bench_wit(args)))
+        print(json.dumps(bench_wit(args) if args.sector == 44 else bench_wit_signed(args)))
         return
     if args.workload == "batch":
         print(json.dumps(bench_batch(args)))
