@@ -863,11 +863,12 @@ __device__ __forceinline__ void unrank_triple(const WitDev& w, long long t, int&
 
 // SECTOR 40: candidate c = minus pair * P + plus pair (signs + + - -);
 // SECTOR 42: c = minus mode * T + plus triple (signs + + + -).
-// Three waves per SIMD (168 VGPRs, 2.6 KB of private memory per lane): the per-candidate
+// Four waves per SIMD (128 VGPRs, 2.8 KB of private memory per lane): the per-candidate
 // evaluation is a long dependent FP64 chain, so waves in flight matter more than spills
-// (M=200 on MI355X: 1.42x (sector 42) and 1.50x (sector 40) over one wave with 348 registers).
+// (M=200 on MI355X: 1.52x (sector 42) and 1.63x (sector 40) over one wave with 348 registers;
+// each step 1 -> 2 -> 3 -> 4 waves was faster).
 template <int SECTOR>
-__global__ __launch_bounds__(WNT, 3) void k_wit_signed(SignedArgs g) {
+__global__ __launch_bounds__(WNT, 4) void k_wit_signed(SignedArgs g) {
   const WitDev& w = g.w;
   const Sgn m{w.md.g, w.md.h, w.md.thr};
   unsigned long long kept = 0, nearc = 0, zo = 0, zi = 0;
