@@ -63,7 +63,8 @@ class CPlanDesc(ctypes.Structure):
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
            "glw_order3_execute", "glw_order4_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
            "glw_wit_sector",
-           "glw_adjoint_execute", "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_batch_create",
+           "glw_adjoint_execute", "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_plan_hold",
+           "glw_batch_create",
            "glw_batch_launch", "glw_batch_destroy", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
            "glw_fft_length_supported", "glw_modal_length_supported", "glw_modal_workspace_size",
            "glw_modal_execute", "glw_modal_stage_times", "glw_eta20_workspace_size", "glw_eta20_execute", "glw_last_error", "glw_version",
@@ -108,6 +109,8 @@ def load_library(path: str | os.PathLike | None = None):
     lib.glw_plan_get_schedule.restype = c_int
     lib.glw_plan_set_schedule.argtypes = [c_void_p, c_char_p]
     lib.glw_plan_set_schedule.restype = c_int
+    lib.glw_plan_hold.argtypes = [c_void_p, c_int32]
+    lib.glw_plan_hold.restype = c_int
     lib.glw_batch_create.argtypes = [POINTER(CBatchItem), c_int32, c_int32, POINTER(c_void_p)]
     lib.glw_batch_create.restype = c_int
     lib.glw_batch_launch.argtypes = [c_void_p, c_void_p]
@@ -116,7 +119,7 @@ def load_library(path: str | os.PathLike | None = None):
     lib.glw_batch_destroy.restype = c_int
     lib.glw_plan_profile.argtypes = [c_void_p, c_int32]
     lib.glw_plan_profile.restype = c_int
-    lib.glw_plan_stage_times.argtypes = [c_void_p, c_int32, c_char_p, c_size_t, _DP, POINTER(c_int32),
+    lib.glw_plan_stage_times.argtypes = [c_void_p, c_int32, c_char_p, c_size_t, _DP, POINTER(c_int32), _DP,
                                          POINTER(c_int32)]
     lib.glw_plan_stage_times.restype = c_int
     lib.glw_plan_buffer.argtypes = [c_void_p, c_char_p, POINTER(c_void_p), POINTER(c_size_t)]
@@ -229,8 +232,9 @@ class DevicePlan:
         self.s0 = torch.from_numpy(s0).to(self.device)
 
     def close(self) -> None:
+        """Destroy the plan (refused while a captured graph holds it: close that first)."""
         if getattr(self, "_handle", None) is not None and self._handle.value:
-            self.lib.glw_plan_destroy(self._handle)
+            _check(self.lib.glw_plan_destroy(self._handle))
             self._handle = None
 
     def __del__(self):
@@ -238,6 +242,10 @@ class DevicePlan:
             self.close()
         except Exception:
             pass
+
+    def hold(self, on: bool = True) -> None:
+        """Hold (or release) the plan for captured graphs that replay it (``glw_plan_hold``)."""
+        _check(self.lib.glw_plan_hold(self._handle, int(bool(on))))
 
     def times_tensor(self, times):
         import torch
@@ -337,16 +345,17 @@ class DevicePlan:
         """Bracket every kernel launch of later runs with HIP events on the run's stream."""
         _check(self.lib.glw_plan_profile(self._handle, int(enable)))
 
-    def stage_times(self) -> dict[str, tuple[float, int]]:
-        """{stage: (total ms, launches)} since the last call (waits for the last event)."""
+    def stage_times(self) -> dict[str, tuple[float, int, float]]:
+        """{stage: (total ms, launches, compulsory HBM bytes)} since the last call (waits for the last event)."""
         n = 64
         names = ctypes.create_string_buffer(4096)
         ms = (c_double * n)()
         launches = (c_int32 * n)()
+        moved = (c_double * n)()
         count = c_int32()
-        _check(self.lib.glw_plan_stage_times(self._handle, n, names, 4096, ms, launches, ctypes.byref(count)))
+        _check(self.lib.glw_plan_stage_times(self._handle, n, names, 4096, ms, launches, moved, ctypes.byref(count)))
         keys = names.value.decode().split(",") if count.value else []
-        return {k: (ms[i], launches[i]) for i, k in enumerate(keys)}
+        return {k: (ms[i], launches[i], moved[i]) for i, k in enumerate(keys)}
 
     def buffer(self, name: str):
         ptr, size = c_void_p(), c_size_t()

@@ -225,9 +225,8 @@ def gl_spectral_surface(coeffs: dict, Lx: float, Ly: float, Nx: int, Ny: int, t,
 
     Returns ``(eta, psi, X, Y, components, audit)`` like the reference.  ``t``
     may also be a 1-D array of instants; every field then gains a leading time
-    axis (one plan, one batched evaluation).  ``precision`` selects the FFT
-    and product arithmetic: ``"fp64"`` (default) or ``"fp32"`` (FP64 GL
-    accumulation).
+    axis (one plan, one batched evaluation).  ``precision`` must be
+    ``"fp64"``: ``"fp32"`` transforms are refused (``_precision``).
     """
     _validate_coefficients(coeffs)
     for name, val in (("Lx", Lx), ("Ly", Ly)):
@@ -265,12 +264,28 @@ def gl_spectral_surface(coeffs: dict, Lx: float, Ly: float, Nx: int, Ny: int, t,
     return out["eta"], out["psi"], X, Y, out, audit
 
 
+#: Why GL reconstructions refuse FP32 transforms (also the C ABI's refusal, glw_plan_create).
+FP32_REFUSAL = ("precision='fp32' is refused for GL reconstructions: FP32 rounding of the intermediate spectra, "
+                "amplified by the GL node factors, exceeds relative L2 1e-5 against FP64 for eta22, eta33, psi33 "
+                "and eta44; use precision='fp64'")
+
+
 def _precision(p: str) -> int:
+    """Arithmetic of the GL transforms and products: FP64 only.
+
+    FP32 transforms lose the spectrum bins ~1e-7 below each spectrum's
+    maximum; the reference keeps them (its floor is 16 eps64 log2 N^2,
+    python/src/green_laplace/_spectral.py:52-64) and the GL node factors
+    amplify them into the result: at the 512^2 order-3 sea state, FP32 with
+    the FP64 floor gives relative L2 ~1e15, with an FP32 floor 1e-2, and even
+    FP32 outer stages alone (inner pair stage in FP64) 1e-5..3e-3 depending
+    on the floor level.
+    """
     if p == "fp64":
         return _plan.PREC_FP64
     if p == "fp32":
-        return _plan.PREC_FP32
-    raise ValueError("precision must be 'fp64' or 'fp32'")
+        raise ValueError(FP32_REFUSAL)
+    raise ValueError("precision must be 'fp64'")
 
 
 def analytic_spectrum_from_real_field(eta1: np.ndarray, *, depth: float, kx: np.ndarray, tolerance: float = 1.0e-11):

@@ -65,9 +65,9 @@ def _signature(draw: BatchState, host: PL.HostPlan) -> tuple:
 class SeaStateBatch:
     """Plans, tuned schedules and the captured graph of an uneven batch.
 
-    ``precision``: "fp64" or "fp32" (FFTs and products; accumulation is FP64
-    in both).  ``times``: the instants evaluated for every state.
-    ``branches``: parallel chains in the graph.
+    ``precision``: "fp64" (FP32 transforms are refused, as by
+    ``gl_spectral_surface``).  ``times``: the instants evaluated for every
+    state.  ``branches``: parallel chains in the graph.
     """
 
     def __init__(self, draws: list[BatchState], *, precision: str = "fp64", times=(0.0,), branches: int = 4,
@@ -77,7 +77,7 @@ class SeaStateBatch:
         import torch
 
         self.precision = precision
-        prec = PL.PREC_FP64 if precision == "fp64" else PL.PREC_FP32
+        prec = api._precision(precision)
         nt = len(times)
         self.entries: list[BatchEntry] = []
         for i, d in enumerate(draws):
@@ -113,8 +113,8 @@ class SeaStateBatch:
         return e.a, e.b
 
     def alg_bytes(self) -> int:
-        """Sum over states of B_alg = T x 4 N^2 x b per instant (b = 16 B FP64, 8 B FP32)."""
-        b = 16 if self.precision == "fp64" else 8
+        """Sum over states of B_alg = T x 4 N^2 x 16 B per instant (the reference's full-grid FP64 transforms)."""
+        b = 16
         return int(sum(REFERENCE_TRANSFORMS[e.draw.order] * 4 * e.host.nx * e.host.ny * b * int(e.times.numel())
                        for e in self.entries))
 

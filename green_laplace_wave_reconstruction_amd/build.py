@@ -19,7 +19,7 @@ UNITS = [
     (CSRC / "glw_batch.hip", [], "glw_batch.o"),
     (CSRC / "glw_modal.hip", [], "glw_modal.o"),
     *[(CSRC / "glw_modal_launch.hip", [f"-DGLW_MODAL_SET={i}"], f"glw_modal_launch{i}.o") for i in range(4)],
-    *[(CSRC / "glw_launch_cols.hip", [f"-DGLW_COL_SET={i}"], f"glw_launch_cols{i}.o") for i in range(4)],
+    *[(CSRC / "glw_launch_cols.hip", [f"-DGLW_COL_SET={i}"], f"glw_launch_cols{i}.o") for i in range(2)],
     *[(CSRC / "glw_launch_rows.hip", [f"-DGLW_ROW_SET={i}"], f"glw_launch_rows{i}.o") for i in range(ROW_SETS)],
 ]
 SOURCES = sorted({u[0] for u in UNITS})

@@ -4,7 +4,10 @@
 // streams (fork/join with events on a private origin stream) and instantiates
 // the graph once; glw_batch_launch replays it with one hipGraphLaunch.  The
 // plans' kernels clear their counters with fill kernels (no memset nodes) and
-// take every table from device memory, so replays are independent.
+// take every table from device memory, so replays are independent.  The batch
+// holds every plan it captured (glw_plan_hold) until it is destroyed, so no
+// later call on those plans can move the scratch the graph replays into;
+// destroy a batch before its plans.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,8 +22,10 @@
 struct glw_batch_s {
   int device = 0;
   hipGraphExec_t exec = nullptr;
+  std::vector<glw_plan*> held;  // distinct plans of the graph, each holding one hold count
   ~glw_batch_s() {
     if (exec) (void)hipGraphExecDestroy(exec);
+    for (glw_plan* p : held) (void)glw_plan_hold(p, 0);
   }
 };
 
@@ -129,6 +134,11 @@ int glw_batch_create(const glw_batch_item* items, int32_t n, int32_t branches, g
   if (!err) err = hip(hipGraphInstantiate(&B->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   if (graph) (void)hipGraphDestroy(graph);
   if (err) return bail(err);
+  for (int i = 0; i < n; ++i)
+    if (std::find(B->held.begin(), B->held.end(), items[i].plan) == B->held.end()) {
+      (void)glw_plan_hold(items[i].plan, 1);
+      B->held.push_back(items[i].plan);
+    }
   *batch = B;
   return GLW_OK;
 }

@@ -1,7 +1,6 @@
-// Column passes, split-row transforms and row synthesis: launcher
-// instantiations for FP64 and FP32 tiles, split by GLW_COL_SET into units the
-// build compiles in parallel (0: FP64 column-inverse + synthesis,
-// 1: FP32 column-inverse, 2: FP64 column-forward + split rows, 3: FP32 ditto).
+// Column passes, split-row transforms and row synthesis: FP64 launcher
+// instantiations, split by GLW_COL_SET into units the build compiles in
+// parallel (0: column-inverse + synthesis, 1: column-forward + split rows).
 #include "glw_launch_impl.hpp"
 
 #ifndef GLW_COL_SET
@@ -44,18 +43,11 @@ int launch_synth(int gx, const SynthArgs& a, int nseq, const void* tw, hipStream
 
 template int launch_colinv<double>(int, const ColInvArgs&, int, const void*, const Exec&, hipStream_t);
 #elif GLW_COL_SET == 1
-template int launch_colinv<float>(int, const ColInvArgs&, int, const void*, const Exec&, hipStream_t);
+template int launch_colfwd<double>(int, const ColFwdArgs&, int, const void*, const Exec&, hipStream_t);
+template int launch_rowinv<double>(int, const RowArgs&, int, const void*, int, void*, hipStream_t);
+template int launch_rowfwd<double>(int, const RowArgs&, int, const void*, int, const void*, hipStream_t);
 #else
-#define GLW_INST(T)                                                                                        \
-  template int launch_colfwd<T>(int, const ColFwdArgs&, int, const void*, const Exec&, hipStream_t);       \
-  template int launch_rowinv<T>(int, const RowArgs&, int, const void*, int, void*, hipStream_t);           \
-  template int launch_rowfwd<T>(int, const RowArgs&, int, const void*, int, const void*, hipStream_t);
-#if GLW_COL_SET == 2
-GLW_INST(double)
-#else
-GLW_INST(float)
-#endif
-#undef GLW_INST
+#error "unknown GLW_COL_SET"
 #endif
 
 }  // namespace glw

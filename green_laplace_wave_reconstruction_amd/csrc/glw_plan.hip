@@ -572,7 +572,7 @@ struct glw_plan_s {
   glw_plan_desc d{};
   int device = 0;
   std::vector<void*> allocs;
-  std::map<int, void*> tw64, tw32;
+  std::map<int, void*> tw64;
   DevTables t1, t2, t3;
   double *omega = nullptr, *damp_o = nullptr, *damp_oi = nullptr;
   uint8_t *mask2 = nullptr, *mask3 = nullptr;
@@ -595,12 +595,15 @@ struct glw_plan_s {
   std::map<std::string, int> stage_kind;
   const char* cur = "";
   hipStream_t cur_stream = nullptr;
-  // scratch of the split row path, grown on demand outside graph capture
+  // scratch of the split row path, grown on demand outside graph capture;
+  // while holds > 0 (captured graphs that replay this plan) it never moves
   void* split_ptr = nullptr;
   size_t split_bytes = 0;
+  int holds = 0;
   // stage timing
   std::vector<hipEvent_t> ev;
   std::vector<const char*> ev_name;
+  std::vector<double> ev_bytes;  // compulsory HBM bytes of each timed launch
   size_t ev_used = 0;
   size_t ev_dropped = 0;
 
@@ -617,6 +620,9 @@ struct glw_plan_s {
       GLW_HIP(hipStreamIsCapturing(P->cur_stream, &cs));
       if (cs != hipStreamCaptureStatusNone)
         return fail(GLW_EINVAL, "split row scratch must be sized before graph capture: execute the plan once first");
+      if (P->holds > 0)
+        return fail(GLW_EINVAL, "split row scratch would move under a captured graph that holds this plan "
+                                "(size it before capture, or release the hold first)");
       if (P->split_ptr) GLW_HIP(hipFree(P->split_ptr));  // hipFree waits for work in flight
       P->split_ptr = nullptr;
       P->split_bytes = 0;
@@ -640,7 +646,7 @@ struct glw_plan_s {
   }
 
   // Bracket one launch with two events (no-op unless profiling is enabled).
-  void mark_begin(const char* name, hipStream_t st) {
+  void mark_begin(const char* name, hipStream_t st, double bytes) {
     cur = name;
     if (ev.empty()) return;
     if (ev_used + 2 > ev.size()) {
@@ -648,6 +654,7 @@ struct glw_plan_s {
       return;
     }
     ev_name[ev_used / 2] = name;
+    ev_bytes[ev_used / 2] = bytes;
     (void)hipEventRecord(ev[ev_used], st);
   }
   void mark_end(hipStream_t st) {
@@ -693,20 +700,14 @@ struct glw_plan_s {
   int twiddles(int n) {
     if (tw64.count(n)) return GLW_OK;
     std::vector<cpx<double>> w(n);
-    std::vector<cpx<float>> wf(n);
     for (int m = 0; m < n; ++m) {
       long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
       w[m] = {(double)cosl(ang), (double)sinl(ang)};
-      wf[m] = {(float)cosl(ang), (float)sinl(ang)};
     }
     cpx<double>* p = nullptr;
     int rc = upload(&p, w.data(), n);
     if (rc) return rc;
     tw64[n] = p;
-    cpx<float>* pf = nullptr;
-    rc = upload(&pf, wf.data(), n);
-    if (rc) return rc;
-    tw32[n] = pf;
     return GLW_OK;
   }
 
@@ -722,7 +723,8 @@ struct glw_plan_s {
   X* buf(const char* name) {
     return (X*)ws.at(name).first;
   }
-  const void* tw(int n, bool fp32) { return fp32 ? tw32.at(n) : tw64.at(n); }
+  // FFT twiddles exp(-2 pi i m / n), FP64 (every GL plan is FP64)
+  const void* tw(int n, bool /*fp32*/) { return tw64.at(n); }
 };
 
 namespace {
@@ -733,7 +735,11 @@ int validate(const glw_plan_desc* d) {
   if (!length_supported(d->nx) || !length_supported(d->ny))
     return fail(GLW_EUNSUPPORTED, "output grid length not supported by the FFT kernels");
   if (d->order < 1 || d->order > 3) return fail(GLW_EINVAL, "order must be 1, 2 or 3");
-  if (d->precision != GLW_PREC_FP64 && d->precision != GLW_PREC_FP32) return fail(GLW_EINVAL, "bad precision");
+  if (d->precision == GLW_PREC_FP32)
+    return fail(GLW_EUNSUPPORTED,
+                "FP32 transforms are refused for GL plans: FP32 rounding of the intermediate spectra, amplified by "
+                "the GL node factors, exceeds relative L2 1e-5 against FP64 (eta22, eta33, psi33, eta44); use FP64");
+  if (d->precision != GLW_PREC_FP64) return fail(GLW_EINVAL, "bad precision");
   if (d->max_instants < 1) return fail(GLW_EINVAL, "max_instants must be positive");
   auto box_ok = [&](const glw_box& b, int gx, int gy) {
     return b.wx >= 1 && b.wy >= 1 && b.wx <= gx && b.wy <= gy;
@@ -785,8 +791,7 @@ int validate(const glw_plan_desc* d) {
 int build(glw_plan* P) {
   const glw_plan_desc& d = P->d;
   const size_t b1 = box_size(d.p1), b2 = box_size(d.p2), b3 = box_size(d.p3);
-  const bool f32 = d.precision == GLW_PREC_FP32;
-  const size_t csz = f32 ? sizeof(cpx<float>) : sizeof(cpx<double>);
+  const size_t csz = sizeof(cpx<double>);
   const size_t zsz = sizeof(cpx<double>);
   const size_t nt = d.max_instants;
   int rc;
@@ -915,6 +920,55 @@ int build(glw_plan* P) {
   return GLW_OK;
 }
 
+// ----------------------------------------------------------------------------
+// Compulsory HBM bytes of one launch on the reduced grids: every input
+// element the launch consumes and every output element it produces, once.
+// Tables that all tasks share (multipliers, damping rows, node factors, masks,
+// twiddles) are excluded, and a spectrum that several tasks read (one
+// spectrum set per `sdiv` tasks) counts once per set.  These are the
+// algorithmic bytes of the roofline: a stage's bytes over its launch time,
+// against the HBM peak.
+// ----------------------------------------------------------------------------
+constexpr double kZ = sizeof(cpx<double>);  // bytes per complex value
+
+double colinv_bytes(const ColInvArgs& a, int gy, int ntask) {
+  bool used[kMaxFields] = {};
+  int nused = 0;
+  for (int f = 0; f < a.nfield; ++f)
+    if (!used[a.fsrc[f]]) {
+      used[a.fsrc[f]] = true;
+      ++nused;
+    }
+  const double sets = std::ceil((double)ntask / a.sdiv);
+  return kZ * (sets * nused * (double)a.wx * a.wy + (double)ntask * a.nfield * gy * a.wx);
+}
+
+double row_bytes(const RowArgs& r, int ntask) {
+  double cols = 0.0;
+  for (int f = 0; f < r.nin; ++f) cols += r.fwx[f];
+  for (int o = 0; o < r.nout; ++o) cols += r.out_per_field ? r.fwx[o] : r.owx;
+  return kZ * ntask * (double)r.gy * cols;
+}
+
+double colfwd_bytes(const ColFwdArgs& c, int gy, int ntask) {
+  return kZ * ntask * (double)c.nsrc * c.owx * ((double)gy + c.owy);
+}
+
+double synth_bytes(const SynthArgs& s, int gx, int nseq) { return kZ * nseq * (double)s.gy * (s.wx + gx); }
+
+// split-path forward row pass: full physical rows in, box columns out
+double rowfwd_bytes(const RowArgs& r, int gx, int ntask) {
+  double cols = 0.0;
+  for (int o = 0; o < r.nout; ++o) cols += gx + (r.out_per_field ? r.fwx[o] : r.owx);
+  return kZ * ntask * (double)r.gy * cols;
+}
+
+// row adjoint: forward inputs and output gradients in, input gradients out
+double rowadj_bytes(const RowArgs& fwd, const RowArgs& gin, const RowArgs& gout, int ntask) {
+  return row_bytes(fwd, ntask) - kZ * ntask * (double)fwd.gy * fwd.nout * fwd.owx + row_bytes(gin, ntask) +
+         row_bytes(gout, ntask);
+}
+
 int fill_zero(uint64_t* p, size_t n, hipStream_t st) {
   if (n == 0) return GLW_OK;
   hipLaunchKernelGGL(k_fill_u64, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
@@ -923,12 +977,12 @@ int fill_zero(uint64_t* p, size_t n, hipStream_t st) {
 }
 
 // Launch with stage-timing brackets; evaluates to the launch's return code.
-#define GLW_STAGE(P, name, st, expr) \
-  ([&]() -> int {                    \
-    (P)->mark_begin(name, st);       \
-    int rc_ = (expr);                \
-    (P)->mark_end(st);               \
-    return rc_;                      \
+#define GLW_STAGE(P, name, st, bytes, expr) \
+  ([&]() -> int {                           \
+    (P)->mark_begin(name, st, (bytes));     \
+    int rc_ = (expr);                       \
+    (P)->mark_end(st);                      \
+    return rc_;                             \
   }())
 
 int launch_plain(hipError_t e) {
@@ -975,7 +1029,7 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
   a.out_nfield = nout;
   a.out_foff = 0;
   a.ostride = sw(box.wx);
-  int rc = GLW_STAGE(P, "synth_colinv", st, launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), P->exec(STAGE_COL), st));
+  int rc = GLW_STAGE(P, "synth_colinv", st, colinv_bytes(a, d.ny, nt), launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), P->exec(STAGE_COL), st));
   if (rc) return rc;
   for (int f = 0; f < nout; ++f) {
     // rows of field f of every instant: sequence t starts at (t*nout + f)*ny*wx
@@ -987,14 +1041,14 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
     s.wx = box.wx;
     s.gy = d.ny;
     s.out = (cpx<double>*)outs[f];
-    if ((rc = GLW_STAGE(P, "synth_row", st, launch_synth(d.nx, s, nt, P->tw(d.nx, false), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "synth_row", st, synth_bytes(s, d.nx, nt), launch_synth(d.nx, s, nt, P->tw(d.nx, false), st)))) return rc;
   }
   return GLW_OK;
 }
 
 int phase(glw_plan* P, const double* s0, const double* times, int nt, hipStream_t st) {
   const int b1 = box_size(P->d.p1);
-  P->mark_begin("phase", st);
+  P->mark_begin("phase", st, kZ * (1.0 + nt) * b1);
   hipLaunchKernelGGL(k_phase, dim3((b1 + 255) / 256, nt), dim3(256), 0, st, (const cpx<double>*)s0, P->omega,
                      times, b1, P->buf<cpx<double>>("st"));
   P->mark_end(st);
@@ -1027,7 +1081,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("tin").first;
     a.out_nfield = 12;
     a.ostride = sw(d.p1.wx);
-    if ((rc = GLW_STAGE(P, "o3_inner_colinv", st, launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_colinv", st, colinv_bytes(a, d.g2y, n_in), launch_colinv<T>(d.g2y, a, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     RowArgs r;
@@ -1045,7 +1099,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.omx0 = d.p2.mx0;
     r.owx = d.p2.wx;
     r.gy = d.g2y;
-    if ((rc = GLW_STAGE(P, "o3_inner_row", st, (launch_row<T, 12, 4, InnerOp>(d.g2x, r, n_in, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st))))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_row", st, row_bytes(r, n_in), (launch_row<T, 12, 4, InnerOp>(d.g2x, r, n_in, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st))))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_in"), (size_t)n_in * 4, st))) return rc;
   {
@@ -1061,7 +1115,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g2x * d.g2y);
     c.out = P->buf<cpx<double>>("sin");
     c.fmax = P->buf<uint64_t>("max_in");
-    if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, colfwd_bytes(c, d.g2y, n_in), launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_et"), (size_t)n_to, st))) return rc;
   {
@@ -1078,7 +1132,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.coef = P->in_coef;
     g.out = P->buf<cpx<double>>("pair");
     g.etmax = P->buf<uint64_t>("max_et");
-    P->mark_begin("o3_inner_acc", st);
+    P->mark_begin("o3_inner_acc", st, kZ * n_to * (4.0 * I + 4.0) * b2);
     hipLaunchKernelGGL(k_acc_inner, dim3((b2 + NT - 1) / NT, n_to), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1102,7 +1156,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out_nfield = 21;
     a.out_foff = 0;
     a.ostride = wmax;
-    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, colinv_bytes(a, d.g3y, n_to), launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     ColInvArgs a = colinv_base(P->buf<cpx<double>>("pair"), 4, 1, d.p2, P->t2);
@@ -1123,7 +1177,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out_nfield = 21;
     a.out_foff = 12;
     a.ostride = wmax;
-    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colinv", st, colinv_bytes(a, d.g3y, n_to), launch_colinv<T>(d.g3y, a, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     RowArgs r;
@@ -1141,7 +1195,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.omx0 = d.p3.mx0;
     r.owx = d.p3.wx;
     r.gy = d.g3y;
-    if ((rc = GLW_STAGE(P, "o3_outer_row", st, (launch_row<T, 21, 4, OuterOp>(d.g3x, r, n_to, P->tw(d.g3x, f32), P->exec(STAGE_ROW), st))))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_row", st, row_bytes(r, n_to), (launch_row<T, 21, 4, OuterOp>(d.g3x, r, n_to, P->tw(d.g3x, f32), P->exec(STAGE_ROW), st))))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_out"), (size_t)n_to * 4, st))) return rc;
   if ((rc = fill_zero(P->buf<uint64_t>("max_combo"), (size_t)n_to * 2, st))) return rc;
@@ -1158,7 +1212,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g3x * d.g3y);
     c.out = P->buf<cpx<double>>("sout");
     c.fmax = P->buf<uint64_t>("max_out");
-    if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, colfwd_bytes(c, d.g3y, n_to), launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     ComboArgs g;
@@ -1169,7 +1223,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.a = P->t3.a;
     g.mask = P->mask3;
     g.cmax = P->buf<uint64_t>("max_combo");
-    P->mark_begin("o3_outer_combo", st);
+    P->mark_begin("o3_outer_combo", st, kZ * n_to * 4.0 * b3);
     hipLaunchKernelGGL(k_combo_outer, dim3((b3 + NT - 1) / NT, n_to), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1188,7 +1242,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.sh = P->out_sh;
     g.coef = P->out_coef;
     g.out = P->buf<cpx<double>>("ep33");
-    P->mark_begin("o3_outer_acc", st);
+    P->mark_begin("o3_outer_acc", st, kZ * nt * (4.0 * O + 2.0) * b3);
     hipLaunchKernelGGL(k_acc_outer, dim3((b3 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1258,11 +1312,11 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("e22_t").first;
     a.out_nfield = 8;
     a.ostride = sw(d.p1.wx);
-    if ((rc = GLW_STAGE(P, "o2_eta_colinv", st, launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
-    if ((rc = GLW_STAGE(P, "o2_eta_row", st, (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
+    if ((rc = GLW_STAGE(P, "o2_eta_colinv", st, colinv_bytes(a, d.g2y, nt * J), launch_colinv<T>(d.g2y, a, nt * J, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o2_eta_row", st, row_bytes(row_args("e22_t", 8, "e22_u", 2), nt * J), (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("e22_max"), (size_t)nt * J * 2, st))) return rc;
-    if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
+    if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, colfwd_bytes(fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), d.g2y, nt * J), launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
                                P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
     AccEta22Args g;
@@ -1276,7 +1330,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.sh = P->e22_sh;
     g.coef = P->e22_coef;
     g.out = P->buf<cpx<double>>("ep22");
-    P->mark_begin("o2_eta_acc", st);
+    P->mark_begin("o2_eta_acc", st, kZ * nt * (2.0 * J + 1.0) * b2);
     hipLaunchKernelGGL(k_acc_eta22, dim3((b2 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1303,11 +1357,11 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
       a.out_nfield = 16;
       a.out_foff = 8 * ch;
       a.ostride = sw(d.p1.wx);
-      if ((rc = GLW_STAGE(P, "o2_psi_colinv", st, launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
+      if ((rc = GLW_STAGE(P, "o2_psi_colinv", st, colinv_bytes(a, d.g2y, nt * 4), launch_colinv<T>(d.g2y, a, nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
     }
-    if ((rc = GLW_STAGE(P, "o2_psi_row", st, (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
+    if ((rc = GLW_STAGE(P, "o2_psi_row", st, row_bytes(row_args("p22_t", 16, "p22_u", 8), nt * 4), (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
+    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, colfwd_bytes(fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), d.g2y, nt * 4), launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   // ---- psi22 Taylor contact: tasks t
@@ -1323,10 +1377,10 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     a.out = P->ws.at("c22_t").first;
     a.out_nfield = 2;
     a.ostride = sw(d.p1.wx);
-    if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
-    if ((rc = GLW_STAGE(P, "o2_contact_row", st, (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
+    if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, colinv_bytes(a, d.g2y, nt), launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
+    if ((rc = GLW_STAGE(P, "o2_contact_row", st, row_bytes(row_args("c22_t", 2, "c22_u", 1), nt), (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
+    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, colfwd_bytes(fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), d.g2y, nt), launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   {
@@ -1340,7 +1394,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     g.coef = P->p22_coef;
     g.sign = P->p22_sign;
     g.out = P->buf<cpx<double>>("ep22");
-    P->mark_begin("o2_psi_acc", st);
+    P->mark_begin("o2_psi_acc", st, kZ * nt * (32.0 + 1.0 + 1.0) * b2);
     hipLaunchKernelGGL(k_acc_psi22, dim3((b2 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1384,7 +1438,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     a.out = P->ws.at("o4_pt").first;
     a.out_nfield = 16;
     a.ostride = sw(d.p1.wx);
-    if ((rc2 = GLW_STAGE(P, "o4_pair_colinv", st, launch_colinv<T>(d.g2y, a, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
+    if ((rc2 = GLW_STAGE(P, "o4_pair_colinv", st, colinv_bytes(a, d.g2y, ntask), launch_colinv<T>(d.g2y, a, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
       return rc2;
     RowArgs r;
     memset(&r, 0, sizeof(r));
@@ -1401,8 +1455,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     r.omx0 = d.p2.mx0;
     r.owx = d.p2.wx;
     r.gy = d.g2y;
-    if ((rc2 = GLW_STAGE(P, "o4_pair_row", st,
-                         (launch_row<T, 16, 6, Pair44Op>(d.g2x, r, ntask, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))))
+    if ((rc2 = GLW_STAGE(P, "o4_pair_row", st, row_bytes(r, ntask), (launch_row<T, 16, 6, Pair44Op>(d.g2x, r, ntask, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))))
       return rc2;
     if ((rc2 = fill_zero(P->buf<uint64_t>("o4_pm"), (size_t)ntask * 6, st))) return rc2;
     ColFwdArgs c;
@@ -1417,7 +1470,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.scale = nn / ((double)d.g2x * d.g2y);
     c.out = P->buf<cpx<double>>("o4_ps");
     c.fmax = P->buf<uint64_t>("o4_pm");
-    if ((rc2 = GLW_STAGE(P, "o4_pair_colfwd", st, launch_colfwd<T>(d.g2y, c, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
+    if ((rc2 = GLW_STAGE(P, "o4_pair_colfwd", st, colfwd_bytes(c, d.g2y, ntask), launch_colfwd<T>(d.g2y, c, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
       return rc2;
     if ((rc2 = fill_zero(P->buf<uint64_t>(setmax), (size_t)nset * 5, st))) return rc2;
     Acc4Args g;
@@ -1435,7 +1488,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     g.coef = P->o4_pco;
     g.out = P->buf<cpx<double>>(set);
     g.omax = P->buf<uint64_t>(setmax);
-    P->mark_begin("o4_pair_acc", st);
+    P->mark_begin("o4_pair_acc", st, kZ * nset * (6.0 * J + 5.0) * b2);
     hipLaunchKernelGGL(k_acc_pair44, dim3((b2 + NT - 1) / NT, nset), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1461,7 +1514,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     a.out_nfield = 31;
     a.out_foff = 0;
     a.ostride = w12;
-    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, a, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, colinv_bytes(a, d.g3y, ntask), launch_colinv<T>(d.g3y, a, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     ColInvArgs b = colinv_base(P->buf<cpx<double>>("o4_set3"), 5, 1, d.p2, P->t2);
     b.fmax = P->buf<uint64_t>("o4_max3");
@@ -1473,7 +1526,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     b.out_nfield = 31;
     b.out_foff = 16;
     b.ostride = w12;
-    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, launch_colinv<T>(d.g3y, b, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "o4_third_colinv", st, colinv_bytes(b, d.g3y, ntask), launch_colinv<T>(d.g3y, b, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     RowArgs r;
     memset(&r, 0, sizeof(r));
@@ -1490,8 +1543,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     r.omx0 = d.p3.mx0;
     r.owx = d.p3.wx;
     r.gy = d.g3y;
-    if ((rc = GLW_STAGE(P, "o4_third_row", st,
-                        (launch_row<T, 31, 4, Third44Op>(d.g3x, r, ntask, P->tw(d.g3x, f32), P->exec(STAGE_ROW), st)))))
+    if ((rc = GLW_STAGE(P, "o4_third_row", st, row_bytes(r, ntask), (launch_row<T, 31, 4, Third44Op>(d.g3x, r, ntask, P->tw(d.g3x, f32), P->exec(STAGE_ROW), st)))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_tm"), (size_t)ntask * 4, st))) return rc;
     ColFwdArgs c;
@@ -1506,7 +1558,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.scale = nn / ((double)d.g3x * d.g3y);
     c.out = P->buf<cpx<double>>("o4_ts");
     c.fmax = P->buf<uint64_t>("o4_tm");
-    if ((rc = GLW_STAGE(P, "o4_third_colfwd", st, launch_colfwd<T>(d.g3y, c, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "o4_third_colfwd", st, colfwd_bytes(c, d.g3y, ntask), launch_colfwd<T>(d.g3y, c, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_thm"), (size_t)nt * J * 3, st))) return rc;
     Acc4Args g;
@@ -1524,7 +1576,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     g.coef = P->o4_tco;
     g.out = P->buf<cpx<double>>("o4_th");
     g.omax = P->buf<uint64_t>("o4_thm");
-    P->mark_begin("o4_third_acc", st);
+    P->mark_begin("o4_third_acc", st, kZ * nt * J * (4.0 * J + 3.0) * b3);
     hipLaunchKernelGGL(k_acc_third44, dim3((b3 + NT - 1) / NT, nt * J), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1541,7 +1593,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     a.out_nfield = 45;
     a.out_foff = 0;
     a.ostride = w123;
-    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, a, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, colinv_bytes(a, o.g4y, ntask), launch_colinv<T>(o.g4y, a, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     ColInvArgs b = colinv_base(P->buf<cpx<double>>("o4_set4"), 5, 1, d.p2, P->t2);
     b.fmax = P->buf<uint64_t>("o4_max4");
@@ -1553,7 +1605,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     b.out_nfield = 45;
     b.out_foff = 19;
     b.ostride = w123;
-    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, b, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, colinv_bytes(b, o.g4y, ntask), launch_colinv<T>(o.g4y, b, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     static const int third_src[8] = {0, 0, 0, 1, 1, 1, 1, 2};
     static const int third_codes[8] = {M_ONE, M_I_QX, M_I_QY, M_I_QX, M_I_QY, M_A, M_Q2, M_A};
@@ -1567,7 +1619,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c3.out_nfield = 45;
     c3.out_foff = 37;
     c3.ostride = w123;
-    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, launch_colinv<T>(o.g4y, c3, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colinv", st, colinv_bytes(c3, o.g4y, ntask), launch_colinv<T>(o.g4y, c3, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     RowArgs r;
     memset(&r, 0, sizeof(r));
@@ -1586,11 +1638,9 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     r.owx = o.p4.wx;
     r.gy = o.g4y;
     if (o.forcing_terms == 1) {  // the 3+1 partition only (split runs)
-      if ((rc = GLW_STAGE(P, "o4_quartic_row", st,
-                          (launch_row<T, 45, 2, Quartic31Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st)))))
+      if ((rc = GLW_STAGE(P, "o4_quartic_row", st, row_bytes(r, ntask), (launch_row<T, 45, 2, Quartic31Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st)))))
         return rc;
-    } else if ((rc = GLW_STAGE(P, "o4_quartic_row", st,
-                               (launch_row<T, 45, 2, Quartic44Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st))))) {
+    } else if ((rc = GLW_STAGE(P, "o4_quartic_row", st, row_bytes(r, ntask), (launch_row<T, 45, 2, Quartic44Op>(o.g4x, r, ntask, P->tw(o.g4x, f32), P->exec(STAGE_ROW), st))))) {
       return rc;
     }
     if ((rc = fill_zero(P->buf<uint64_t>("o4_qm"), (size_t)ntask * 2, st))) return rc;
@@ -1606,7 +1656,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.scale = nn / ((double)o.g4x * o.g4y);
     c.out = P->buf<cpx<double>>("o4_qs");
     c.fmax = P->buf<uint64_t>("o4_qm");
-    if ((rc = GLW_STAGE(P, "o4_quartic_colfwd", st, launch_colfwd<T>(o.g4y, c, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "o4_quartic_colfwd", st, colfwd_bytes(c, o.g4y, ntask), launch_colfwd<T>(o.g4y, c, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_outm"), (size_t)nt, st))) return rc;
     Acc4Args g;
@@ -1624,7 +1674,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     g.coef = P->o4_qco;
     g.out = P->buf<cpx<double>>("o4_out");
     g.omax = P->buf<uint64_t>("o4_outm");
-    P->mark_begin("o4_quartic_acc", st);
+    P->mark_begin("o4_quartic_acc", st, kZ * nt * (2.0 * J + 1.0) * b4);
     hipLaunchKernelGGL(k_acc_quartic44, dim3((b4 + NT - 1) / NT, nt), dim3(NT), 0, st, g);
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
@@ -1637,7 +1687,7 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
 }
 
 int order4_dispatch(glw_plan* P, int nt, double* a, double* /*unused*/, hipStream_t st) {
-  return P->d.precision == GLW_PREC_FP32 ? order4_chunk<float>(P, nt, a, st) : order4_chunk<double>(P, nt, a, st);
+  return order4_chunk<double>(P, nt, a, st);
 }
 
 typedef int (*chunk_fn)(glw_plan*, int, double*, double*, hipStream_t);
@@ -1676,10 +1726,10 @@ int linear_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
 }
 
 int order3_dispatch(glw_plan* P, int nt, double* a, double* b, hipStream_t st) {
-  return P->d.precision == GLW_PREC_FP32 ? order3_chunk<float>(P, nt, a, b, st) : order3_chunk<double>(P, nt, a, b, st);
+  return order3_chunk<double>(P, nt, a, b, st);
 }
 int order2_dispatch(glw_plan* P, int nt, double* a, double* b, hipStream_t st) {
-  return P->d.precision == GLW_PREC_FP32 ? order2_chunk<float>(P, nt, a, b, st) : order2_chunk<double>(P, nt, a, b, st);
+  return order2_chunk<double>(P, nt, a, b, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -1859,11 +1909,11 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     ra.ostride = U.wx;
     ra.omx0 = U.mx0;
     ra.owx = U.wx;
-    if ((rc = GLW_STAGE(P, "adj_analysis_row", st, launch_rowfwd<double>(d.nx, ra, 1, P->tw(d.nx, false), 0, gfield, st))))
+    if ((rc = GLW_STAGE(P, "adj_analysis_row", st, rowfwd_bytes(ra, d.nx, 1), launch_rowfwd<double>(d.nx, ra, 1, P->tw(d.nx, false), 0, gfield, st))))
       return rc;
     ColFwdArgs ca = col_dft(B("adj_rows"), 1, 0, 1, U.wx, U, P->buf<uint8_t>("adj_onesU"), B("adj_u"));
     ca.scale = d.depth / nn;
-    if ((rc = GLW_STAGE(P, "adj_analysis_col", st, launch_colfwd<double>(d.ny, ca, 1, P->tw(d.ny, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_analysis_col", st, colfwd_bytes(ca, d.ny, 1), launch_colfwd<double>(d.ny, ca, 1, P->tw(d.ny, false), P->exec(STAGE_COL), st))))
       return rc;
   }
   cpx<double>* gs = B("adj_gs");
@@ -1888,24 +1938,24 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     g.sh = P->e22_sh;
     g.coef = P->e22_coef;
     g.out = nullptr;
-    P->mark_begin("adj_o2_acc", st);
+    P->mark_begin("adj_o2_acc", st, 0.0);
     hipLaunchKernelGGL(k_adj_acc_eta22, dim3((b2 + NT - 1) / NT, J), dim3(NT), 0, st, g, B("adj_e22"), P->mask2,
                        B("adj_s22"));
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
     const double up = nn / ((double)d.g2x * d.g2y);
     ColInvArgs a = col_idft(B("adj_s22"), 2, d.p2, P->t2, up, B("adj_u22"));
-    if ((rc = GLW_STAGE(P, "adj_o2_colinv", st, launch_colinv<double>(d.g2y, a, J, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_o2_colinv", st, colinv_bytes(a, d.g2y, J), launch_colinv<double>(d.g2y, a, J, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
       return rc;
     glw_box in_boxes[8], g_boxes[2] = {d.p2, d.p2};
     for (int f = 0; f < 8; ++f) in_boxes[f] = d.p1;
     const RowArgs fwd = row_desc(P->ws.at("e22_t").first, 8, sw(d.p1.wx), in_boxes, d.g2y);
     const RowArgs gin = row_desc(B("adj_u22"), 2, sw(d.p2.wx), g_boxes, d.g2y);
     const RowArgs gout = row_grad_out(B("adj_t22"), 8, sw(d.p1.wx), in_boxes, d.g2y);
-    if ((rc = GLW_STAGE(P, "adj_o2_row", st, (launch_row_adjoint<double, 8, 2, Eta22Op>(d.g2x, fwd, gin, gout, J, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
+    if ((rc = GLW_STAGE(P, "adj_o2_row", st, rowadj_bytes(fwd, gin, gout, J), (launch_row_adjoint<double, 8, 2, Eta22Op>(d.g2x, fwd, gin, gout, J, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
       return rc;
     const ColFwdArgs c = col_dft(B("adj_t22"), 8, 0, 8, sw(d.p1.wx), d.p1, ones1, B("adj_x22"));
-    if ((rc = GLW_STAGE(P, "adj_o2_colfwd", st, launch_colfwd<double>(d.g2y, c, J, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_o2_colfwd", st, colfwd_bytes(c, d.g2y, J), launch_colfwd<double>(d.g2y, c, J, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
       return rc;
     ColInvArgs fa = colinv_base(stp, 1, J, d.p1, P->t1);
     fa.damp = P->e22_damp;
@@ -1934,25 +1984,25 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     g.sh = P->out_sh;
     g.coef = P->out_coef;
     g.out = nullptr;
-    P->mark_begin("adj_o3_outer_acc", st);
+    P->mark_begin("adj_o3_outer_acc", st, 0.0);
     hipLaunchKernelGGL(k_adj_acc_outer, dim3((b3 + NT - 1) / NT, O), dim3(NT), 0, st, g, B("adj_e33"), B("adj_s33"));
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
     ColInvArgs a = col_idft(B("adj_s33"), 2, d.p3, P->t3, up3, B("adj_u33"));
-    if ((rc = GLW_STAGE(P, "adj_o3_outer_colinv", st, launch_colinv<double>(d.g3y, a, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_colinv", st, colinv_bytes(a, d.g3y, O), launch_colinv<double>(d.g3y, a, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
       return rc;
     glw_box fb[21], g3b[2] = {d.p3, d.p3};
     for (int f = 0; f < 21; ++f) fb[f] = f < 12 ? d.p1 : d.p2;
     const RowArgs fwd = row_desc(P->ws.at("tout").first, 21, wmax, fb, d.g3y);
     const RowArgs gin = row_desc(B("adj_u33"), 2, sw(d.p3.wx), g3b, d.g3y);
     const RowArgs gout = row_grad_out(B("adj_t33"), 21, wmax, fb, d.g3y);
-    if ((rc = GLW_STAGE(P, "adj_o3_outer_row", st, (launch_row_adjoint<double, 21, 2, OuterOp>(d.g3x, fwd, gin, gout, O, P->tw(d.g3x, false), P->exec(STAGE_ROW), st)))))
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_row", st, rowadj_bytes(fwd, gin, gout, O), (launch_row_adjoint<double, 21, 2, OuterOp>(d.g3x, fwd, gin, gout, O, P->tw(d.g3x, false), P->exec(STAGE_ROW), st)))))
       return rc;
     const ColFwdArgs ca = col_dft(B("adj_t33"), 21, 0, 12, wmax, d.p1, ones1, B("adj_x33a"));
-    if ((rc = GLW_STAGE(P, "adj_o3_outer_colfwd", st, launch_colfwd<double>(d.g3y, ca, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_colfwd", st, colfwd_bytes(ca, d.g3y, O), launch_colfwd<double>(d.g3y, ca, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
       return rc;
     const ColFwdArgs cb = col_dft(B("adj_t33"), 21, 12, 9, wmax, d.p2, ones2, B("adj_x33b"));
-    if ((rc = GLW_STAGE(P, "adj_o3_outer_colfwd", st, launch_colfwd<double>(d.g3y, cb, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_o3_outer_colfwd", st, colfwd_bytes(cb, d.g3y, O), launch_colfwd<double>(d.g3y, cb, O, P->tw(d.g3y, false), P->exec(STAGE_COL), st))))
       return rc;
     {  // first-order fields of the outer nodes -> parent spectrum
       static const int codes[12] = {M_ONE,  M_I_QX, M_I_QY,     M_QX_NU,   M_QY_NU, M_MI_NU,
@@ -2000,22 +2050,22 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     gi.coef = P->in_coef;
     gi.out = nullptr;
     gi.etmax = nullptr;
-    P->mark_begin("adj_o3_inner_acc", st);
+    P->mark_begin("adj_o3_inner_acc", st, 0.0);
     hipLaunchKernelGGL(k_adj_acc_inner, dim3((b2 + NT - 1) / NT, O * I), dim3(NT), 0, st, gi, B("adj_pair"), B("adj_sin"));
     P->mark_end(st);
     GLW_HIP(hipGetLastError());
     ColInvArgs ai = col_idft(B("adj_sin"), 4, d.p2, P->t2, up2, B("adj_uin"));
-    if ((rc = GLW_STAGE(P, "adj_o3_inner_colinv", st, launch_colinv<double>(d.g2y, ai, O * I, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_o3_inner_colinv", st, colinv_bytes(ai, d.g2y, O * I), launch_colinv<double>(d.g2y, ai, O * I, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
       return rc;
     glw_box ib[12], g2b[4] = {d.p2, d.p2, d.p2, d.p2};
     for (int f = 0; f < 12; ++f) ib[f] = d.p1;
     const RowArgs ifwd = row_desc(P->ws.at("tin").first, 12, sw(d.p1.wx), ib, d.g2y);
     const RowArgs igin = row_desc(B("adj_uin"), 4, sw(d.p2.wx), g2b, d.g2y);
     const RowArgs igout = row_grad_out(B("adj_tin"), 12, sw(d.p1.wx), ib, d.g2y);
-    if ((rc = GLW_STAGE(P, "adj_o3_inner_row", st, (launch_row_adjoint<double, 12, 4, InnerOp>(d.g2x, ifwd, igin, igout, O * I, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
+    if ((rc = GLW_STAGE(P, "adj_o3_inner_row", st, rowadj_bytes(ifwd, igin, igout, O * I), (launch_row_adjoint<double, 12, 4, InnerOp>(d.g2x, ifwd, igin, igout, O * I, P->tw(d.g2x, false), P->exec(STAGE_ROW), st)))))
       return rc;
     const ColFwdArgs ci = col_dft(B("adj_tin"), 12, 0, 12, sw(d.p1.wx), d.p1, ones1, B("adj_xin"));
-    if ((rc = GLW_STAGE(P, "adj_o3_inner_colfwd", st, launch_colfwd<double>(d.g2y, ci, O * I, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
+    if ((rc = GLW_STAGE(P, "adj_o3_inner_colfwd", st, colfwd_bytes(ci, d.g2y, O * I), launch_colfwd<double>(d.g2y, ci, O * I, P->tw(d.g2y, false), P->exec(STAGE_COL), st))))
       return rc;
     static const int icodes[12] = {M_ONE, M_NU, M_NU2, M_NU3, M_QX_NU, M_QY_NU, M_QX, M_QY, M_Q2_NU, M_Q2, M_NU_QX, M_NU_QY};
     ColInvArgs fi = colinv_base(stp, 1, O * I, d.p1, P->t1);
@@ -2056,11 +2106,13 @@ int enable_events(glw_plan* P, bool on) {
   for (hipEvent_t e : P->ev) (void)hipEventDestroy(e);
   P->ev.clear();
   P->ev_name.clear();
+  P->ev_bytes.clear();
   P->ev_used = P->ev_dropped = 0;
   if (!on) return GLW_OK;
   const size_t n = 8192;
   P->ev.resize(n);
   P->ev_name.resize(n / 2);
+  P->ev_bytes.resize(n / 2);
   for (size_t i = 0; i < n; ++i) GLW_HIP(hipEventCreate(&P->ev[i]));
   return GLW_OK;
 }
@@ -2127,6 +2179,7 @@ int glw_execute_kind(glw_plan* P, int kind, const double* s0, const double* time
 bool glw_plan_profiling(const glw_plan* P) { return P && !P->ev.empty(); }
 
 int glw_plan_release_scratch(glw_plan* P) {
+  if (P->holds > 0) return fail(GLW_EINVAL, "split row scratch is held by a captured graph");
   if (P->split_ptr) GLW_HIP(hipFree(P->split_ptr));
   P->split_ptr = nullptr;
   P->split_bytes = 0;
@@ -2168,6 +2221,7 @@ int glw_plan_create(const glw_plan_desc* desc, glw_plan** plan) {
 }
 
 int glw_plan_destroy(glw_plan* plan) {
+  if (plan && plan->holds > 0) return fail(GLW_EINVAL, "plan is held by a captured graph: destroy the graph first");
   delete plan;
   return GLW_OK;
 }
@@ -2196,6 +2250,7 @@ int glw_plan_autotune(glw_plan* plan, int32_t kind, const double* s0, const doub
                       double* b, int32_t reps, void* stream) {
   if (!plan) return fail(GLW_EINVAL, "null plan");
   if (reps < 1 || reps > 64) return fail(GLW_EINVAL, "reps must be 1..64");
+  if (plan->holds > 0) return fail(GLW_EINVAL, "autotuning would resize the scratch of a plan a captured graph holds");
   if (nt < 1) return fail(GLW_EINVAL, "autotuning needs at least one instant");
   hipStream_t st = (hipStream_t)stream;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -2222,6 +2277,17 @@ int glw_adjoint_execute(glw_plan* plan, const double* s0, const double* times, i
   hipStream_t st = (hipStream_t)stream;
   plan->cur_stream = st;
   return adjoint_run(plan, s0, times, orders, (const cpx<double>*)grad_field, (cpx<double>*)grad_s0, recompute, st);
+}
+
+int glw_plan_hold(glw_plan* plan, int32_t hold) {
+  if (!plan) return fail(GLW_EINVAL, "null plan");
+  if (hold) {
+    ++plan->holds;
+    return GLW_OK;
+  }
+  if (plan->holds < 1) return fail(GLW_EINVAL, "plan hold count is already zero");
+  --plan->holds;
+  return GLW_OK;
 }
 
 int glw_plan_get_schedule(glw_plan* plan, char* text, size_t len) {
@@ -2265,13 +2331,14 @@ int glw_plan_profile(glw_plan* plan, int32_t enable) {
 }
 
 int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_len, double* ms,
-                         int32_t* launches, int32_t* count) {
+                         int32_t* launches, double* bytes, int32_t* count) {
   if (!plan || !names || !ms || !launches || !count || max < 1 || names_len < 1) return fail(GLW_EINVAL, "null argument");
   *count = 0;
   names[0] = 0;
   if (plan->ev.empty()) return fail(GLW_EINVAL, "profiling is not enabled on this plan");
   std::vector<std::string> order;
   std::map<std::string, std::pair<double, int>> agg;
+  std::map<std::string, double> moved;
   if (plan->ev_used) GLW_HIP(hipEventSynchronize(plan->ev[plan->ev_used - 1]));
   for (size_t i = 0; i + 1 < plan->ev_used; i += 2) {
     float t = 0.f;
@@ -2280,6 +2347,7 @@ int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_
     if (!agg.count(nm)) order.push_back(nm);
     agg[nm].first += t;
     agg[nm].second += 1;
+    moved[nm] += plan->ev_bytes[i / 2];
   }
   std::string csv;
   int k = 0;
@@ -2287,6 +2355,7 @@ int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_
     if (k >= max) break;
     ms[k] = agg[nm].first;
     launches[k] = agg[nm].second;
+    if (bytes) bytes[k] = moved[nm];
     csv += (k ? "," : "") + nm;
     ++k;
   }

@@ -98,6 +98,20 @@ class GLElevationModel:
             ga, gb = backward()
         self._cap = {"a": a_in, "b": b_in, "g": g_in, "fwd": fwd_graph, "bwd": bwd_graph, "eta": eta, "ga": ga,
                      "gb": gb}
+        self.plan.hold(True)  # the graphs replay into the plan's workspace: nothing of it may move
+
+    def close(self) -> None:
+        """Drop the captured graphs (releasing the plan's hold) and destroy the plan."""
+        if self._cap is not None:
+            self._cap = None
+            self.plan.hold(False)
+        self.plan.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
     def _elevation_graph(self, a, b):
         cap = self._cap

@@ -25,8 +25,13 @@
  * (mx0.., my0..) of the ny x nx FFT grid.  Outputs are [nt][ny][nx] complex128
  * physical fields, row = y, already scaled to metres (eta) and m^2/s (psi),
  * identical in meaning to the reference's *_plus components.  Execute calls
- * allocate nothing, never synchronise the host, and run on `stream`
- * (a hipStream_t, or NULL for the default stream).  A plan must not be used
+ * never synchronise the host and run on `stream` (a hipStream_t, or NULL for
+ * the default stream).  They allocate device memory in two cases only: the
+ * split-row scratch, grown by the first eager call whose FFT schedule needs
+ * it (refused under graph capture, and while a graph holds the plan,
+ * glw_plan_hold), and the adjoint workspace on the first
+ * glw_adjoint_execute.  Neither frees memory that an earlier call used
+ * while a hold is active.  A plan must not be used
  * by two host threads or two streams at once.  Every function returns 0 on
  * success and a nonzero code on error; glw_last_error() returns the text
  * (thread-local).
@@ -47,7 +52,11 @@ extern "C" {
 #define GLW_EUNSUPPORTED 3
 
 #define GLW_PREC_FP64 0 /* FFTs and products in FP64 */
-#define GLW_PREC_FP32 1 /* FFTs and products in FP32, FP64 accumulation */
+/* WIT only (glw_wit_desc.precision).  glw_plan_create refuses it with
+ * GLW_EUNSUPPORTED: FP32 transforms of the GL graphs miss relative L2 1e-5
+ * against FP64 for eta22, eta33, psi33 and eta44 (the node factors amplify
+ * spectrum bins ~1e-7 below their maximum, which FP32 cannot resolve). */
+#define GLW_PREC_FP32 1
 
 /* Evaluation kinds (glw_plan_autotune, glw_batch_item). */
 #define GLW_KIND_LINEAR 1 /* eta11, psi11 */
@@ -198,11 +207,15 @@ int glw_wit_sector(glw_wit* wit, int32_t sector, const double* amplitudes, int32
  * bracketed by two events on the execute stream.  glw_plan_stage_times waits
  * for the last event, aggregates elapsed times per stage name and resets the
  * pool: names is filled with "name,name,..." (NUL-terminated), ms[i] and
- * launches[i] receive the total time and launch count of stage i, *count the
- * number of stages (at most max). */
+ * launches[i] receive the total time and launch count of stage i, bytes[i]
+ * (when bytes is not NULL) the stage's compulsory HBM bytes summed over its
+ * launches -- every input element it consumes and output element it
+ * produces on the reduced grids, once; tables shared by all tasks excluded
+ * (0 for the adjoint's accumulations) -- and *count the number of stages
+ * (at most max). */
 int glw_plan_profile(glw_plan* plan, int32_t enable);
 int glw_plan_stage_times(glw_plan* plan, int32_t max, char* names, size_t names_len, double* ms,
-                         int32_t* launches, int32_t* count);
+                         int32_t* launches, double* bytes, int32_t* count);
 
 /* Adjoint (no reference counterpart: the reference has no gradients).  For
  * a real loss L of the elevation fields at one instant times[0], given
@@ -235,6 +248,13 @@ int glw_plan_autotune(glw_plan* plan, int32_t kind, const double* s0, const doub
  * plans of the same grids. */
 int glw_plan_get_schedule(glw_plan* plan, char* text, size_t len);
 int glw_plan_set_schedule(glw_plan* plan, const char* text);
+/* Hold (hold != 0) or release (hold == 0) a plan for captured graphs that
+ * replay it (no reference counterpart: the reference has no graphs).  While
+ * the hold count is positive, an execute that would need a larger split-row
+ * scratch, and glw_plan_autotune, fail with GLW_EINVAL instead of moving
+ * memory the graphs use, and glw_plan_destroy refuses the plan.
+ * glw_batch_create holds each plan it captures until glw_batch_destroy. */
+int glw_plan_hold(glw_plan* plan, int32_t hold);
 
 /* Batched evaluation of many plans in one HIP graph (no reference
  * counterpart: the reference evaluates one sea state per call of
