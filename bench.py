@@ -604,11 +604,8 @@ def bench_wit(args):
 def bench_wit_signed(args):
     """WIT sector 40 (signs + + - -) or 42 (+ + + -) on the reference's frozen DIR4IN01 inputs
     (512^2): every candidate of the sector's index space (P^2 pairs of pairs or T x M triples and
-    modes), FP64. The CPU leg runs the reference executable built in oracle/_ref on the 50-mode
-    input, whose own timer covers the sector loop only."""
-    import subprocess
-    import tempfile
-
+    modes), FP64. The CPU leg runs the C++/OpenMP restatement (oracle/wit_oracle.cpp) on a bounded
+    slice of the same input's outer candidate loop."""
     import torch
 
     from green_laplace_wave_reconstruction_amd import formats, wit
@@ -634,17 +631,22 @@ def bench_wit_signed(args):
     if not args.no_cpu:
         from oracle import wit as ow
 
-        with tempfile.TemporaryDirectory() as tmp:
-            r = subprocess.run([str(ow.REF_EXE), str(ROOT / "tests" / "golden" / "wit_n0050_input.bin"),
-                                str(Path(tmp) / "out.bin"), "--sector", str(args.sector)],
-                               check=True, capture_output=True, text=True)
-        done = [ln for ln in (r.stdout + r.stderr).splitlines() if ln.startswith("DIRECTIONAL_CPP_DONE")][-1]
-        f = dict(kv.split("=", 1) for kv in done.split()[1:])
-        cpu = {"value": float(f["kept"]) / float(f["seconds"]), "unit": "candidates/s", "cores": int(f["threads"]),
-               "kind": "reference",
-               "sample": f"the whole sector-{args.sector} loop of the 50-mode input ({int(f['kept'])} kept "
-                         f"candidates in {float(f['seconds']):.2f} s by the loop's own timer): the reference's "
-                         "directional_eta4_openmp.cpp built from its source in oracle/_ref"}
+        m = inp["modes"].shape[0]
+        cores = len(os.sched_getaffinity(0))
+        npair, ntrip = m * (m + 1) // 2, m * (m + 1) * (m + 2) // 6
+        inner = npair if args.sector == 40 else ntrip  # candidates per outer loop index
+        # bounded sample: the first n outer indices, ~10 s of CPU work at ~18 core-us per candidate
+        n = max(1, min(npair if args.sector == 40 else m, int(10.0 * cores / 18e-6 / inner)))
+        small = dict(inp, modes=inp["modes"][:8], amplitudes=inp["amplitudes"][:, :8])
+        ow.signed(small, args.sector)  # OpenMP warm-up
+        t1 = time.perf_counter()
+        r = ow.signed(inp, args.sector, (0, n))
+        el = time.perf_counter() - t1
+        loop = "minus pairs" if args.sector == 40 else "minus modes"
+        cpu = {"value": n * inner / el, "unit": "candidates/s", "cores": cores, "kind": "port",
+               "sample": f"the first {n} {loop} of the M={m} sector-{args.sector} loop ({n * inner} candidates, "
+                         f"{int(r['kept'])} kept), C++/OpenMP restatement oracle/wit_oracle.cpp",
+               **host_info()}
     return {
         "metric": f"WIT sector-{args.sector} candidates/s, M={inp['modes'].shape[0]}", "value": rate,
         "unit": "candidates/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

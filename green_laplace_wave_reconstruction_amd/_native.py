@@ -61,7 +61,7 @@ class CPlanDesc(ctypes.Structure):
 
 #: every symbol include/glw.h declares
 EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_order2_execute",
-           "glw_order3_execute", "glw_order4_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44",
+           "glw_order3_execute", "glw_order4_execute", "glw_wit_create", "glw_wit_destroy", "glw_wit_sector44", "glw_wit_sector44_range",
            "glw_wit_sector",
            "glw_adjoint_execute", "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_plan_hold",
            "glw_batch_create",

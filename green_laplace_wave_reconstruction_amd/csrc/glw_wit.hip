@@ -393,6 +393,7 @@ struct QuartetArgs {
   // pair (i, j) land on mode(i) + mode(j) + the pair sums of the other two,
   // all inside the pw x ph box starting at (pxmin, pymin) = 2 min(mode).
   int lds_bins, pw, ph, pxmin, pymin;
+  long long pair0;          // leading pair of workgroup 0 (first-index ranges)
 };
 
 __device__ __forceinline__ void atomic_addc(double* p, cpx<double> v) {
@@ -424,7 +425,7 @@ __device__ __forceinline__ double wmin_d(double v) {
 template <typename T, int OCC = 1>
 __global__ __launch_bounds__(WNT, OCC) void k_wit_quartets(QuartetArgs g) {
   const WitDev& w = g.w;
-  const long long p = blockIdx.x;
+  const long long p = g.pair0 + blockIdx.x;
   int lo = 0, hi = w.M - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) / 2;
@@ -1093,7 +1094,16 @@ int glw_wit_destroy(glw_wit* w) {
 }
 
 int glw_wit_sector44(glw_wit* W, const double* amps, int32_t cases, double* out, double* counters, void* stream) {
+  if (!W) return wfail(GLW_EINVAL, "null argument");
+  return glw_wit_sector44_range(W, 0, W->d.modes_count, 0, amps, cases, out, counters, stream);
+}
+
+int glw_wit_sector44_range(glw_wit* W, int32_t first_begin, int32_t first_end, int32_t flags, const double* amps,
+                           int32_t cases, double* out, double* counters, void* stream) {
   if (!W || !amps || !out || !counters || cases < 1) return wfail(GLW_EINVAL, "null argument");
+  if (first_begin < 0 || first_end > W->d.modes_count || first_begin >= first_end)
+    return wfail(GLW_EINVAL, "first-index range must satisfy 0 <= begin < end <= M");
+  if (flags & ~GLW_WIT_GLOBAL_ATOMICS) return wfail(GLW_EINVAL, "unknown WIT flag");
   W_HIP(hipSetDevice(W->device));
   hipStream_t st = (hipStream_t)stream;
   const size_t n = 5ull * cases * W->d.nx * W->d.ny * 2;
@@ -1112,22 +1122,28 @@ int glw_wit_sector44(glw_wit* W, const double* amps, int32_t cases, double* out,
   g.pxmin = 2 * W->xmin;
   g.pymin = 2 * W->ymin;
   const size_t lds = (size_t)g.pw * g.ph * cases * 4 * sizeof(double);
-  g.lds_bins = lds <= kWitLdsBytes ? g.pw * g.ph : 0;
+  g.lds_bins = (lds <= kWitLdsBytes && !(flags & GLW_WIT_GLOBAL_ATOMICS)) ? g.pw * g.ph : 0;
   const size_t shm = g.lds_bins ? lds : 0;
+  // leading pairs (i, j) with i in [first_begin, first_end): offP[i] = i M - i (i - 1) / 2
+  auto off_p = [M](long long i) { return i * M - i * (i - 1) / 2; };
+  g.pair0 = off_p(first_begin);
+  const unsigned nblk = (unsigned)(off_p(first_end) - g.pair0);
   if (W->d.precision == GLW_PREC_FP32 && shm <= kLdsPerCu / 2) {  // two workgroups per CU fit: occupancy 2
     if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_quartets<float, 2>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL((k_wit_quartets<float, 2>), dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), shm, st, g);
+    hipLaunchKernelGGL((k_wit_quartets<float, 2>), dim3(nblk), dim3(WNT), shm, st, g);
   } else if (W->d.precision == GLW_PREC_FP32) {
     if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_quartets<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)shm));
-    hipLaunchKernelGGL(k_wit_quartets<float>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), shm, st, g);
+    hipLaunchKernelGGL(k_wit_quartets<float>, dim3(nblk), dim3(WNT), shm, st, g);
   } else {
     if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_quartets<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)shm));
-    hipLaunchKernelGGL(k_wit_quartets<double>, dim3((unsigned)(M * (M + 1) / 2)), dim3(WNT), shm, st, g);
+    hipLaunchKernelGGL(k_wit_quartets<double>, dim3(nblk), dim3(WNT), shm, st, g);
   }
-  const double total = (double)M * (M + 1) * (M + 2) * (M + 3) / 24.0;
+  // quartets i <= j <= k <= l < M with i in the range: sum over i of C(M - i + 2, 3)
+  double total = 0.0;
+  for (long long i = first_begin; i < first_end; ++i) total += (double)(M - i + 2) * (M - i + 1) * (M - i) / 6.0;
   hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, total, counters);
   W_HIP(hipGetLastError());
   return GLW_OK;

@@ -38,6 +38,9 @@ def _bind(lib):
     lib.glw_wit_destroy.restype = c_int
     lib.glw_wit_sector44.argtypes = [c_void_p, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
     lib.glw_wit_sector44.restype = c_int
+    lib.glw_wit_sector44_range.argtypes = [c_void_p, c_int32, c_int32, c_int32, c_void_p, c_int32, c_void_p, c_void_p,
+                                           c_void_p]
+    lib.glw_wit_sector44_range.restype = c_int
     lib.glw_wit_sector.argtypes = [c_void_p, c_int32, c_void_p, c_int32, c_void_p, c_void_p, c_void_p]
     lib.glw_wit_sector.restype = c_int
     return lib
@@ -81,8 +84,12 @@ class WitPlan:
         """Device run: amplitudes complex128 [cases][M] (numpy or torch); returns (out, counters) tensors."""
         return self.sector_device(44, amplitudes, out, counters)
 
-    def sector_device(self, sector: int, amplitudes, out=None, counters=None):
-        """Device run of sector 40, 42 or 44 (see sector44_device)."""
+    def sector_device(self, sector: int, amplitudes, out=None, counters=None, *, first_range=None,
+                      global_atomics: bool = False):
+        """Device run of sector 40, 42 or 44 (see sector44_device).  Sector 44 only: ``first_range``
+        (begin, end) restricts the sum to quartets whose first index lies in [begin, end) (the outermost
+        loop of the reference's sector-44 sum); ``global_atomics`` adds every quartet to the output with
+        global atomics instead of the workgroup's LDS window."""
         import torch
 
         if sector not in (40, 42, 44):
@@ -98,19 +105,27 @@ class WitPlan:
         if counters is None:
             counters = torch.empty(8, dtype=torch.float64, device=self.device)
         stream = torch.cuda.current_stream(self.device).cuda_stream
+        if first_range is not None or global_atomics:
+            if sector != 44:
+                raise ValueError("first-index ranges and global atomics apply to sector 44")
+            lo, hi = first_range if first_range is not None else (0, self.modes.shape[0])
+            _check(self.lib.glw_wit_sector44_range(self._h, int(lo), int(hi), 1 if global_atomics else 0,
+                                                   c_void_p(a.data_ptr()), cases, c_void_p(out.data_ptr()),
+                                                   c_void_p(counters.data_ptr()), c_void_p(stream)))
+            return out, counters
         _check(self.lib.glw_wit_sector(self._h, int(sector), c_void_p(a.data_ptr()), cases, c_void_p(out.data_ptr()),
                                        c_void_p(counters.data_ptr()), c_void_p(stream)))
         return out, counters
 
-    def sector44(self, amplitudes) -> dict:
-        return self.sector(44, amplitudes)
+    def sector44(self, amplitudes, *, first_range=None, global_atomics: bool = False) -> dict:
+        return self.sector(44, amplitudes, first_range=first_range, global_atomics=global_atomics)
 
-    def sector(self, sector: int, amplitudes) -> dict:
+    def sector(self, sector: int, amplitudes, **kw) -> dict:
         """Host run of one sector: the five output arrays and the counters, plus the wall time."""
         import torch
 
         t0 = time.perf_counter()
-        out, cnt = self.sector_device(sector, amplitudes)
+        out, cnt = self.sector_device(sector, amplitudes, **kw)
         torch.cuda.synchronize()
         secs = time.perf_counter() - t0
         o = out.cpu().numpy()

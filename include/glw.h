@@ -193,6 +193,17 @@ int glw_wit_destroy(glw_wit* wit);
  * schedule-dependent order (relative differences at the 1e-16 level). */
 int glw_wit_sector44(glw_wit* wit, const double* amplitudes, int32_t cases, double* out, double* counters,
                      void* stream);
+/* glw_wit_sector44 over the quartets whose first index i lies in
+ * [first_begin, first_end) -- the iterations first_begin..first_end-1 of the
+ * reference's outermost loop (directional_eta4_openmp.cpp:690-734 runs all
+ * of them), and the unit a multi-GPU run would split the sum by.  `out` and
+ * `counters` are overwritten with the range's sums; counters[0] (total) is
+ * the range's quartet count.  flags: 0, or GLW_WIT_GLOBAL_ATOMICS to add
+ * every quartet to `out` with global atomics instead of through the
+ * workgroup's LDS window (the path windows above 128 KB take). */
+#define GLW_WIT_GLOBAL_ATOMICS 1
+int glw_wit_sector44_range(glw_wit* wit, int32_t first_begin, int32_t first_end, int32_t flags,
+                           const double* amplitudes, int32_t cases, double* out, double* counters, void* stream);
 /* Any sector of the reference runner (`--sector 40|42|44`,
  * directional_eta4_openmp.cpp:603-734): 44 is glw_wit_sector44; 40 (plus
  * pair x minus pair, analytic half plane) and 42 (plus triple x minus mode)

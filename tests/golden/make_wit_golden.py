@@ -11,7 +11,12 @@
   these runs: near = 0);
 * runs it on n0050 in sectors 40 and 42 (signed parents) and keeps all five
   arrays on their nonzero bins (eta and psi coordinates, and in sector 42
-  two near-resonant retained bins).
+  two near-resonant retained bins);
+* with ``--m200 [DIR]`` only: runs it on n0200 (the 512^2, 200-mode runtime
+  input) in sectors 44, 40 and 42 -- about 3, 8 and 11 minutes on 8 cores --
+  or reuses DIR/out_s{44,40,42}.bin from an earlier run of
+  ``wit_eta4 tests/golden/wit_n0200_input.bin DIR/out_sS.bin --sector S``,
+  and keeps all five arrays on their nonzero bins.
 """
 
 import shutil
@@ -27,7 +32,9 @@ ROOT = HERE.parents[1]
 sys.path.insert(0, str(ROOT))
 from green_laplace_wave_reconstruction_amd import formats  # noqa: E402
 
-REF = Path(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+ARGS = [x for x in sys.argv[1:] if not x.startswith("--")]
+M200 = "--m200" in sys.argv
+REF = Path(ARGS[0] if ARGS and not M200 else "/root/reference")
 EXE = ROOT / "oracle" / "_ref" / "wit_eta4"
 
 
@@ -43,6 +50,33 @@ def save(name, out):
     print(name, "nonzero bins", idx.size, "kept", out["kept"])
 
 
+ARRAYS = ("eta", "psi", "eta_coordinate", "psi_coordinate", "retained")
+COUNTERS = ("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate", "max_residual",
+            "min_detuning", "max_strict_zero_source")
+
+
+def save_all(path, out):
+    idx = np.flatnonzero(np.any([out[k] != 0 for k in ARRAYS], axis=0))
+    np.savez_compressed(path, index=idx, shape=np.array(out["eta"].shape),
+                        counters=np.array([out[k] for k in COUNTERS], float),
+                        **{k: out[k].ravel()[idx] for k in ARRAYS})
+    return idx.size
+
+
+if M200:
+    given = Path(ARGS[0]) if ARGS else None
+    for sector in (44, 40, 42):
+        reuse = given / f"out_s{sector}.bin" if given else None
+        with tempfile.TemporaryDirectory() as td:
+            out_path = reuse if reuse and reuse.exists() else Path(td) / "out.bin"
+            if out_path != reuse:
+                subprocess.run([str(EXE), str(HERE / "wit_n0200_input.bin"), str(out_path), "--sector", str(sector)],
+                               check=True)
+            out = formats.read_dir4out(out_path)
+        n = save_all(HERE / f"wit_n0200_sector{sector}.npz", out)
+        print("n0200 sector", sector, "nonzero bins", n, "kept", out["kept"], "near", out["near"])
+    sys.exit(0)
+
 for name, rel in (("n0050", "inputs/runtime/n0050"), ("n0100", "inputs/runtime/n0100"),
                   ("n0200", "inputs/runtime/n0200"), ("n0400", "inputs/runtime/n0400"),
                   ("n0800", "inputs/runtime/n0800"), ("n0382", "inputs/field_n0382")):
@@ -55,18 +89,12 @@ for name in ("n0050", "n0100"):
 save("n0382", formats.read_dir4out(REF / "paper" / "order4" / "data" / "wit_field_n0382.bin"))
 
 
-ARRAYS = ("eta", "psi", "eta_coordinate", "psi_coordinate", "retained")
-COUNTERS = ("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate", "max_residual",
-            "min_detuning", "max_strict_zero_source")
 for sector in (40, 42):
     with tempfile.TemporaryDirectory() as td:
         out_path = Path(td) / "out.bin"
         subprocess.run([str(EXE), str(HERE / "wit_n0050_input.bin"), str(out_path), "--sector", str(sector)],
                        check=True)
         out = formats.read_dir4out(out_path)
-    idx = np.flatnonzero(np.any([out[k] != 0 for k in ARRAYS], axis=0))
-    np.savez_compressed(HERE / f"wit_n0050_sector{sector}.npz", index=idx, shape=np.array(out["eta"].shape),
-                        counters=np.array([out[k] for k in COUNTERS], float),
-                        **{k: out[k].ravel()[idx] for k in ARRAYS})
-    print("n0050 sector", sector, "nonzero bins", idx.size, "kept", out["kept"], "near", out["near"])
+    n = save_all(HERE / f"wit_n0050_sector{sector}.npz", out)
+    print("n0050 sector", sector, "nonzero bins", n, "kept", out["kept"], "near", out["near"])
 

@@ -166,7 +166,9 @@ def test_order3_1024_split_path_homogeneity(torch):
 def test_full_256_state_graph(torch):
     """The whole uneven batch (256 states, N = 64..1024, orders 2 and 3) in one graph launch: every
     state bitwise equal to its eager execute call; the N <= 128 states and every 8th N in {192, 256}
-    state against the oracle at 1e-7."""
+    state against the oracle at 3e-7 for order 3 (rounding at the 1e-16 level flips floor-threshold bins
+    and moves these states by up to ~1e-7, tests/test_conditioning.py; measured worst 1.1e-7) and 1e-7
+    for order 2."""
     from green_laplace_wave_reconstruction_amd.batch import SeaStateBatch
 
     draws = _draws()
@@ -192,6 +194,6 @@ def test_full_256_state_graph(torch):
         a, b = [x[0].cpu().numpy() for x in batch.fields(i)]
         err = max(rel(a, ra), rel(b, rb))
         worst = max(worst, err)
-        assert err < 1e-7, (i, draws[i].state.nx, draws[i].order, err)
+        assert err < (3e-7 if draws[i].order == 3 else 1e-7), (i, draws[i].state.nx, draws[i].order, err)
     print(f"256-state batch: {len(small) + len(mid)} states vs the oracle, worst rel-L2 {worst:.2e}", flush=True)
     batch.close()

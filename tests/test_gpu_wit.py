@@ -44,17 +44,13 @@ def test_wit_matches_reference_outputs(witmod, name):
 SIGNED_ARRAYS = ("eta", "psi", "eta_coordinate", "psi_coordinate", "retained")
 
 
-@pytest.mark.parametrize("sector", [40, 42])
-def test_wit_signed_sectors_match_reference_outputs(witmod, sector):
-    """Sectors 40 and 42 (signed parents: retained-V3 push, y31 coordinate) on n0050 against the
-    reference executable's own outputs (tests/golden/make_wit_golden.py): all five arrays, the
-    counters; sector 42 has two near-resonant quartets (retained bins)."""
-    g = np.load(GOLDEN / f"wit_n0050_sector{sector}.npz")
+def _check_all_arrays(witmod, name, sector):
+    g = np.load(GOLDEN / f"wit_{name}_sector{sector}.npz")
     c = dict(zip(("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate", "max_residual",
                   "min_detuning", "max_strict_zero_source"), g["counters"]))
-    print(f"WIT n0050 sector {sector}", flush=True)
-    r = witmod.run_dir4in(GOLDEN / "wit_n0050_input.bin", sector=sector)
-    print(f"WIT n0050 sector {sector}: {r['kept']} candidates in {r['seconds']:.3f} s", flush=True)
+    print(f"WIT {name} sector {sector}", flush=True)
+    r = witmod.run_dir4in(GOLDEN / f"wit_{name}_input.bin", sector=sector)
+    print(f"WIT {name} sector {sector}: {r['kept']} candidates in {r['seconds']:.3f} s", flush=True)
     for k in ("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate"):
         assert r[k] == int(c[k]), k
     # a term with normalized detuning rho is solved through a denominator ~rho: rounding moves it by
@@ -71,7 +67,81 @@ def test_wit_signed_sectors_match_reference_outputs(witmod, sector):
         assert err < tol, k
     assert r["max_residual"] < 10 * c["max_residual"] + 1e-15
     assert r["min_detuning"] == pytest.approx(c["min_detuning"], rel=1e-12)
-    assert r["max_strict_zero_source"] == pytest.approx(c["max_strict_zero_source"], rel=1e-12)
+    assert r["max_strict_zero_source"] == pytest.approx(c["max_strict_zero_source"], rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("sector", [40, 42])
+def test_wit_signed_sectors_match_reference_outputs(witmod, sector):
+    """Sectors 40 and 42 (signed parents: retained-V3 push, y31 coordinate) on n0050 against the
+    reference executable's own outputs (tests/golden/make_wit_golden.py): all five arrays, the
+    counters; sector 42 has two near-resonant quartets (retained bins)."""
+    _check_all_arrays(witmod, "n0050", sector)
+
+
+@pytest.mark.parametrize("sector", [44, 40, 42])
+def test_wit_m200_matches_reference_outputs(witmod, sector):
+    """The 512^2, 200-mode runtime input (the reference's M = 200 timing case) in every sector against
+    the reference executable's outputs: 6.9e7 quartets (44), 4.0e8 / 2.7e8 candidates (40 / 42) --
+    the kernels and launch shapes of the timed configuration, including the four-wave signed kernel."""
+    _check_all_arrays(witmod, "n0200", sector)
+
+
+@pytest.mark.parametrize("name", ["n0050", "n0100"])
+def test_wit_global_atomic_path(witmod, name):
+    """Every quartet added with global atomics (the path of LDS windows above 128 KB) against the
+    reference outputs."""
+    inp = formats.read_dir4in(GOLDEN / f"wit_{name}_input.bin")
+    plan = witmod.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                          dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
+    r = plan.sector44(inp["amplitudes"], global_atomics=True)
+    plan.close()
+    eta, psi, c = golden(name)
+    assert rel(r["eta"], eta) < 1e-12 and rel(r["psi"], psi) < 1e-12
+    for k in ("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate"):
+        assert r[k] == c[k], k
+
+
+def test_wit_m800_first_index_slice_vs_oracle(witmod):
+    """M = 800 (the reference's 21,857 s case, beyond a full CPU oracle here): the quartets with first
+    index in [703, 800) -- 3.9e6 of 1.7e10 -- against the oracle restatement over the same slice."""
+    from oracle import wit as ow
+
+    inp = formats.read_dir4in(GOLDEN / "wit_n0800_input.bin")
+    rng = (703, 800)
+    print("WIT n0800 slice: oracle", flush=True)
+    ref = ow.sector44(inp, rng)
+    print("WIT n0800 slice: GPU", flush=True)
+    plan = witmod.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                          dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
+    r = plan.sector44(inp["amplitudes"], first_range=rng)
+    plan.close()
+    assert r["total"] == 3921225  # sum over i of C(800 - i + 2, 3) = C(100, 4)
+    for k in ("kept", "near", "strict_zero_output", "strict_zero_intermediate"):
+        assert r[k] == int(ref[k]), k
+    for k in ("eta", "psi"):
+        assert rel(r[k], ref[k]) < 1e-12, k
+    assert r["max_residual"] < 10 * ref["max_residual"] + 1e-15
+
+
+def test_wit_ranges_add_up_and_spread(witmod):
+    """Linearity over the first index: two ranges sum to the whole sector-44 sum, and the run-to-run
+    spread of the FP64 atomic accumulation (schedule-dependent summation order) is at rounding level."""
+    inp = formats.read_dir4in(GOLDEN / "wit_n0100_input.bin")
+    plan = witmod.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                          dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"])
+    full = plan.sector44(inp["amplitudes"])
+    again = plan.sector44(inp["amplitudes"])
+    lo = plan.sector44(inp["amplitudes"], first_range=(0, 37))
+    hi = plan.sector44(inp["amplitudes"], first_range=(37, 100))
+    plan.close()
+    assert lo["total"] + hi["total"] == full["total"] and lo["kept"] + hi["kept"] == full["kept"]
+    for k in ("eta", "psi"):
+        assert rel(lo[k] + hi[k], full[k]) < 1e-14, k
+    nz = full["eta"] != 0
+    spread = float(np.max(np.abs(again["eta"][nz] - full["eta"][nz]) / np.abs(full["eta"][nz])))
+    print(f"WIT n0100 run-to-run: max relative bin difference {spread:.2e}, rel-L2 "
+          f"{rel(again['eta'], full['eta']):.2e}", flush=True)
+    assert rel(again["eta"], full["eta"]) < 1e-15 and spread < 1e-12
 
 
 def test_wit_near_resonant_branch_vs_restatement(witmod):

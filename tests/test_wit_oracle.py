@@ -102,3 +102,48 @@ def test_glrti_split_arguments_like_reference(tmp_path):
     with pytest.raises(ValueError, match="split mode requires one outer4 index"):
         api.run_glrti(missing, None, 6, part=0)
 
+
+
+def golden_all(name, sector):
+    """All five arrays and the counters of a reference output fixture (tests/golden/make_wit_golden.py)."""
+    g = np.load(GOLDEN / f"wit_{name}_sector{sector}.npz")
+    shape = tuple(g["shape"])
+    arrays = {}
+    for k in ("eta", "psi", "eta_coordinate", "psi_coordinate", "retained"):
+        a = np.zeros(int(np.prod(shape)), complex)
+        a[g["index"]] = g[k]
+        arrays[k] = a.reshape(shape)
+    return arrays, dict(zip(COUNTERS, g["counters"]))
+
+
+@pytest.mark.parametrize("sector", [40, 42])
+def test_signed_restatement_matches_reference_n0050(sector):
+    """Sectors 40 and 42 (signed parents: the retained-V3 split of charge +-1 cubic subsets, its push
+    through the quadratic normal map, the y31 coordinate, near-resonant quartics in sector 42) against
+    the reference executable's own n0050 outputs: every array and counter."""
+    inp = formats.read_dir4in(GOLDEN / "wit_n0050_input.bin")
+    r = wit.signed(inp, sector)
+    ref, c = golden_all("n0050", sector)
+    for k in ("total", "kept", "near", "strict_zero_output", "strict_zero_intermediate"):
+        assert r[k] == c[k], k
+    tol = max(1e-13, 1e-15 / c["min_detuning"])  # a rho-small denominator amplifies rounding by 1/rho
+    for k, v in ref.items():
+        if not np.any(v):
+            assert not np.any(r[k]), k
+            continue
+        assert rel(r[k], v) < tol, (k, rel(r[k], v))
+    assert r["min_detuning"] == pytest.approx(c["min_detuning"], rel=1e-12)
+    # the paired residual is rounding noise of the solve: bounded, not reproduced digit for digit
+    assert r["max_residual"] < 10 * c["max_residual"] + 1e-15
+
+
+def test_signed_restatement_outer_split_is_additive():
+    inp = formats.read_dir4in(GOLDEN / "wit_n0050_input.bin")
+    inp["modes"] = inp["modes"][:16]
+    inp["amplitudes"] = inp["amplitudes"][:, :16]
+    for sector, n in ((40, 16 * 17 // 2), (42, 16)):
+        full = wit.signed(inp, sector)
+        a = wit.signed(inp, sector, (0, n // 3))
+        b = wit.signed(inp, sector, (n // 3, n))
+        assert rel(a["eta"] + b["eta"], full["eta"]) < 1e-14
+        assert a["kept"] + b["kept"] == full["kept"]
