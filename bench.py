@@ -435,6 +435,43 @@ def bench_inverse(args):
     }
 
 
+def bench_api(args):
+    """The whole public call at the headline sea state: gl_spectral_surface(coeffs, 2000, 2000, 512, 512, t)
+    for the 64 instants -- host validation, deposit and FP64 plan (NumPy), device plan, eta11/psi11,
+    eta22/psi22 and eta33/psi33 on the GPU, every component copied to host arrays."""
+    import torch
+
+    import green_laplace_wave_reconstruction_amd as G
+    from green_laplace_wave_reconstruction_amd import seastate
+
+    s = seastate.order3_headline_512()
+    times = seastate.headline_instants()
+    c = G.gl_spectral_coefficients(3, s.gravity, s.depth, s.a, s.b, s.kx, s.ky, 0, 0)
+    for _ in range(args.warmup):
+        G.gl_spectral_surface(c, s.lx, s.ly, s.nx, s.ny, times)
+    torch.cuda.synchronize()
+    walls = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        eta, psi, X, Y, comps, audit = G.gl_spectral_surface(c, s.lx, s.ly, s.nx, s.ny, times)
+        walls.append(time.perf_counter() - t0)
+    wall = sum(walls)
+    return {
+        "metric": "gl_spectral_surface calls at 512^2, order 3, 64 instants: instants/s end to end "
+                  "(host plan + GPU + copies of every component to host)",
+        "value": INSTANTS * args.steps / wall, "unit": "instants/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic seeded JONSWAP sea state (numpy default_rng seed 2)",
+        "config": {"workload": "gl_spectral_surface(coeffs, 2000, 2000, 512, 512, t) with t = the 64 headline "
+                               "instants; returns 14 host arrays [64][512][512] (eta, psi and the *_plus / real "
+                               "components of orders 1-3)",
+                   "min_ms": min(walls) * 1e3, "max_ms": max(walls) * 1e3,
+                   "host_bytes_returned": int(sum(v.nbytes for v in comps.values()))},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 MODAL_POINTS = 8192   # default modal grid of the bench (--modal-points)
 
 
@@ -677,7 +714,8 @@ def _wit_roofline(precision: str, quartets_per_s: float, key=None, kernel="k_wit
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="order3", choices=("order3", "eta44", "wit", "batch", "inverse", "modal"))
+    ap.add_argument("--workload", default="order3",
+                    choices=("order3", "api", "eta44", "wit", "batch", "inverse", "modal"))
     ap.add_argument("--states", type=int, default=256, help="batch workload: sea states per step")
     ap.add_argument("--branches", type=int, default=4, help="batch workload: parallel graph branches")
     ap.add_argument("--rank", type=int, default=6, help="eta44 GL rank")
@@ -739,6 +777,9 @@ def main():
         return
     if args.workload == "inverse":
         print(json.dumps(bench_inverse(args)))
+        return
+    if args.workload == "api":
+        print(json.dumps(bench_api(args)))
         return
     s, c, kxg, kyg, spec, omega = workload()
     if args.precision != "fp64":

@@ -107,3 +107,23 @@ def test_eta44_and_wit_reproduce_reference_smoke(glw, rank):
     m = smoke_metric(eta, r["eta"][0])
     print(f"smoke GL{rank}: {m:.15f} vs {SMOKE32_GL_WIT[rank]:.15f}", flush=True)
     assert abs(m - SMOKE32_GL_WIT[rank]) < 1e-9 * SMOKE32_GL_WIT[rank]
+
+
+def test_eta44_seeded_config_vs_oracle(glw):
+    """The seeded order-4 sea state (256^2, g = h = 1, kp h = 1.5, kp Hs / 2 = 0.12, JONSWAP over 5 peak
+    wavelengths, seastate.order4_dimensionless_256) at GL6 against the FP64 oracle.  Bound 1e-7: the
+    roundoff floor makes the GL graphs discontinuous at the 1e-16 rounding level (tests/test_conditioning.py);
+    n0382's GL6/GL8 fields sit 1e-10 from the oracle."""
+    from green_laplace_wave_reconstruction_amd import api, seastate
+
+    st = seastate.order4_dimensionless_256()
+    c = api.gl_spectral_coefficients(1, st.gravity, st.depth, st.a, st.b, st.kx, st.ky, 0, 0)
+    kxg, kyg = api._wavenumber_grid(st.lx, st.ly, st.nx, st.ny)
+    spec, _ = api.deposit_analytic_input(c, st.lx, st.ly, st.nx, st.ny)
+    qx, qy = st.depth * kxg, st.depth * kyg
+    print(f"eta44 seeded 256^2 GL6: {st.count} parents", flush=True)
+    eta, _, _ = glw.gl_pure_sum_order4(spec, qx, qy, c["peak_depth"], 6)
+    ref = O.eta44(spec, qx, qy, c["peak_depth"], 6)
+    err = rel(eta, ref)
+    print(f"eta44 seeded 256^2 GL6 vs oracle: {err:.2e}", flush=True)
+    assert err < 1e-7 and np.linalg.norm(ref) > 0
