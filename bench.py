@@ -821,11 +821,13 @@ def main():
     wall = time.perf_counter() - t0
     # per-stage times (HIP events around every launch, on the execute stream) and compulsory bytes,
     # from the same number of further steps
+    # (one step holds 64 / chunk x ~16 launches; the event pool times up to 4096 launches per read)
+    prof_steps = args.steps if args.chunk >= INSTANTS else max(1, args.steps * args.chunk // INSTANTS)
     dp.profile(True)
     dp.stage_times()
     torch.cuda.synchronize()
     tp = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(prof_steps):
         dp.run("order3", times, out=(eta, psi))
     torch.cuda.synchronize()
     wall_profiled = time.perf_counter() - tp
@@ -849,7 +851,7 @@ def main():
             "note": "eta+psi of the 64 instants copied to pinned host memory after each step (serial)"}
     ms_per_step = wall / args.steps * 1e3
     value = INSTANTS * args.steps / wall
-    roof, pipe = stage_roofline(stages, args.steps, wall_profiled / args.steps * 1e3)
+    roof, pipe = stage_roofline(stages, prof_steps, wall_profiled / prof_steps * 1e3)
     traffic = traffic_from_profiles(roof["kernel"], schedule=schedule)
     roof["traffic"] = traffic
     roof.update(measured_hbm(traffic, roof["avg_launch_ms"]))

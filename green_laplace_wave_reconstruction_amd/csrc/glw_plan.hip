@@ -10,6 +10,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <type_traits>
 #include <vector>
 
@@ -28,6 +29,10 @@ thread_local std::string g_err;
 // Row stride of column-pass intermediates: box width rounded up to 8 complex
 // values, so every 8-column tile starts on a 128-byte line.
 inline int sw(int w) { return (w + 7) & ~7; }
+
+// Output grids: any even length up to this; FFT lengths (2^a 3^b, 16..2048)
+// synthesize through the FFT passes, the others through direct DFT passes.
+constexpr int kMaxOutputLength = 16384;
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -60,6 +65,82 @@ __global__ void k_phase(const cpx<double>* s0, const double* omega, const double
   sincos(omega[b] * times[t], &sn, &cs);
   const cpx<double> s = s0[b];
   st[(size_t)t * box + b] = {s.x * cs + s.y * sn, s.y * cs - s.x * sn};
+}
+
+// ----------------------------------------------------------------------------
+// Output grids whose length is not an FFT length (any even N): the synthesis
+// passes as direct inverse DFTs over the box, with tables of
+// exp(+2 pi i m n / N) built on the host for the box's signed modes m.  Only
+// the final synthesis depends on N (the working grids are FFT lengths).
+// ----------------------------------------------------------------------------
+constexpr int kDftCols = 8;  // box columns per workgroup of k_dft_colinv
+
+// y-pass: column c of field f -> out[task][f][y][ostride] for every y < gy.
+// ty: [wy][gy] = exp(2 pi i (my0 + r) y / gy).  The box columns of one tile
+// go through the same loader as the FFT pass (damping, floor, multiplier,
+// scale) into LDS; each output is a wy-term sum.
+__global__ __launch_bounds__(NT) void k_dft_colinv(ColInvArgs a, int gy, const cpx<double>* __restrict__ ty) {
+  extern __shared__ cpx<double> col[];  // [kDftCols][wy]
+  const int f = blockIdx.y;
+  const int task = blockIdx.z;
+  const int set = task / a.sdiv;
+  const int src = a.fsrc[f];
+  const int box = a.wx * a.wy;
+  ColInvLoad<double> ld;
+  ld.spec = a.spec + ((size_t)set * a.nsrc + src) * box;
+  ld.damp = a.damp ? a.damp + (size_t)((task % a.dmod) * a.dmul + a.dadd) * box : nullptr;
+  ld.level = -1.0;
+  if (a.fmax && ((a.floor_src_mask >> src) & 1)) {
+    const size_t k = (size_t)set * a.fmax_stride + (a.fmax_stride == 1 ? 0 : src);
+    ld.level = a.floor_level * __longlong_as_double((long long)a.fmax[k]);
+  }
+  ld.scale = a.fscale[f];
+  ld.mul = a.tab.mul + (size_t)a.fcode[f] * box;
+  ld.col0 = blockIdx.x * kDftCols;
+  ld.wx = a.wx;
+  ld.wy = a.wy;
+  ld.my0 = 0;           // element e of the loader is box row e (the table carries the mode offset)
+  ld.gy = a.wy;
+  for (int i = threadIdx.x; i < kDftCols * a.wy; i += NT) {
+    const int c = i / a.wy, r = i - c * a.wy;
+    col[i] = ld(c, r);
+  }
+  __syncthreads();
+  cpx<double>* out = reinterpret_cast<cpx<double>*>(a.out) + ((size_t)task * a.out_nfield + a.out_foff + f) * gy *
+                     a.ostride;
+  const int ncol = min(kDftCols, a.wx - ld.col0);
+  for (int i = threadIdx.x; i < gy * ncol; i += NT) {
+    const int y = i / ncol, c = i - y * ncol;
+    const cpx<double>* x = col + c * a.wy;
+    double re = 0.0, im = 0.0;
+    for (int r = 0; r < a.wy; ++r) {
+      const cpx<double> w = ty[(size_t)r * gy + y], v = x[r];
+      re += v.x * w.x - v.y * w.y;
+      im += v.x * w.y + v.y * w.x;
+    }
+    out[(size_t)y * a.ostride + ld.col0 + c] = {re, im};
+  }
+}
+
+// x-pass: row y of sequence seq (box columns, istride apart) -> out[seq][y][gx].
+// tx: [wx][gx] = exp(2 pi i (mx0 + c) x / gx); one workgroup per row.
+__global__ __launch_bounds__(NT) void k_dft_row(SynthArgs a, int gx, const cpx<double>* __restrict__ tx) {
+  extern __shared__ cpx<double> row[];  // [wx]
+  const int y = blockIdx.x;
+  const int seq = blockIdx.y;
+  const cpx<double>* in = reinterpret_cast<const cpx<double>*>(a.in) + (size_t)seq * a.in_seq + (size_t)y * a.istride;
+  for (int c = threadIdx.x; c < a.wx; c += NT) row[c] = in[c];
+  __syncthreads();
+  cpx<double>* out = a.out + ((size_t)seq * a.gy + y) * gx;
+  for (int x = threadIdx.x; x < gx; x += NT) {
+    double re = 0.0, im = 0.0;
+    for (int c = 0; c < a.wx; ++c) {
+      const cpx<double> w = tx[(size_t)c * gx + x], v = row[c];
+      re += v.x * w.x - v.y * w.y;
+      im += v.x * w.y + v.y * w.x;
+    }
+    out[x] = {re, im};
+  }
 }
 
 __device__ __forceinline__ cpx<double> floored(cpx<double> x, double level) {
@@ -573,6 +654,9 @@ struct glw_plan_s {
   int device = 0;
   std::vector<void*> allocs;
   std::map<int, void*> tw64;
+  // direct-DFT synthesis tables of output lengths that are not FFT lengths:
+  // (first signed mode, count, length) -> [count][length] exp(2 pi i m n / length)
+  std::map<std::tuple<int, int, int>, cpx<double>*> dft;
   DevTables t1, t2, t3;
   double *omega = nullptr, *damp_o = nullptr, *damp_oi = nullptr;
   uint8_t *mask2 = nullptr, *mask3 = nullptr;
@@ -725,6 +809,28 @@ struct glw_plan_s {
   }
   // FFT twiddles exp(-2 pi i m / n), FP64 (every GL plan is FP64)
   const void* tw(int n, bool /*fp32*/) { return tw64.at(n); }
+
+  // exp(+2 pi i m k / n) for m = m0 .. m0 + count - 1 and k < n, from the exact integer residue
+  // (m k) mod n; built once per (box axis, length) at plan creation
+  int dft_table(int m0, int count, int n) {
+    const auto key = std::make_tuple(m0, count, n);
+    if (dft.count(key)) return GLW_OK;
+    std::vector<cpx<double>> t((size_t)count * n);
+    const long double two_pi = 6.283185307179586476925286766559005768L;
+    for (int i = 0; i < count; ++i) {
+      const long long m = (((long long)(m0 + i) % n) + n) % n;
+      for (int k = 0; k < n; ++k) {
+        const long double ang = two_pi * (long double)((m * k) % n) / (long double)n;
+        t[(size_t)i * n + k] = {(double)cosl(ang), (double)sinl(ang)};
+      }
+    }
+    cpx<double>* p = nullptr;
+    int rc = upload(&p, t.data(), t.size());
+    if (rc) return rc;
+    dft[key] = p;
+    return GLW_OK;
+  }
+  const cpx<double>* dft_at(int m0, int count, int n) const { return dft.at(std::make_tuple(m0, count, n)); }
 };
 
 namespace {
@@ -732,8 +838,8 @@ namespace {
 int validate(const glw_plan_desc* d) {
   if (!d) return fail(GLW_EINVAL, "null descriptor");
   if (d->nx < 8 || d->ny < 8 || d->nx % 2 || d->ny % 2) return fail(GLW_EINVAL, "grid must be even and >= 8");
-  if (!length_supported(d->nx) || !length_supported(d->ny))
-    return fail(GLW_EUNSUPPORTED, "output grid length not supported by the FFT kernels");
+  if (d->nx > kMaxOutputLength || d->ny > kMaxOutputLength)
+    return fail(GLW_EUNSUPPORTED, "output grid longer than " + std::to_string(kMaxOutputLength) + " per side");
   if (d->order < 1 || d->order > 3) return fail(GLW_EINVAL, "order must be 1, 2 or 3");
   if (d->precision == GLW_PREC_FP32)
     return fail(GLW_EUNSUPPORTED,
@@ -797,8 +903,15 @@ int build(glw_plan* P) {
   int rc;
   if ((rc = P->upload_tables(P->t1, d.t1, b1))) return rc;
   if ((rc = P->upload(&P->omega, d.omega, b1))) return rc;
-  if ((rc = P->twiddles(d.nx))) return rc;
-  if ((rc = P->twiddles(d.ny))) return rc;
+  // output synthesis: FFT twiddles, or direct-DFT tables of every box the plan synthesizes
+  std::vector<glw_box> out_boxes{d.p1};
+  if (d.order >= 2) out_boxes.push_back(d.p2);
+  if (d.order >= 3) out_boxes.push_back(d.p3);
+  if (d.with_order4) out_boxes.push_back(d.o4.p4);
+  for (const glw_box& b : out_boxes) {
+    if ((rc = length_supported(d.nx) ? P->twiddles(d.nx) : P->dft_table(b.mx0, b.wx, d.nx))) return rc;
+    if ((rc = length_supported(d.ny) ? P->twiddles(d.ny) : P->dft_table(b.my0, b.wy, d.ny))) return rc;
+  }
   int wfin = d.p1.wx;
   if ((rc = P->reserve("st", nt * b1 * zsz))) return rc;
   if (d.order >= 2) {
@@ -1029,7 +1142,21 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
   a.out_nfield = nout;
   a.out_foff = 0;
   a.ostride = sw(box.wx);
-  int rc = GLW_STAGE(P, "synth_colinv", st, colinv_bytes(a, d.ny, nt), launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), P->exec(STAGE_COL), st));
+  int rc;
+  if (length_supported(d.ny)) {
+    rc = GLW_STAGE(P, "synth_colinv", st, colinv_bytes(a, d.ny, nt),
+                   launch_colinv<double>(d.ny, a, nt, P->tw(d.ny, false), P->exec(STAGE_COL), st));
+  } else {  // direct DFT: kDftCols box columns per workgroup, all box rows in LDS
+    const size_t shm = (size_t)kDftCols * box.wy * sizeof(cpx<double>);
+    if (shm > (size_t)160 * 1024) return fail(GLW_EUNSUPPORTED, "spectral box too tall for the direct-DFT synthesis");
+    if (shm > (size_t)64 * 1024)
+      GLW_HIP(hipFuncSetAttribute((const void*)k_dft_colinv, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    P->mark_begin("synth_dft_col", st, colinv_bytes(a, d.ny, nt));
+    hipLaunchKernelGGL(k_dft_colinv, dim3((box.wx + kDftCols - 1) / kDftCols, nout, nt), dim3(NT), shm, st, a, d.ny,
+                       P->dft_at(box.my0, box.wy, d.ny));
+    P->mark_end(st);
+    rc = launch_plain(hipGetLastError());
+  }
   if (rc) return rc;
   for (int f = 0; f < nout; ++f) {
     // rows of field f of every instant: sequence t starts at (t*nout + f)*ny*wx
@@ -1041,7 +1168,18 @@ int synthesize(glw_plan* P, const cpx<double>* spec, int nsrc, const glw_box& bo
     s.wx = box.wx;
     s.gy = d.ny;
     s.out = (cpx<double>*)outs[f];
-    if ((rc = GLW_STAGE(P, "synth_row", st, synth_bytes(s, d.nx, nt), launch_synth(d.nx, s, nt, P->tw(d.nx, false), st)))) return rc;
+    if (length_supported(d.nx)) {
+      if ((rc = GLW_STAGE(P, "synth_row", st, synth_bytes(s, d.nx, nt), launch_synth(d.nx, s, nt, P->tw(d.nx, false), st))))
+        return rc;
+    } else {  // direct DFT: one row per workgroup, its box columns in LDS
+      const size_t shm = (size_t)box.wx * sizeof(cpx<double>);
+      if (shm > (size_t)64 * 1024)
+        GLW_HIP(hipFuncSetAttribute((const void*)k_dft_row, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      P->mark_begin("synth_dft_row", st, synth_bytes(s, d.nx, nt));
+      hipLaunchKernelGGL(k_dft_row, dim3(d.ny, nt), dim3(NT), shm, st, s, d.nx, P->dft_at(box.mx0, box.wx, d.nx));
+      P->mark_end(st);
+      if ((rc = launch_plain(hipGetLastError()))) return rc;
+    }
   }
   return GLW_OK;
 }
@@ -1843,6 +1981,8 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
                 cpx<double>* gs0, int recompute, hipStream_t st) {
   const glw_plan_desc& d = P->d;
   if (d.precision != GLW_PREC_FP64) return fail(GLW_EINVAL, "the adjoint runs on FP64 plans");
+  if (!length_supported(d.nx) || !length_supported(d.ny))
+    return fail(GLW_EUNSUPPORTED, "the adjoint needs an output grid of FFT lengths (2^a 3^b, 16..2048 per side)");
   if (!orders || (orders & ~7)) return fail(GLW_EINVAL, "orders must be a mask of 1 (eta11), 2 (eta22), 4 (eta33)");
   if ((orders & 2) && d.order < 2) return fail(GLW_EINVAL, "plan built without order-2 tables");
   if ((orders & 4) && d.order < 3) return fail(GLW_EINVAL, "plan built without order-3 tables");

@@ -12,7 +12,7 @@
 * runs it on n0050 in sectors 40 and 42 (signed parents) and keeps all five
   arrays on their nonzero bins (eta and psi coordinates, and in sector 42
   two near-resonant retained bins);
-* with ``--m200 [DIR]`` only: runs it on n0200 (the 512^2, 200-mode runtime
+* with ``--m200 [DIR] [--sectors=44,40,42]`` only: runs it on n0200 (the 512^2, 200-mode runtime
   input) in sectors 44, 40 and 42 -- about 3, 8 and 11 minutes on 8 cores --
   or reuses DIR/out_s{44,40,42}.bin from an earlier run of
   ``wit_eta4 tests/golden/wit_n0200_input.bin DIR/out_sS.bin --sector S``,
@@ -34,6 +34,7 @@ from green_laplace_wave_reconstruction_amd import formats  # noqa: E402
 
 ARGS = [x for x in sys.argv[1:] if not x.startswith("--")]
 M200 = "--m200" in sys.argv
+SECTORS = [int(x) for a in sys.argv if a.startswith("--sectors=") for x in a.split("=", 1)[1].split(",")] or [44, 40, 42]
 REF = Path(ARGS[0] if ARGS and not M200 else "/root/reference")
 EXE = ROOT / "oracle" / "_ref" / "wit_eta4"
 
@@ -65,7 +66,7 @@ def save_all(path, out):
 
 if M200:
     given = Path(ARGS[0]) if ARGS else None
-    for sector in (44, 40, 42):
+    for sector in SECTORS:
         reuse = given / f"out_s{sector}.bin" if given else None
         with tempfile.TemporaryDirectory() as td:
             out_path = reuse if reuse and reuse.exists() else Path(td) / "out.bin"
