@@ -67,7 +67,7 @@ EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_ord
            "glw_batch_create",
            "glw_batch_launch", "glw_batch_destroy", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
            "glw_fft_length_supported", "glw_modal_length_supported", "glw_modal_workspace_size",
-           "glw_modal_execute", "glw_modal_stage_times", "glw_eta20_workspace_size", "glw_eta20_execute", "glw_last_error", "glw_version",
+           "glw_modal_execute", "glw_modal_stage_times", "glw_modal_execute_noalloc", "glw_eta20_workspace_size", "glw_eta20_execute", "glw_last_error", "glw_version",
            "glw_abi_sizeof")
 
 #: GLW_KIND_* of include/glw.h

@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "glw_internal.hpp"
@@ -409,6 +410,58 @@ extern "C" {
 int glw_modal_execute(const glw_modal_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,
                       double* eta11, double* eta22, double* eta33, double* audit, void* stream) {
   return modal_run(d, deposit, workspace, workspace_bytes, eta11, eta22, eta33, audit, (hipStream_t)stream, nullptr);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant of glw_modal_execute: its workspace is `scratch`, a
+// static device array of kNoAllocScratchBytes that every GPU the library runs
+// on holds from module load to process end.  In host code `scratch` names a
+// host-side stand-in, not device memory: its device address is looked up once
+// per device (hipGetSymbolAddress, with that device current) and only that
+// address is used.  Calls on one GPU share the array, so calls that overlap
+// on one GPU (two streams, two threads) race; nothing needs it cleared (every
+// stage writes what the next one reads).
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(void** ptr) {
+  int dev = -1;
+  GLW_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return mfail(GLW_EDEVICE, "device index out of range for the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return mfail(GLW_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *ptr = g_scratch_ptr[dev];
+  return GLW_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int glw_modal_execute_noalloc(const glw_modal_desc* d, const double* deposit, double* eta11, double* eta22,
+                              double* eta33, double* audit, void* stream) {
+  if (int rc = check_modal(d)) return rc;
+  const size_t need = modal_layout(*d).total;
+  if (need > kNoAllocScratchBytes)
+    return mfail(GLW_EINVAL, "modal level needs " + std::to_string(need) + " bytes of workspace, above the " +
+                                 std::to_string(kNoAllocScratchBytes) + "-byte static scratch of " +
+                                 "glw_modal_execute_noalloc (use glw_modal_execute with a workspace)");
+  void* ws = nullptr;
+  if (int rc = scratch_on_current_device(&ws)) return rc;
+  return modal_run(d, deposit, ws, kNoAllocScratchBytes, eta11, eta22, eta33, audit, (hipStream_t)stream, nullptr);
 }
 
 int glw_modal_stage_times(const glw_modal_desc* d, const double* deposit, void* workspace, size_t workspace_bytes,

@@ -63,6 +63,9 @@ def _bind(lib):
     lib.glw_modal_stage_times.argtypes = [POINTER(CModalDesc), c_void_p, c_void_p, c_size_t, c_void_p, c_void_p,
                                           c_void_p, c_void_p, c_void_p, POINTER(ctypes.c_float)]
     lib.glw_modal_stage_times.restype = c_int
+    lib.glw_modal_execute_noalloc.argtypes = [POINTER(CModalDesc), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                              c_void_p]
+    lib.glw_modal_execute_noalloc.restype = c_int
     lib.glw_eta20_workspace_size.argtypes = [POINTER(CEta20Desc), POINTER(c_size_t)]
     lib.glw_eta20_workspace_size.restype = c_int
     lib.glw_eta20_execute.argtypes = [POINTER(CEta20Desc), c_void_p, c_void_p, c_size_t, c_void_p, c_void_p]

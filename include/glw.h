@@ -348,6 +348,15 @@ int glw_modal_execute(const glw_modal_desc* desc, const double* deposit, void* w
  * no counterpart in the reference. */
 int glw_modal_stage_times(const glw_modal_desc* desc, const double* deposit, void* workspace, size_t workspace_bytes,
                           double* eta11, double* eta22, double* eta33, double* audit, void* stream, float* ms);
+/* glw_modal_execute without a workspace argument, allocating nothing (no
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops).  The
+ * workspace is a static 128 MiB device array the library holds on every GPU
+ * it runs on until the process ends; levels whose glw_modal_workspace_size
+ * exceeds it fail with GLW_EINVAL.  Calls on one GPU share the array: calls
+ * that overlap on one GPU race. */
+int glw_modal_execute_noalloc(const glw_modal_desc* desc, const double* deposit, double* eta11, double* eta22,
+                              double* eta33, double* audit, void* stream);
 
 /* glw_eta20_execute replaces unidirectional_eta20
  * (python/src/green_laplace/_eta20.py:13-132): the difference-frequency
