@@ -738,7 +738,30 @@ def main():
     ap.add_argument("--schedule", default="", help="order3: use this FFT schedule (stage=plan/tile;...) instead of tuning")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration for rocprofv3 --pmc)")
+    ap.add_argument("--calibrate-hbm", action="store_true",
+                    help="time plain 4 GiB copy (read + write), fill (write) and sum (read) kernels: the HBM rates "
+                         "a streaming kernel reaches on this device")
     args = ap.parse_args()
+    if args.calibrate_hbm:
+        import torch
+
+        x = torch.ones(1 << 29, dtype=torch.float64, device="cuda")  # 4 GiB
+        y = torch.empty_like(x)
+        n = x.numel() * 8
+        rates = {}
+        for name, fn, nbytes in (("copy (read + write)", lambda: y.copy_(x), 2 * n), ("fill (write)", lambda: y.fill_(2.0), n),
+                                 ("sum (read)", lambda: x.sum(), n)):
+            fn()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(10):
+                fn()
+            e1.record()
+            torch.cuda.synchronize()
+            rates[name] = nbytes / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
+        print(json.dumps({"calibration": "hbm", "GBps": rates, "frac_of_8TBps": {k: v / HBM_PEAK_GBS for k, v in rates.items()}}))
+        return
     if args.calibrate_copy:
         import torch
 
