@@ -691,7 +691,7 @@ def bench_wit_signed(args):
         "dtype": "f64", "data": f"reference frozen input paper/order4/inputs/runtime/{name} (copied as a fixture)",
         "config": {"workload": f"WIT sector {args.sector}, M={inp['modes'].shape[0]}, {inp['nx']}x{inp['ny']}",
                    "candidates": int(c[0]), "kept": int(c[1]), "near_resonant": int(c[2])},
-        "roofline": _wit_roofline("fp64", rate, key=f"sector{args.sector} fp64", kernel=f"k_wit_signed<{args.sector}>"),
+        "roofline": _wit_roofline("fp64", rate, key=f"sector{args.sector} fp64", kernel=f"k_wit_signed_tab<{args.sector}>"),
         "cpu_baseline": cpu,
     }
 

@@ -22,6 +22,11 @@
 // In sector 44 every parent sign is +1, so charge = degree: the retained-V3
 // push (:284-298) and the y31 coordinate (:300-336) vanish; only the
 // degree-4 near-resonant branch (:393-411) retains a V3 term.
+// Sectors 40 and 42 have signed parents.  Flipping every sign of a subset
+// conjugates its state, so two more tables -- pairs {a+, b-} and triples
+// {a+, b+, c-} -- give every signed subset, and those sectors too evaluate
+// only the degree-4 level per candidate, plus the push and y31 terms, which
+// pair each cubic subset with the parent it lacks.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -115,13 +120,15 @@ struct Limits<float> {
   static constexpr float eps = 1.1920928955078125e-7f, tiny = 1.1754943508222875e-38f;
 };
 
-// State of the full set FULL = (1<<D)-1 from the records of its proper
-// subsets.  `rec(m)` returns the record of local sub-mask m.
+// Source of the full set FULL = (1<<D)-1 from the records of its proper
+// subsets (`rec(m)` returns the record of local sub-mask m): the Taylor sum
+// and the two source components (reference :264-282 and :337-350).
 template <int D, class Acc, typename T>
-__device__ __forceinline__ void subset_state(const Acc& rec, const RecT<T>& S, const Model& md, TopOut<T>& o) {
+__device__ __forceinline__ void subset_sources(const Acc& rec, const RecT<T>& S, cpx<T>& taylor, cpx<T>& src0,
+                                               cpx<T>& src1) {
   constexpr int FULL = (1 << D) - 1;
-  const T g = T(md.g);
-  cpx<T> horiz = {0, 0}, kin = {0, 0}, taylor = {0, 0}, lv = {0, 0}, lt = {0, 0};
+  cpx<T> horiz = {0, 0}, kin = {0, 0}, lv = {0, 0}, lt = {0, 0};
+  taylor = {0, 0};
 #pragma unroll
   for (int L = 1; L < FULL; ++L) {
     const int R = FULL ^ L;
@@ -157,11 +164,55 @@ __device__ __forceinline__ void subset_state(const Acc& rec, const RecT<T>& S, c
     lt = ca(lt, ci(-p.om, pt));
   }
   // source = [-(horiz - lv) - q taylor, -(lt + kin) - i Omega taylor]
-  o.src0 = {-(horiz.x - lv.x) - S.q * taylor.x, -(horiz.y - lv.y) - S.q * taylor.y};
+  src0 = {-(horiz.x - lv.x) - S.q * taylor.x, -(horiz.y - lv.y) - S.q * taylor.y};
   const cpx<T> it = ci(S.om, taylor);
-  o.src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
+  src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
+}
+
+// Strict zero of a subset's wavevector (reference :356-364).
+template <typename T>
+__device__ __forceinline__ bool strict_zero(const RecT<T>& S) {
   const T mag = sqrt(S.k2);
-  o.zero = mag <= T(100) * Limits<T>::eps * fmax(mag, T(1));
+  return mag <= T(100) * Limits<T>::eps * fmax(mag, T(1));
+}
+
+// Free-wave split of a forced response whose frequency is `ts` times the
+// dispersion frequency of its wavevector (reference :370-411): the retained
+// part `ret` and the bound part (eta, psi).
+template <typename T>
+__device__ __forceinline__ void retained_split(cpx<T> f0, cpx<T> f1, T q, T om, T ts, T g, cpx<T>& ret, cpx<T>& eta,
+                                               cpx<T>& psi) {
+  const T w = sqrt(g * q), ms = sqrt(w / (T(2) * g)), ns = sqrt(g / (T(2) * w));
+  const cpx<T> a0 = cs(f0, T(1) / (T(2) * ms));
+  const cpx<T> a1 = ci(ts, cs(f1, T(1) / (T(2) * ns)));
+  ret = ca(a0, a1);
+  const cpx<T> comp = glw::csub(a0, a1);
+  const T den = om + ts * w;
+  eta = cs(ci(ms, comp), T(1) / den);
+  psi = cs(comp, -ts * ns / den);
+}
+
+// 2x2 forced linear response at wavevector K (q = q(K)) and frequency om
+// (reference :111-121).
+template <typename T>
+__device__ __forceinline__ void forced_solve(cpx<T> f0, cpx<T> f1, T q, T om, T g, cpx<T>& eta, cpx<T>& psi) {
+  const T den = om * om - g * q;
+  const cpx<T> a = glw::csub(ci(om, f0), cs(f1, q));
+  const cpx<T> b = ca(cs(f0, g), ci(om, f1));
+  eta = cs(a, T(1) / den);
+  psi = cs(b, T(1) / den);
+}
+
+// State of the full set FULL = (1<<D)-1 from the records of its proper
+// subsets.  split != 0 (a cubic subset of charge +-1, sign `split`) keeps the
+// free-wave part (:370-386); degree 4 keeps it when near resonant.
+template <int D, class Acc, typename T>
+__device__ __forceinline__ void subset_state(const Acc& rec, const RecT<T>& S, const Model& md, TopOut<T>& o,
+                                             T split = T(0)) {
+  const T g = T(md.g);
+  cpx<T> taylor;
+  subset_sources<D>(rec, S, taylor, o.src0, o.src1);
+  o.zero = strict_zero(S);
   o.near = false;
   o.rho = T(0);
   o.retained = {0, 0};
@@ -170,31 +221,14 @@ __device__ __forceinline__ void subset_state(const Acc& rec, const RecT<T>& S, c
     o.eta = o.psi = {0, 0};
     o.zero_src = hypot(cabs2(o.src0), cabs2(o.src1));
   } else {
-    bool solve = true;
     if constexpr (D == 4) {
       const T lin = g * S.q, tmp = S.om * S.om;
       o.rho = fabs(lin - tmp) / fmax(lin + tmp, Limits<T>::tiny);
-      if (o.rho < T(md.thr)) {
-        const T w = sqrt(lin), ms = sqrt(w / (T(2) * g)), ns = sqrt(g / (T(2) * w));
-        const T sg = S.om >= T(0) ? T(1) : T(-1);
-        const cpx<T> a0 = cs(o.src0, T(1) / (T(2) * ms));
-        const cpx<T> a1 = ci(sg, cs(o.src1, T(1) / (T(2) * ns)));
-        o.retained = ca(a0, a1);
-        const cpx<T> comp = glw::csub(a0, a1);
-        const T den = S.om + sg * w;
-        o.eta = cs(ci(ms, comp), T(1) / den);
-        o.psi = cs(comp, -sg * ns / den);
-        o.near = true;
-        solve = false;
-      }
+      o.near = o.rho < T(md.thr);
+      split = o.near ? (S.om >= T(0) ? T(1) : T(-1)) : T(0);
     }
-    if (solve) {
-      const T den = S.om * S.om - g * S.q;
-      const cpx<T> a = glw::csub(ci(S.om, o.src0), cs(o.src1, S.q));
-      const cpx<T> b = ca(cs(o.src0, g), ci(S.om, o.src1));
-      o.eta = cs(a, T(1) / den);
-      o.psi = cs(b, T(1) / den);
-    }
+    if (split != T(0)) retained_split(o.src0, o.src1, S.q, S.om, split, g, o.retained, o.eta, o.psi);
+    else forced_solve(o.src0, o.src1, S.q, S.om, g, o.eta, o.psi);
   }
   o.phi = glw::csub(o.psi, taylor);
 }
@@ -245,6 +279,12 @@ struct WitDev {
   RecT<float>* triple32;
   const long long* offP; // [M+1]  pairs with first index < i
   const long long* offT; // [M+1]  triples with first index < i
+  // Signed subsets (sectors 40 and 42; built by their first call).  Flipping
+  // the sign of every parent conjugates a state (wavevector and frequency
+  // negate), so the sign patterns (+-) and (++-) give every signed subset.
+  Rec* mpair;            // [M][M]  {a+, b-} at a M + b
+  Rec* mtrip;            // [P][M]  {a+, b+, c-} (a <= b) at pidx(a, b) M + c
+  cpx<double>* mtrip_x;  // [P][M][2] psi and retained V3 of those triples
 };
 
 __device__ __forceinline__ long long pidx(const WitDev& w, int i, int j) { return w.offP[i] + (j - i); }
@@ -414,6 +454,19 @@ __device__ __forceinline__ double wmin_d(double v) {
   return v;
 }
 
+// f -> (k, l), 0 <= k <= l < n, in row-major order of the upper triangle:
+// row k starts at k n - k (k - 1) / 2.
+__device__ __forceinline__ void unrank_upper(long long f, int n, int& k, int& l) {
+  const double disc = (2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * (double)f;
+  int kk = (int)floor(((2.0 * n + 1.0) - sqrt(fmax(disc, 0.0))) / 2.0);
+  if (kk < 0) kk = 0;
+  if (kk > n - 1) kk = n - 1;
+  while (kk > 0 && (long long)kk * n - (long long)kk * (kk - 1) / 2 > f) --kk;
+  while (kk + 1 < n && (long long)(kk + 1) * n - (long long)(kk + 1) * kk / 2 <= f) ++kk;
+  k = kk;
+  l = kk + (int)(f - ((long long)kk * n - (long long)kk * (kk - 1) / 2));
+}
+
 // One workgroup per leading pair (i, j); threads sweep the (k, l) triangle.
 // T = arithmetic of the degree-4 evaluation (the subset tables are built in
 // FP64; the FP32 path reads FP32 copies).  Weights and accumulation are FP64
@@ -446,15 +499,9 @@ __global__ __launch_bounds__(WNT, OCC) void k_wit_quartets(QuartetArgs g) {
     __syncthreads();
   }
   for (long long f = threadIdx.x; f < tri; f += WNT) {
-    // unrank f -> (k', l'), 0 <= k' <= l' < n: row k' starts at k' n - k'(k'-1)/2
-    double disc = (2.0 * n + 1.0) * (2.0 * n + 1.0) - 8.0 * (double)f;
-    int kk = (int)floor(((2.0 * n + 1.0) - sqrt(fmax(disc, 0.0))) / 2.0);
-    if (kk < 0) kk = 0;
-    if (kk > n - 1) kk = n - 1;
-    while (kk > 0 && (long long)kk * n - (long long)kk * (kk - 1) / 2 > f) --kk;
-    while (kk + 1 < n && (long long)(kk + 1) * n - (long long)(kk + 1) * kk / 2 <= f) ++kk;
-    const long long start = (long long)kk * n - (long long)kk * (kk - 1) / 2;
-    const int k = j + kk, l = k + (int)(f - start);
+    int kk, ll;
+    unrank_upper(f, n, kk, ll);
+    const int k = j + kk, l = j + ll;
     const int tx = w.modes[2 * i] + w.modes[2 * j] + w.modes[2 * k] + w.modes[2 * l];
     const int ty = w.modes[2 * i + 1] + w.modes[2 * j + 1] + w.modes[2 * k + 1] + w.modes[2 * l + 1];
     if (tx == 0 && ty == 0) {
@@ -577,266 +624,6 @@ __global__ __launch_bounds__(WNT, OCC) void k_wit_quartets(QuartetArgs g) {
   }
 }
 
-// ----------------------------------------------------------------------------
-// Sectors 40 and 42: quartets with signed parents (directional_eta4_openmp.cpp
-// sector loops :603-688).  A subset's state now depends on its members' signs,
-// so nothing is tabulated: every candidate evaluates all 15 subset states
-// (reference evaluate, :157-453) in registers and private memory, including
-// the two terms that vanish in sector 44: the retained-V3 state of a cubic
-// subset whose charge is not +-3 (and its push on the quartic source,
-// :282-298) and the y31 coordinate of the quartic output (:300-336).
-// ----------------------------------------------------------------------------
-struct Sgn {
-  double g, h, thr;
-  __device__ double q(double kx, double ky) const {
-    const double r = hypot(kx, ky);
-    return r * tanh(h * r);
-  }
-  __device__ double om(double kx, double ky) const { return sqrt(g * q(kx, ky)); }
-  // vertical-derivative factor |k|^n -> q, k^2, k^2 q, k^4 (n = 0..4)
-  __device__ double zd(double kx, double ky, int n) const {
-    const double k2 = kx * kx + ky * ky;
-    switch (n) {
-      case 0: return 1.0;
-      case 1: return q(kx, ky);
-      case 2: return k2;
-      case 3: return k2 * q(kx, ky);
-      default: return k2 * k2;
-    }
-  }
-  // 2x2 forced linear response (:111-121)
-  __device__ void solve(cpx<double> s0, cpx<double> s1, double kx, double ky, double w, cpx<double>& eta,
-                        cpx<double>& psi) const {
-    const double qq = q(kx, ky), den = w * w - g * qq;
-    eta = {(-w * s0.y - qq * s1.x) / den, (w * s0.x - qq * s1.y) / den};
-    psi = {(g * s0.x - w * s1.y) / den, (g * s0.y + w * s1.x) / den};
-  }
-  __device__ double g1(double ax, double ay, double bx, double by) const {
-    return -q(ax + bx, ay + by) * q(bx, by) + (bx * bx + by * by) + (ax * bx + ay * by);
-  }
-  // second-order response of two physical states (:136-146)
-  __device__ void cross(double ax, double ay, double wa, cpx<double> ea, cpx<double> pa, double bx, double by,
-                        double wb, cpx<double> eb, cpx<double> pb, cpx<double>& eta, cpx<double>& psi) const {
-    eta = psi = {0.0, 0.0};
-    if (hypot(ax + bx, ay + by) == 0.0) return;
-    const double gab = g1(ax, ay, bx, by), gba = g1(bx, by, ax, ay);
-    const cpx<double> s0 = glw::cadd(cs(glw::cmul(ea, pb), gab), cs(glw::cmul(eb, pa), gba));
-    const cpx<double> s1 = cs(glw::cmul(pa, pb), ax * bx + ay * by + q(ax, ay) * q(bx, by));
-    solve(s0, s1, ax + bx, ay + by, wa + wb, eta, psi);
-  }
-  // normal-form map of a cubic mode (unit normal amplitude) and a parent (:122-135)
-  __device__ void normal(double ax, double ay, int sa, double bx, double by, int sb, cpx<double>& eta,
-                         cpx<double>& psi) const {
-    const double wa = om(ax, ay), wb = om(bx, by);
-    const double ma = sqrt(wa / (2.0 * g)), na = sqrt(g / (2.0 * wa));
-    const cpx<double> ea = {ma, 0.0}, pa = {0.0, -sa * na};
-    const cpx<double> eb = {0.5, 0.0}, pb = {0.0, -sb * wb / (2.0 * q(bx, by))};
-    const double gab = g1(ax, ay, bx, by), gba = g1(bx, by, ax, ay);
-    const cpx<double> s0 = glw::cadd(cs(glw::cmul(ea, pb), gab), cs(glw::cmul(eb, pa), gba));
-    const cpx<double> s1 = cs(glw::cmul(pa, pb), ax * bx + ay * by + q(ax, ay) * q(bx, by));
-    solve(s0, s1, ax + bx, ay + by, sa * wa + sb * wb, eta, psi);
-  }
-};
-
-struct SignedOut {
-  cpx<double> eta, psi, ecoord, pcoord, retained;
-  double rho, residual, zsrc;
-  int zcount;
-  bool near;
-};
-
-__device__ __forceinline__ cpx<double> mul_i(double s, cpx<double> a) { return {-s * a.y, s * a.x}; }  // (i s) a
-
-// State of the four-parent set for parents `pk` (physical wavevectors) with signs `sg`.
-__device__ SignedOut evaluate_signed(const Sgn& m, const double (&pkx)[4], const double (&pky)[4], const int (&sg)[4]) {
-  const double inv_fact[4] = {1.0, 1.0, 0.5, 1.0 / 6.0};
-  double kx[16], ky[16], w[16];
-  int deg[16], chg[16];
-  double skx[4], sky[4], sw[4];
-  for (int s = 0; s < 4; ++s) {
-    skx[s] = sg[s] * pkx[s];
-    sky[s] = sg[s] * pky[s];
-    sw[s] = sg[s] * m.om(pkx[s], pky[s]);
-  }
-  for (int mask = 0; mask < 16; ++mask) {
-    kx[mask] = ky[mask] = w[mask] = 0.0;
-    deg[mask] = chg[mask] = 0;
-    for (int s = 0; s < 4; ++s)
-      if (mask >> s & 1) {
-        kx[mask] += skx[s];
-        ky[mask] += sky[s];
-        w[mask] += sw[s];
-        deg[mask] += 1;
-        chg[mask] += sg[s];
-      }
-  }
-  cpx<double> eta[16], phi[16], psi[16], ret[16], shx[16], shy[16], sv[16];
-  for (int mask = 0; mask < 16; ++mask) eta[mask] = phi[mask] = psi[mask] = ret[mask] = shx[mask] = shy[mask] = sv[mask] = {0.0, 0.0};
-  SignedOut o{};
-  cpx<double> eff0 = {0.0, 0.0}, eff1 = {0.0, 0.0};
-  // Partitions of `mask` into nb labelled non-empty blocks: the members are
-  // assigned base-nb digits of `code`; block 0 carries the potential, blocks
-  // 1.. the elevations (product `ep`).  f(block0, order, ep) is applied to each.
-  auto partitions = [&](int mask, int nb, auto&& f) {
-    int slots[4], n = 0;
-    for (int s = 0; s < 4; ++s)
-      if (mask >> s & 1) slots[n++] = s;
-    int count = 1;
-    for (int i = 0; i < n; ++i) count *= nb;
-    for (int code = 0; code < count; ++code) {
-      int blocks[4] = {0, 0, 0, 0}, seen = 0, c = code;
-      for (int i = 0; i < n; ++i) {
-        const int a = c % nb;
-        c /= nb;
-        seen |= 1 << a;
-        blocks[a] |= 1 << slots[i];
-      }
-      if (seen != (1 << nb) - 1) continue;
-      cpx<double> ep = {1.0, 0.0};
-      for (int b = 1; b < nb; ++b) ep = glw::cmul(ep, eta[blocks[b]]);
-      f(blocks, ep);
-    }
-  };
-  for (int d = 1; d <= 4; ++d) {
-    for (int mask = 1; mask < 16; ++mask) {
-      if (deg[mask] != d) continue;
-      cpx<double> taylor = {0.0, 0.0};
-      if (d == 1) {
-        eta[mask] = {0.5, 0.0};
-        phi[mask] = {0.0, -w[mask] / (2.0 * m.q(kx[mask], ky[mask]))};
-        psi[mask] = phi[mask];
-      } else {
-        // ordinary two-block source (:264-282) and the Taylor terms of orders 1..d-1
-        cpx<double> hor = {0.0, 0.0}, kin = {0.0, 0.0}, lv = {0.0, 0.0}, lt = {0.0, 0.0};
-        partitions(mask, 2, [&](const int* b, cpx<double>) {
-          const int l = b[0], r = b[1];
-          const cpx<double> hk = glw::cadd(glw::cmul(shx[l], {0.0, kx[r]}), glw::cmul(shy[l], {0.0, ky[r]}));
-          hor = glw::cadd(hor, glw::cmul(hk, eta[r]));
-          const cpx<double> kk = glw::cadd(glw::cadd(glw::cmul(shx[l], shx[r]), glw::cmul(shy[l], shy[r])),
-                                           glw::cmul(sv[l], sv[r]));
-          kin = glw::cadd(kin, cs(kk, 0.5));
-        });
-        for (int order = 1; order < d; ++order)
-          partitions(mask, order + 1, [&](const int* b, cpx<double> ep) {
-            const int pm = b[0];
-            const cpx<double> base = cs(glw::cmul(phi[pm], ep), inv_fact[order]);
-            const double z0 = m.zd(kx[pm], ky[pm], order), z1 = m.zd(kx[pm], ky[pm], order + 1);
-            taylor = glw::cadd(taylor, cs(base, z0));
-            lv = glw::cadd(lv, cs(base, z1));
-            lt = glw::cadd(lt, mul_i(-w[pm] * z0, base));
-          });
-        const double qm = m.q(kx[mask], ky[mask]);
-        const cpx<double> src0 = {-(hor.x - lv.x) - qm * taylor.x, -(hor.y - lv.y) - qm * taylor.y};
-        const cpx<double> it = mul_i(w[mask], taylor);
-        const cpx<double> src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
-        const double mag = hypot(kx[mask], ky[mask]);
-        if (mag <= 100.0 * 2.220446049250313e-16 * fmax(mag, 1.0)) {  // strict zero (:356-364)
-          eta[mask] = psi[mask] = {0.0, 0.0};
-          o.zcount += 1;
-          o.zsrc = fmax(o.zsrc, hypot(hypot(src0.x, src0.y), hypot(src1.x, src1.y)));
-        } else if (d == 2 || (d == 3 && abs(chg[mask]) == 3)) {
-          m.solve(src0, src1, kx[mask], ky[mask], w[mask], eta[mask], psi[mask]);
-        } else {
-          cpx<double> f0 = src0, f1 = src1;
-          bool resonant = d == 3;  // a cubic subset of charge +-1 keeps its free-wave part (:370-386)
-          int ts = chg[mask] > 0 ? 1 : -1;
-          if (d == 4) {
-            // push of the retained cubic states on the quartic source (:282-298)
-            for (int s = 0; s < 4; ++s) {
-              const int cm = mask & ~(1 << s);
-              if (ret[cm].x == 0.0 && ret[cm].y == 0.0) continue;
-              cpx<double> pe, pp;
-              m.normal(kx[cm], ky[cm], chg[cm] > 0 ? 1 : -1, skx[s], sky[s], sg[s], pe, pp);
-              f0 = glw::csub(f0, glw::cmul(ret[cm], pe));
-              f1 = glw::csub(f1, glw::cmul(ret[cm], pp));
-            }
-            eff0 = f0;
-            eff1 = f1;
-            // y31 coordinate (:300-336)
-            for (int s = 0; s < 4; ++s) {
-              const int cm = mask & ~(1 << s), cc = chg[cm];
-              if (abs(cc) != 1) continue;
-              if (hypot(eta[cm].x, eta[cm].y) + hypot(psi[cm].x, psi[cm].y) == 0.0) continue;
-              const double cf = m.om(kx[cm], ky[cm]);
-              const cpx<double> ps = mul_i(cc * cf / m.g, psi[cm]);
-              const cpx<double> ae = cs(glw::cadd(eta[cm], ps), 0.5), oe = cs(glw::csub(eta[cm], ps), 0.5);
-              const cpx<double> ap = mul_i(-m.g * cc / cf, ae), op = mul_i(m.g * cc / cf, oe);
-              const double sq = m.q(skx[s], sky[s]);
-              const cpx<double> se = {0.5, 0.0}, spsi = {0.0, -sg[s] * m.om(pkx[s], pky[s]) / (2.0 * sq)};
-              cpx<double> e1, p1, e2, p2;
-              m.cross(kx[cm], ky[cm], cc * cf, ae, ap, skx[s], sky[s], sw[s], se, spsi, e1, p1);
-              m.cross(kx[cm], ky[cm], -cc * cf, oe, op, skx[s], sky[s], sw[s], se, spsi, e2, p2);
-              o.ecoord = glw::cadd(o.ecoord, glw::cadd(e1, e2));
-              o.pcoord = glw::cadd(o.pcoord, glw::cadd(p1, p2));
-            }
-            const double qo = m.q(kx[mask], ky[mask]), lin = m.g * qo, tmp = w[mask] * w[mask];
-            const double rho = fabs(lin - tmp) / fmax(lin + tmp, 2.2250738585072014e-308);
-            resonant = rho < m.thr;
-            ts = w[mask] >= 0.0 ? 1 : -1;
-          }
-          if (resonant) {
-            const double wo = m.om(kx[mask], ky[mask]);
-            const double ms = sqrt(wo / (2.0 * m.g)), ns = sqrt(m.g / (2.0 * wo));
-            const cpx<double> a0 = cs(f0, 1.0 / (2.0 * ms)), a1 = mul_i(ts, cs(f1, 1.0 / (2.0 * ns)));
-            ret[mask] = glw::cadd(a0, a1);
-            const cpx<double> comp = glw::csub(a0, a1);
-            const double den = w[mask] + ts * wo;
-            eta[mask] = cs(mul_i(ms, comp), 1.0 / den);
-            psi[mask] = cs(comp, -ts * ns / den);
-          } else {
-            m.solve(f0, f1, kx[mask], ky[mask], w[mask], eta[mask], psi[mask]);
-          }
-        }
-        phi[mask] = glw::csub(psi[mask], taylor);
-      }
-      if (d < 4) {  // surface derivatives of this subset, orders 0..d-1 (:185-225)
-        cpx<double> hx = {0.0, 0.0}, hy = {0.0, 0.0}, v = {0.0, 0.0};
-        for (int order = 0; order < d; ++order)
-          partitions(mask, order + 1, [&](const int* b, cpx<double> ep) {
-            const int pm = b[0];
-            const cpx<double> base = cs(glw::cmul(phi[pm], ep), inv_fact[order]);
-            const double z0 = m.zd(kx[pm], ky[pm], order), z1 = m.zd(kx[pm], ky[pm], order + 1);
-            hx = glw::cadd(hx, mul_i(kx[pm] * z0, base));
-            hy = glw::cadd(hy, mul_i(ky[pm] * z0, base));
-            v = glw::cadd(v, cs(base, z1));
-          });
-        shx[mask] = hx;
-        shy[mask] = hy;
-        sv[mask] = v;
-      }
-    }
-  }
-  // residual of the quartic output against its effective source (:413-433)
-  const double qo = m.q(kx[15], ky[15]), lin = m.g * qo, tmp = w[15] * w[15];
-  o.rho = fabs(lin - tmp) / fmax(lin + tmp, 2.2250738585072014e-308);
-  cpx<double> er = glw::csub(glw::csub(mul_i(-w[15], eta[15]), cs(psi[15], qo)), eff0);
-  cpx<double> pr = glw::csub(glw::cadd(mul_i(-w[15], psi[15]), cs(eta[15], m.g)), eff1);
-  o.near = o.rho < m.thr;
-  if (o.near) {
-    const double wo = m.om(kx[15], ky[15]);
-    const double ms = sqrt(wo / (2.0 * m.g)), ns = sqrt(m.g / (2.0 * wo));
-    const int ts = w[15] >= 0.0 ? 1 : -1;
-    er = glw::cadd(er, cs(ret[15], ms));
-    pr = glw::cadd(pr, mul_i(-ts * ns, ret[15]));
-  }
-  o.eta = eta[15];
-  o.psi = psi[15];
-  o.retained = ret[15];
-  o.residual = hypot(hypot(er.x, er.y), hypot(pr.x, pr.y));
-  return o;
-}
-
-struct SignedArgs {
-  WitDev w;
-  double dkx, dky;
-  const cpx<double>* amps;  // [cases][M]
-  int cases;
-  double* out;              // [5][cases][ny][nx] complex
-  unsigned long long* cnt;
-  unsigned long long* dmax;
-  long long total;          // candidates
-};
-
 // pair index p (i <= j, row i starting at offP[i]) -> (i, j)
 __device__ __forceinline__ void unrank_pair(const WitDev& w, long long p, int& i, int& j) {
   int lo = 0, hi = w.M - 1;
@@ -848,55 +635,149 @@ __device__ __forceinline__ void unrank_pair(const WitDev& w, long long p, int& i
   j = i + (int)(p - w.offP[i]);
 }
 
-// triple index t (i <= j <= k, tidx order) -> (i, j, k)
-__device__ __forceinline__ void unrank_triple(const WitDev& w, long long t, int& i, int& j, int& k) {
-  int lo = 0, hi = w.M - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) / 2;
-    if (w.offT[mid] <= t) lo = mid; else hi = mid - 1;
-  }
-  i = lo;
-  int a, b;
-  unrank_pair(w, w.offP[i] + (t - w.offT[i]), a, b);
-  j = a;
-  k = b;
+// ----------------------------------------------------------------------------
+// Tabulated signed sectors.
+// ----------------------------------------------------------------------------
+// State of the same subset with every parent sign flipped: the conjugate
+// state at the negated wavevector and frequency.
+__device__ __forceinline__ Rec flip(Rec r) {
+  r.eta.y = -r.eta.y;
+  r.phi.y = -r.phi.y;
+  r.sh0.y = -r.sh0.y;
+  r.sh1.y = -r.sh1.y;
+  r.sv.y = -r.sv.y;
+  r.kx = -r.kx;
+  r.ky = -r.ky;
+  r.om = -r.om;
+  return r;
+}
+__device__ __forceinline__ cpx<double> conj(cpx<double> a) { return {a.x, -a.y}; }
+
+struct LocalAcc {
+  const Rec* r;
+  __device__ const Rec& operator()(int m) const { return r[m]; }
+};
+
+// {a+, b-} for a, b in [0, M): ordinary pair solve (:357-369).
+__global__ void k_wit_mixed_pairs(WitDev w) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= (long long)w.M * w.M) return;
+  const int a = (int)(t / w.M), b = (int)(t % w.M);
+  Rec loc[4];
+  loc[1] = w.single[a];
+  loc[2] = flip(w.single[b]);
+  const Rec* sub[2] = {&loc[1], &loc[2]};
+  Rec S;
+  base_record(w, sub, 2, S);
+  TopOut<double> o;
+  const LocalAcc acc{loc};
+  subset_state<2>(acc, S, w.md, o);
+  S.eta = o.eta;
+  S.phi = o.phi;
+  S.zero = o.zero ? 1.0 : 0.0;
+  S.zero_src = o.zero_src;
+  surface<2>(acc, S, o.phi, S);
+  w.mpair[t] = S;
 }
 
-// SECTOR 40: candidate c = minus pair * P + plus pair (signs + + - -);
-// SECTOR 42: c = minus mode * T + plus triple (signs + + + -).
-// Four waves per SIMD (128 VGPRs, 2.8 KB of private memory per lane): the per-candidate
-// evaluation is a long dependent FP64 chain, so waves in flight matter more than spills
-// (M=200 on MI355X: 1.52x (sector 42) and 1.63x (sector 40) over one wave with 348 registers;
-// each step 1 -> 2 -> 3 -> 4 waves was faster).
+// {a+, b+, c-} (a <= b): a cubic subset of charge +1 keeps its free-wave
+// part (:370-386).
+__global__ void k_wit_mixed_triples(WitDev w, long long n) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long long p = t / w.M;
+  const int c = (int)(t % w.M);
+  int a, b;
+  unrank_pair(w, p, a, b);
+  Rec loc[8];
+  loc[1] = w.single[a];
+  loc[2] = w.single[b];
+  loc[4] = flip(w.single[c]);
+  loc[3] = w.pair[p];
+  loc[5] = w.mpair[(long long)a * w.M + c];
+  loc[6] = w.mpair[(long long)b * w.M + c];
+  const Rec* sub[3] = {&loc[1], &loc[2], &loc[4]};
+  Rec S;
+  base_record(w, sub, 3, S);
+  TopOut<double> o;
+  const LocalAcc acc{loc};
+  subset_state<3>(acc, S, w.md, o, 1.0);
+  S.eta = o.eta;
+  S.phi = o.phi;
+  S.zero = o.zero ? 1.0 : 0.0;
+  S.zero_src = o.zero_src;
+  surface<3>(acc, S, o.phi, S);
+  w.mtrip[t] = S;
+  w.mtrip_x[2 * t] = o.psi;
+  w.mtrip_x[2 * t + 1] = o.retained;
+}
+
+// Record accessor of a signed quartet: sub-mask m reads *r[m], flipped when
+// bit m of FLIPS is set.
+template <int FLIPS>
+struct SignedAcc {
+  const Rec* r[16];
+  __device__ Rec operator()(int m) const { return (FLIPS >> m & 1) ? flip(*r[m]) : *r[m]; }
+};
+
+struct SignedTabArgs {
+  WitDev w;
+  const cpx<double>* amps;  // [cases][M]
+  int cases;
+  double* out;              // [5][cases][ny][nx] complex
+  unsigned long long* cnt;
+  unsigned long long* dmax;
+  // LDS window of a workgroup (0 bins: global atomics): bin
+  // (ty - sy + offy) pw + (tx - sx + offx), (sx, sy) = mode(i) + mode(j).
+  int lds_bins, pw, ph, offx, offy;
+};
+
+// Sector 40 (signs + + - -; plus pair (i, j), minus pair (k, l)) and sector 42
+// (signs + + + -; plus triple (i, j, k), minus mode l), reference sector loops
+// :603-688 and evaluate :157-453.  One workgroup per plus pair (i, j) =
+// leading pair; its threads sweep the minus pairs (k <= l) (40) or the
+// (k >= j, l) rectangle (42).  Subset states come from the tables; per
+// candidate only the degree-4 level is evaluated: the partition-sum source,
+// the push of the four cubic subsets' retained V3 through the normal map
+// (:282-298), the y31 coordinate (:300-336), the near-resonance split and the
+// residual (:393-433).
 template <int SECTOR>
-__global__ __launch_bounds__(WNT, 4) void k_wit_signed(SignedArgs g) {
+__global__ __launch_bounds__(WNT, 1) void k_wit_signed_tab(SignedTabArgs g) {
   const WitDev& w = g.w;
-  const Sgn m{w.md.g, w.md.h, w.md.thr};
+  const int M = w.M;
+  int i, j;
+  unrank_pair(w, blockIdx.x, i, j);
+  const long long pij = blockIdx.x;
+  const long long nf = SECTOR == 40 ? w.offP[M] : (long long)(M - j) * M;
+  constexpr int FLIPS = SECTOR == 40 ? (1 << 4 | 1 << 8 | 1 << 12 | 1 << 13 | 1 << 14) : (1 << 8);
+  // charge of the cubic sub-mask without parent s (cm = 15 ^ (1 << s))
+  constexpr int kCharge[4] = {SECTOR == 40 ? -1 : 1, SECTOR == 40 ? -1 : 1, SECTOR == 40 ? 1 : 1,
+                              SECTOR == 40 ? 1 : 3};
+  constexpr int kSign[4] = {1, 1, SECTOR == 40 ? -1 : 1, -1};
+  const double gv = w.md.g;
   unsigned long long kept = 0, nearc = 0, zo = 0, zi = 0;
   double maxres = 0.0, maxz = 0.0, mindet = INFINITY;
   const size_t plane = (size_t)w.nx * w.ny * g.cases;
-  for (long long c = blockIdx.x * (long long)WNT + threadIdx.x; c < g.total; c += (long long)gridDim.x * WNT) {
-    int id[4], sg[4];
-    int repeat;
+  extern __shared__ double wacc[];
+  const int sx = w.modes[2 * i] + w.modes[2 * j], sy = w.modes[2 * i + 1] + w.modes[2 * j + 1];
+  if (g.lds_bins) {
+    for (int e = threadIdx.x; e < g.lds_bins * g.cases * 8; e += WNT) wacc[e] = 0.0;
+    __syncthreads();
+  }
+  for (long long f = threadIdx.x; f < nf; f += WNT) {
+    int k, l;
     if constexpr (SECTOR == 40) {
-      const long long P = w.offP[w.M];
-      unrank_pair(w, c % P, id[0], id[1]);
-      unrank_pair(w, c / P, id[2], id[3]);
-      sg[0] = sg[1] = 1;
-      sg[2] = sg[3] = -1;
-      repeat = (1 + (id[0] == id[1])) * (1 + (id[2] == id[3]));
+      unrank_upper(f, M, k, l);
     } else {
-      const long long T = w.offT[w.M];
-      unrank_triple(w, c % T, id[0], id[1], id[2]);
-      id[3] = (int)(c / T);
-      sg[0] = sg[1] = sg[2] = 1;
-      sg[3] = -1;
-      repeat = id[0] == id[2] ? 6 : ((id[0] == id[1] || id[1] == id[2]) ? 2 : 1);
+      k = j + (int)(f / M);
+      l = (int)(f % M);
     }
+    const int id[4] = {i, j, k, l};
     int tx = 0, ty = 0;
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
-      tx += sg[s] * w.modes[2 * id[s]];
-      ty += sg[s] * w.modes[2 * id[s] + 1];
+      tx += kSign[s] * w.modes[2 * id[s]];
+      ty += kSign[s] * w.modes[2 * id[s] + 1];
     }
     if (tx == 0 && ty == 0) {
       ++zo;
@@ -904,34 +785,176 @@ __global__ __launch_bounds__(WNT, 4) void k_wit_signed(SignedArgs g) {
     }
     if (SECTOR == 40 && !(tx > 0 || (tx == 0 && ty > 0))) continue;  // analytic half plane (:566-568)
     if (abs(tx) >= w.nx / 2 || abs(ty) >= w.ny / 2) continue;
-    double pkx[4], pky[4];
-    for (int s = 0; s < 4; ++s) {
-      pkx[s] = w.modes[2 * id[s]] * g.dkx;
-      pky[s] = w.modes[2 * id[s] + 1] * g.dky;
+    SignedAcc<FLIPS> a;
+    const cpx<double>* xs[4];  // psi, retained of the cubic sub-mask without parent s
+    a.r[1] = &w.single[i];
+    a.r[2] = &w.single[j];
+    a.r[4] = &w.single[k];
+    a.r[8] = &w.single[l];
+    if constexpr (SECTOR == 40) {
+      const long long pkl = pidx(w, k, l);
+      a.r[3] = &w.pair[pij];
+      a.r[12] = &w.pair[pkl];
+      a.r[5] = &w.mpair[(long long)i * M + k];
+      a.r[9] = &w.mpair[(long long)i * M + l];
+      a.r[6] = &w.mpair[(long long)j * M + k];
+      a.r[10] = &w.mpair[(long long)j * M + l];
+      const long long t7 = pij * M + k, t11 = pij * M + l, t13 = pkl * M + i, t14 = pkl * M + j;
+      a.r[7] = &w.mtrip[t7];
+      a.r[11] = &w.mtrip[t11];
+      a.r[13] = &w.mtrip[t13];
+      a.r[14] = &w.mtrip[t14];
+      xs[0] = w.mtrip_x + 2 * t14;
+      xs[1] = w.mtrip_x + 2 * t13;
+      xs[2] = w.mtrip_x + 2 * t11;
+      xs[3] = w.mtrip_x + 2 * t7;
+    } else {
+      const long long pik = pidx(w, i, k), pjk = pidx(w, j, k);
+      a.r[3] = &w.pair[pij];
+      a.r[5] = &w.pair[pik];
+      a.r[6] = &w.pair[pjk];
+      a.r[9] = &w.mpair[(long long)i * M + l];
+      a.r[10] = &w.mpair[(long long)j * M + l];
+      a.r[12] = &w.mpair[(long long)k * M + l];
+      a.r[7] = &w.triple[tidx(w, i, j, k)];
+      const long long t11 = pij * M + l, t13 = pik * M + l, t14 = pjk * M + l;
+      a.r[11] = &w.mtrip[t11];
+      a.r[13] = &w.mtrip[t13];
+      a.r[14] = &w.mtrip[t14];
+      xs[0] = w.mtrip_x + 2 * t14;
+      xs[1] = w.mtrip_x + 2 * t13;
+      xs[2] = w.mtrip_x + 2 * t11;
+      xs[3] = nullptr;
     }
-    const SignedOut o = evaluate_signed(m, pkx, pky, sg);
-    const size_t ix = (size_t)(((tx % w.nx) + w.nx) % w.nx), iy = (size_t)(((ty % w.ny) + w.ny) % w.ny);
-    for (int cs_ = 0; cs_ < g.cases; ++cs_) {
-      const cpx<double>* ac = g.amps + (size_t)cs_ * w.M;
-      cpx<double> wt = {2.0, 0.0};
+    Rec S;
+    {
+      double kx = 0, ky = 0, om = 0;
+#pragma unroll
       for (int s = 0; s < 4; ++s) {
-        const cpx<double> a = ac[id[s]];
-        wt = glw::cmul(wt, sg[s] > 0 ? a : cpx<double>{a.x, -a.y});
+        const Rec b = a(1 << s);
+        kx += b.kx;
+        ky += b.ky;
+        om += b.om;
       }
+      S.kx = kx;
+      S.ky = ky;
+      S.om = om;
+      S.q = w.md.q(kx, ky, &S.k2);
+    }
+    cpx<double> taylor, f0, f1;
+    subset_sources<4>(a, S, taylor, f0, f1);
+    cpx<double> ecoord = {0, 0}, pcoord = {0, 0};
+    const double qK = S.q;
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int cc = kCharge[s];
+      if (cc != 1 && cc != -1) continue;  // a cubic subset of charge +-3 has no free-wave part
+      const int cmask = 15 ^ (1 << s);
+      const Rec C = a(cmask);
+      const Rec b = a(1 << s);
+      const bool fl = (FLIPS >> cmask) & 1;
+      const cpx<double> psic = fl ? conj(xs[s][0]) : xs[s][0];
+      const cpx<double> retc = fl ? conj(xs[s][1]) : xs[s][1];
+      const double adb = C.kx * b.kx + C.ky * b.ky;
+      const double gab = -qK * b.q + b.k2 + adb, gba = -qK * C.q + C.k2 + adb, kk = adb + C.q * b.q;
+      const double cf = sqrt(gv * C.q);
+      // b.phi = -i (signed frequency) / (2 q): the parent's potential, pb of the maps below
+      if (retc.x != 0.0 || retc.y != 0.0) {  // push through the normal map (unit normal amplitude)
+        const double ma = sqrt(cf / (2.0 * gv)), na = sqrt(gv / (2.0 * cf));
+        const cpx<double> pa = {0.0, -cc * na};
+        const cpx<double> s0 = ca(cs(b.phi, ma * gab), cs(pa, 0.5 * gba));
+        const cpx<double> s1 = cs(cm(pa, b.phi), kk);
+        cpx<double> pe, pp;
+        forced_solve(s0, s1, qK, cc * cf + b.om, gv, pe, pp);
+        f0 = glw::csub(f0, cm(retc, pe));
+        f1 = glw::csub(f1, cm(retc, pp));
+      }
+      if (C.zero == 0.0) {  // y31 coordinate: the cubic state split into its +- frequency parts
+        const cpx<double> ps = ci(cc * cf / gv, psic);
+        const cpx<double> ae = cs(ca(C.eta, ps), 0.5), oe = cs(glw::csub(C.eta, ps), 0.5);
+        const cpx<double> ap = ci(-gv * cc / cf, ae), op = ci(gv * cc / cf, oe);
+        cpx<double> e1, p1, e2, p2;
+        forced_solve(ca(cs(cm(ae, b.phi), gab), cs(ap, 0.5 * gba)), cs(cm(ap, b.phi), kk), qK, cc * cf + b.om, gv,
+                     e1, p1);
+        forced_solve(ca(cs(cm(oe, b.phi), gab), cs(op, 0.5 * gba)), cs(cm(op, b.phi), kk), qK, -cc * cf + b.om, gv,
+                     e2, p2);
+        ecoord = ca(ecoord, ca(e1, e2));
+        pcoord = ca(pcoord, ca(p1, p2));
+      }
+    }
+    const double lin = gv * qK, tmp = S.om * S.om;
+    const double rho = fabs(lin - tmp) / fmax(lin + tmp, Limits<double>::tiny);
+    const bool near = rho < w.md.thr;
+    const double ts = S.om >= 0.0 ? 1.0 : -1.0;
+    cpx<double> eta, psi, ret = {0.0, 0.0};
+    if (near) retained_split(f0, f1, qK, S.om, ts, gv, ret, eta, psi);
+    else forced_solve(f0, f1, qK, S.om, gv, eta, psi);
+    // residual against the pushed source (:413-433)
+    cpx<double> er = glw::csub(glw::csub(ci(-S.om, eta), cs(psi, qK)), f0);
+    cpx<double> pr = glw::csub(ca(ci(-S.om, psi), cs(eta, gv)), f1);
+    if (near) {
+      const double wo = sqrt(lin), ms = sqrt(wo / (2.0 * gv)), ns = sqrt(gv / (2.0 * wo));
+      er = ca(er, cs(ret, ms));
+      pr = ca(pr, ci(-ts * ns, ret));
+    }
+    const double res = hypot(cabs2(er), cabs2(pr));
+    double zcount = 0.0, zsrc = 0.0;
+    constexpr int kInner[10] = {3, 5, 6, 9, 10, 12, 7, 11, 13, 14};
+#pragma unroll
+    for (int q = 0; q < 10; ++q) {
+      zcount += a.r[kInner[q]]->zero;
+      zsrc = fmax(zsrc, a.r[kInner[q]]->zero_src);
+    }
+    const int repeat = SECTOR == 40 ? (1 + (i == j)) * (1 + (k == l))
+                                    : (i == k ? 6 : ((i == j || j == k) ? 2 : 1));
+    const size_t ix = (size_t)(((tx % w.nx) + w.nx) % w.nx), iy = (size_t)(((ty % w.ny) + w.ny) % w.ny);
+    for (int c = 0; c < g.cases; ++c) {
+      const cpx<double>* ac = g.amps + (size_t)c * M;
+      cpx<double> wt = {2.0, 0.0};
+#pragma unroll
+      for (int s = 0; s < 4; ++s) wt = cm(wt, kSign[s] > 0 ? ac[id[s]] : conj(ac[id[s]]));
       wt = {wt.x / repeat, wt.y / repeat};
-      const size_t loc = (size_t)cs_ * w.nx * w.ny + ix + (size_t)w.nx * iy;
-      atomic_addc(g.out + 2 * loc, glw::cmul(wt, o.eta));
-      atomic_addc(g.out + 2 * (plane + loc), glw::cmul(wt, o.psi));
-      atomic_addc(g.out + 2 * (2 * plane + loc), glw::cmul(wt, o.ecoord));
-      atomic_addc(g.out + 2 * (3 * plane + loc), glw::cmul(wt, o.pcoord));
-      if (o.near) atomic_addc(g.out + 2 * (4 * plane + loc), glw::cmul(wt, o.retained));
+      const cpx<double> ve = cm(wt, eta), vp = cm(wt, psi), vc = cm(wt, ecoord), vq = cm(wt, pcoord);
+      const size_t loc = (size_t)c * w.nx * w.ny + ix + (size_t)w.nx * iy;
+      if (g.lds_bins) {
+        const int lb = (ty - sy + g.offy) * g.pw + (tx - sx + g.offx);
+        double* a8 = wacc + ((size_t)lb * g.cases + c) * 8;
+        atomicAdd(a8 + 0, ve.x);
+        atomicAdd(a8 + 1, ve.y);
+        atomicAdd(a8 + 2, vp.x);
+        atomicAdd(a8 + 3, vp.y);
+        atomicAdd(a8 + 4, vc.x);
+        atomicAdd(a8 + 5, vc.y);
+        atomicAdd(a8 + 6, vq.x);
+        atomicAdd(a8 + 7, vq.y);
+      } else {
+        atomic_addc(g.out + 2 * loc, ve);
+        atomic_addc(g.out + 2 * (plane + loc), vp);
+        atomic_addc(g.out + 2 * (2 * plane + loc), vc);
+        atomic_addc(g.out + 2 * (3 * plane + loc), vq);
+      }
+      if (near) atomic_addc(g.out + 2 * (4 * plane + loc), cm(wt, ret));
     }
     ++kept;
-    nearc += o.near ? 1 : 0;
-    zi += (unsigned long long)o.zcount;
-    maxres = fmax(maxres, o.residual);
-    mindet = fmin(mindet, o.rho);
-    maxz = fmax(maxz, o.zsrc);
+    nearc += near ? 1 : 0;
+    zi += (unsigned long long)zcount;
+    maxres = fmax(maxres, res);
+    mindet = fmin(mindet, rho);
+    maxz = fmax(maxz, zsrc);
+  }
+  if (g.lds_bins) {  // flush: one global add per touched bin and array
+    __syncthreads();
+    for (int e = threadIdx.x; e < g.lds_bins * g.cases; e += WNT) {
+      const double* a8 = wacc + (size_t)e * 8;
+      const int lb = e / g.cases, c = e - lb * g.cases;
+      const int tx = sx - g.offx + lb % g.pw, ty = sy - g.offy + lb / g.pw;
+      const size_t ix = (size_t)(((tx % w.nx) + w.nx) % w.nx), iy = (size_t)(((ty % w.ny) + w.ny) % w.ny);
+      const size_t loc = (size_t)c * w.nx * w.ny + ix + (size_t)w.nx * iy;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (a8[2 * q] != 0.0 || a8[2 * q + 1] != 0.0)
+          atomic_addc(g.out + 2 * (q * plane + loc), cpx<double>{a8[2 * q], a8[2 * q + 1]});
+    }
   }
   kept = wsum_u64(kept);
   nearc = wsum_u64(nearc);
@@ -1159,21 +1182,49 @@ int glw_wit_sector(glw_wit* W, int32_t sector, const double* amps, int32_t cases
   hipStream_t st = (hipStream_t)stream;
   const size_t n = 5ull * cases * W->d.nx * W->d.ny * 2;
   hipLaunchKernelGGL(k_wit_clear, dim3((n + 255) / 256), dim3(256), 0, st, out, n, W->cnt, W->dmax);
-  SignedArgs g;
+  const long long M = W->d.modes_count, P = M * (M + 1) / 2, T = M * (M + 1) * (M + 2) / 6;
+  if (!W->w.mpair) {  // signed subset tables, built once on this stream
+    const long long nm = M * M, n3 = P * M;
+    void *mp, *mt, *mx;
+    int rc = W->alloc(&mp, sizeof(Rec) * nm);
+    if (!rc) rc = W->alloc(&mt, sizeof(Rec) * n3);
+    if (!rc) rc = W->alloc(&mx, sizeof(cpx<double>) * 2 * n3);
+    if (rc) return rc;
+    WitDev w = W->w;
+    w.mpair = (Rec*)mp;
+    w.mtrip = (Rec*)mt;
+    w.mtrip_x = (cpx<double>*)mx;
+    hipLaunchKernelGGL(k_wit_mixed_pairs, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, w);
+    hipLaunchKernelGGL(k_wit_mixed_triples, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, w, n3);
+    W_HIP(hipGetLastError());
+    W->w = w;
+  }
+  SignedTabArgs g;
   g.w = W->w;
-  g.dkx = W->d.dkx;
-  g.dky = W->d.dky;
   g.amps = (const cpx<double>*)amps;
   g.cases = cases;
   g.out = out;
   g.cnt = W->cnt;
   g.dmax = W->dmax;
-  const long long M = W->d.modes_count, P = M * (M + 1) / 2, T = M * (M + 1) * (M + 2) / 6;
-  g.total = sector == 40 ? P * P : T * M;
-  const long long blocks = std::min<long long>((g.total + WNT - 1) / WNT, 1 << 20);
-  if (sector == 40) hipLaunchKernelGGL(k_wit_signed<40>, dim3((unsigned)blocks), dim3(WNT), 0, st, g);
-  else hipLaunchKernelGGL(k_wit_signed<42>, dim3((unsigned)blocks), dim3(WNT), 0, st, g);
-  hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, (double)g.total, counters);
+  // LDS window: (tx - sx, ty - sy) is minus a pair sum (40) or a mode difference (42)
+  g.pw = 2 * (W->xmax - W->xmin) + 1;
+  g.ph = 2 * (W->ymax - W->ymin) + 1;
+  g.offx = sector == 40 ? 2 * W->xmax : W->xmax - W->xmin;
+  g.offy = sector == 40 ? 2 * W->ymax : W->ymax - W->ymin;
+  const size_t lds = (size_t)g.pw * g.ph * cases * 8 * sizeof(double);
+  g.lds_bins = lds <= kWitLdsBytes ? g.pw * g.ph : 0;
+  const size_t shm = g.lds_bins ? lds : 0;
+  const double total = (double)(sector == 40 ? P * P : T * M);
+  if (sector == 40) {
+    if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_signed_tab<40>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+    hipLaunchKernelGGL(k_wit_signed_tab<40>, dim3((unsigned)P), dim3(WNT), shm, st, g);
+  } else {
+    if (shm) W_HIP(hipFuncSetAttribute((const void*)k_wit_signed_tab<42>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+    hipLaunchKernelGGL(k_wit_signed_tab<42>, dim3((unsigned)P), dim3(WNT), shm, st, g);
+  }
+  hipLaunchKernelGGL(k_wit_counters, dim3(1), dim3(64), 0, st, W->cnt, W->dmax, total, counters);
   W_HIP(hipGetLastError());
   return GLW_OK;
 }

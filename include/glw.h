@@ -207,9 +207,12 @@ int glw_wit_sector44_range(glw_wit* wit, int32_t first_begin, int32_t first_end,
 /* Any sector of the reference runner (`--sector 40|42|44`,
  * directional_eta4_openmp.cpp:603-734): 44 is glw_wit_sector44; 40 (plus
  * pair x minus pair, analytic half plane) and 42 (plus triple x minus mode)
- * have signed parents, evaluated per candidate with the retained-V3 push and
- * the y31 coordinate (:282-336) in FP64.  Same arrays and counters as above
- * (total = the sector's candidate count). */
+ * have signed parents, evaluated in FP64 with the retained-V3 push and the
+ * y31 coordinate (:282-336).  Same arrays and counters as above (total = the
+ * sector's candidate count).  The first call of sector 40 or 42 on a plan
+ * allocates and fills the signed subset tables on `stream`: M^2 pair records
+ * of 136 B and M^2 (M + 1) / 2 triple records of 168 B (675 MB at M = 200,
+ * 43 GB at M = 800); later calls reuse them. */
 int glw_wit_sector(glw_wit* wit, int32_t sector, const double* amplitudes, int32_t cases, double* out,
                    double* counters, void* stream);
 
