@@ -250,12 +250,21 @@ struct ColStore {
 #endif
 constexpr int kRowWork = GLW_MODAL_ROW_WORK;
 constexpr int kPoints = GLW_MODAL_POINTS;
-__host__ __device__ constexpr int t_rows(int T) { return T >= kRowWork ? 1 : (kRowWork / T > 4 ? 4 : kRowWork / T); }
-__host__ __device__ constexpr int t_threads(int T) {
+// The order-3 pass at T = 1024 takes one row per workgroup (256 threads) and is
+// held to 256 registers (two waves per SIMD, two workgroups per CU): its
+// fifteen transforms are barrier-bound, and a second workgroup per CU fills
+// the barrier waits (MI355X: 0.586 ms against 0.631 ms for two rows, 512
+// threads and one workgroup per CU; one row with 512 registers: 0.997 ms).
+__host__ __device__ constexpr int t_row_work(int T, int MODE) { return MODE == MODE_O3 && T == 1024 ? 1024 : kRowWork; }
+__host__ __device__ constexpr int t_rows(int T, int MODE) {
+  return T >= t_row_work(T, MODE) ? 1 : (t_row_work(T, MODE) / T > 4 ? 4 : t_row_work(T, MODE) / T);
+}
+__host__ __device__ constexpr int t_threads(int T, int MODE) {
   // past 2048 points twice the points per thread: 512 threads keep the 256-VGPR budget
-  const int n = (t_rows(T) * T / (T > 2048 ? 2 * kPoints : kPoints)) / 64 * 64;
+  const int n = (t_rows(T, MODE) * T / (T > 2048 ? 2 * kPoints : kPoints)) / 64 * 64;
   return n < 64 ? 64 : (n > 1024 ? 1024 : n);
 }
+__host__ __device__ constexpr int t_min_waves(int T, int MODE) { return MODE == MODE_O3 && T == 1024 ? 2 : 1; }
 // Radix plan of the time-pass transforms (1: greedy radix 8/4/2/3, fewer
 // registers per butterfly than the composite radix-16 plan 0).
 #ifndef GLW_MODAL_TPLAN
@@ -300,8 +309,8 @@ __device__ __forceinline__ int col_block(int b, int nblocks) {
 __host__ __device__ constexpr int fields_per_fft(int T) { return T % 3 == 0 ? 1 : GLW_MODAL_FIELDS_PER_FFT; }
 
 template <int T, int MODE>
-__global__ __launch_bounds__(t_threads(T)) void k_modal_t(Args a) {
-  constexpr int RB = t_rows(T), NT = t_threads(T);
+__global__ __launch_bounds__(t_threads(T, MODE), t_min_waves(T, MODE)) void k_modal_t(Args a) {
+  constexpr int RB = t_rows(T, MODE), NT = t_threads(T, MODE);
   constexpr int SS = seq_stride<T>();
   constexpr int PT = (RB * T + NT - 1) / NT;
   constexpr int FB = fields_per_fft(T);

@@ -15,8 +15,8 @@ namespace {
 
 template <int T, int MODE>
 int t_go(const Args& a, hipStream_t st) {
-  if (a.q_len % t_rows(T)) return 1;
-  hipLaunchKernelGGL((k_modal_t<T, MODE>), dim3(a.q_len / t_rows(T)), dim3(t_threads(T)), 0, st, a);
+  if (a.q_len % t_rows(T, MODE)) return 1;
+  hipLaunchKernelGGL((k_modal_t<T, MODE>), dim3(a.q_len / t_rows(T, MODE)), dim3(t_threads(T, MODE)), 0, st, a);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

@@ -33,6 +33,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "glw.h"
@@ -169,6 +170,57 @@ __device__ __forceinline__ void subset_sources(const Acc& rec, const RecT<T>& S,
   src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
 }
 
+// Degree-4 source by complementary pairs: the same sums as subset_sources<4>,
+// with the etas of the fourteen proper subsets read first and then one pair of
+// records (L, 15 ^ L) at a time.  Each pair contributes both orders of the
+// two-block source and the Taylor terms whose potential block is L or 15 ^ L.
+template <class Acc, typename T>
+__device__ __forceinline__ void quartet_sources(const Acc& rec, const RecT<T>& S, cpx<T>& taylor, cpx<T>& src0,
+                                                cpx<T>& src1) {
+  cpx<T> eta[16];
+#pragma unroll
+  for (int m = 1; m < 15; ++m) eta[m] = rec(m).eta;
+  // E_n(C), n = 1..3: sums over unordered partitions of C into n blocks of prod eta[block]
+  auto en = [&](int C, cpx<T>& e1, cpx<T>& e2, cpx<T>& e3) {
+    e1 = eta[C];
+    e2 = e3 = {0, 0};
+    if (popc(C) == 2) {
+      const int a = lowbit(C), b = C ^ a;
+      e2 = cm(eta[a], eta[b]);
+    } else if (popc(C) == 3) {
+      const int a = lowbit(C), b = lowbit(C ^ a), c = C ^ a ^ b;
+      e2 = ca(ca(cm(eta[a], eta[b | c]), cm(eta[b], eta[a | c])), cm(eta[c], eta[a | b]));
+      e3 = cm(cm(eta[a], eta[b]), eta[c]);
+    }
+  };
+  cpx<T> horiz = {0, 0}, kin = {0, 0}, lv = {0, 0}, lt = {0, 0};
+  taylor = {0, 0};
+  auto taylor_term = [&](const RecT<T>& p, int C) {
+    cpx<T> e1, e2, e3;
+    en(C, e1, e2, e3);
+    const cpx<T> t = ca(ca(cs(e1, zd(p, 1)), cs(e2, zd(p, 2))), cs(e3, zd(p, 3)));
+    const cpx<T> v = ca(ca(cs(e1, zd(p, 2)), cs(e2, zd(p, 3))), cs(e3, zd(p, 4)));
+    const cpx<T> pt = cm(p.phi, t);
+    taylor = ca(taylor, pt);
+    lv = ca(lv, cm(p.phi, v));
+    lt = ca(lt, ci(-p.om, pt));
+  };
+  constexpr int kLeft[7] = {1, 2, 4, 8, 3, 5, 6};
+#pragma unroll
+  for (int u = 0; u < 7; ++u) {
+    const int L = kLeft[u], R = 15 ^ L;
+    const RecT<T> a = rec(L), b = rec(R);
+    horiz = ca(horiz, cm(ca(ci(b.kx, a.sh0), ci(b.ky, a.sh1)), eta[R]));
+    horiz = ca(horiz, cm(ca(ci(a.kx, b.sh0), ci(a.ky, b.sh1)), eta[L]));
+    kin = ca(kin, ca(ca(cm(a.sh0, b.sh0), cm(a.sh1, b.sh1)), cm(a.sv, b.sv)));
+    taylor_term(a, R);
+    taylor_term(b, L);
+  }
+  src0 = {-(horiz.x - lv.x) - S.q * taylor.x, -(horiz.y - lv.y) - S.q * taylor.y};
+  const cpx<T> it = ci(S.om, taylor);
+  src1 = {-(lt.x + kin.x) - it.x, -(lt.y + kin.y) - it.y};
+}
+
 // Strict zero of a subset's wavevector (reference :356-364).
 template <typename T>
 __device__ __forceinline__ bool strict_zero(const RecT<T>& S) {
@@ -211,7 +263,10 @@ __device__ __forceinline__ void subset_state(const Acc& rec, const RecT<T>& S, c
                                              T split = T(0)) {
   const T g = T(md.g);
   cpx<T> taylor;
-  subset_sources<D>(rec, S, taylor, o.src0, o.src1);
+  // degree 4 in FP32 by complementary pairs (M=400 on MI355X: 92 ms against 118 ms);
+  // in FP64 the pair order costs registers at one wave per SIMD (298 ms against 282 ms)
+  if constexpr (D == 4 && std::is_same<T, float>::value) quartet_sources(rec, S, taylor, o.src0, o.src1);
+  else subset_sources<D>(rec, S, taylor, o.src0, o.src1);
   o.zero = strict_zero(S);
   o.near = false;
   o.rho = T(0);
@@ -741,6 +796,8 @@ struct SignedTabArgs {
 // the push of the four cubic subsets' retained V3 through the normal map
 // (:282-298), the y31 coordinate (:300-336), the near-resonance split and the
 // residual (:393-433).
+// One wave per SIMD (256 VGPRs + ~160 AGPRs, no spill): held to two waves the
+// kernel spills 600-800 B per lane and runs 1.2-1.6x slower (M=200, MI355X).
 template <int SECTOR>
 __global__ __launch_bounds__(WNT, 1) void k_wit_signed_tab(SignedTabArgs g) {
   const WitDev& w = g.w;
