@@ -250,12 +250,13 @@ struct ColStore {
 #endif
 constexpr int kRowWork = GLW_MODAL_ROW_WORK;
 constexpr int kPoints = GLW_MODAL_POINTS;
-// The order-3 pass at T = 1024 takes one row per workgroup (256 threads) and is
-// held to 256 registers (two waves per SIMD, two workgroups per CU): its
-// fifteen transforms are barrier-bound, and a second workgroup per CU fills
-// the barrier waits (MI355X: 0.586 ms against 0.631 ms for two rows, 512
-// threads and one workgroup per CU; one row with 512 registers: 0.997 ms).
-__host__ __device__ constexpr int t_row_work(int T, int MODE) { return MODE == MODE_O3 && T == 1024 ? 1024 : kRowWork; }
+// The order-2 and order-3 passes at T = 1024 take one row per workgroup (256
+// threads) held to 256 registers (two waves per SIMD, two workgroups per CU):
+// their transforms are barrier-bound, and a second workgroup per CU fills the
+// barrier waits (order 3 on MI355X: 0.586 ms against 0.631 ms for two rows,
+// 512 threads and one workgroup per CU; one row with 512 registers: 0.997 ms).
+__host__ __device__ constexpr bool t_one_row(int T, int MODE) { return (MODE == MODE_O3 || MODE == MODE_O2) && T == 1024; }
+__host__ __device__ constexpr int t_row_work(int T, int MODE) { return t_one_row(T, MODE) ? 1024 : kRowWork; }
 __host__ __device__ constexpr int t_rows(int T, int MODE) {
   return T >= t_row_work(T, MODE) ? 1 : (t_row_work(T, MODE) / T > 4 ? 4 : t_row_work(T, MODE) / T);
 }
@@ -264,7 +265,7 @@ __host__ __device__ constexpr int t_threads(int T, int MODE) {
   const int n = (t_rows(T, MODE) * T / (T > 2048 ? 2 * kPoints : kPoints)) / 64 * 64;
   return n < 64 ? 64 : (n > 1024 ? 1024 : n);
 }
-__host__ __device__ constexpr int t_min_waves(int T, int MODE) { return MODE == MODE_O3 && T == 1024 ? 2 : 1; }
+__host__ __device__ constexpr int t_min_waves(int T, int MODE) { return t_one_row(T, MODE) ? 2 : 1; }
 // Radix plan of the time-pass transforms (1: greedy radix 8/4/2/3, fewer
 // registers per butterfly than the composite radix-16 plan 0).
 #ifndef GLW_MODAL_TPLAN
@@ -308,6 +309,34 @@ __device__ __forceinline__ int col_block(int b, int nblocks) {
 #endif
 __host__ __device__ constexpr int fields_per_fft(int T) { return T % 3 == 0 ? 1 : GLW_MODAL_FIELDS_PER_FFT; }
 
+// LDS of the time pass beyond its transform buffer.  The twiddles of T
+// points are copied to LDS when the workgroup's LDS stays within 80 KB (two
+// workgroups per CU): every radix stage of the fifteen transforms reads them.
+// The order-3 pass also builds its row's three first-order base rows (q^0,
+// q^1, q^2 at every input bin, order * span + 1 <= T, so at most T / 3 + 1
+// bins) once, in dynamic LDS, where nine of its fields read them; each
+// element is two dependent global gathers (deposit, then phase table) when
+// built per field.
+template <int T, int MODE>
+__host__ __device__ constexpr int t_buf_bytes() {
+  return fields_per_fft(T) * t_rows(T, MODE) * seq_stride<T>() * 16;
+}
+template <int T, int MODE>
+__host__ __device__ constexpr bool t_twiddles_lds() {
+  return t_buf_bytes<T, MODE>() + T * 16 <= 80 * 1024;
+}
+template <int T, int MODE>
+__host__ __device__ constexpr bool t_rows_lds() {
+  return MODE == MODE_O3 && t_buf_bytes<T, MODE>() + (t_twiddles_lds<T, MODE>() ? T * 16 : 0) +
+                                    3 * t_rows(T, MODE) * (T / 3 + 1) * 16 <=
+                                96 * 1024;
+}
+// dynamic LDS bytes of a launch with nbin input bins
+template <int T, int MODE>
+__host__ __device__ constexpr int t_dyn_bytes(int nbin) {
+  return t_rows_lds<T, MODE>() ? 3 * t_rows(T, MODE) * nbin * 16 : 0;
+}
+
 template <int T, int MODE>
 __global__ __launch_bounds__(t_threads(T, MODE), t_min_waves(T, MODE)) void k_modal_t(Args a) {
   constexpr int RB = t_rows(T, MODE), NT = t_threads(T, MODE);
@@ -315,11 +344,27 @@ __global__ __launch_bounds__(t_threads(T, MODE), t_min_waves(T, MODE)) void k_mo
   constexpr int PT = (RB * T + NT - 1) / NT;
   constexpr int FB = fields_per_fft(T);
   __shared__ C buf[FB * RB * SS];
+  constexpr bool TWL = t_twiddles_lds<T, MODE>(), ROWL = t_rows_lds<T, MODE>();
+  __shared__ C twl[TWL ? T : 1];
+  extern __shared__ C rowl[];  // ROWL: [3][RB][nbin] first-order base rows q^0, q^1, q^2
   const int j0 = row_block(blockIdx.x, gridDim.x, RB) * RB;
   const int nbin = a.span + 1;
   const int nw2 = 2 * a.span + 1;
   const size_t fs2 = (size_t)nw2 * a.q_len;
   const C* tw = a.tw_t;
+  if constexpr (TWL) {  // visible after the first barrier of the field loop
+    for (int i = threadIdx.x; i < T; i += NT) twl[i] = a.tw_t[i];
+    tw = twl;
+  }
+  if constexpr (ROWL) {
+    for (int e = threadIdx.x; e < RB * nbin; e += NT) {
+      const int r = e % RB, w = e / RB;
+      const Powers b = first_values(a.dep, a.tw_q, a.k0, a.q_len, a.dq, j0 + r, w);
+      rowl[(0 * RB + r) * nbin + w] = b.p0;
+      rowl[(1 * RB + r) * nbin + w] = b.p1;
+      rowl[(2 * RB + r) * nbin + w] = b.p2;
+    }
+  }
   C f[PT], e[PT], x[PT], h[PT], acc_d[PT], acc_k[PT];
 #pragma unroll
   for (int i = 0; i < PT; ++i) acc_d[i] = acc_k[i] = x[i] = h[i] = e[i] = {0.0, 0.0};
@@ -354,16 +399,21 @@ __global__ __launch_bounds__(t_threads(T, MODE), t_min_waves(T, MODE)) void k_mo
       C v = {0.0, 0.0};
       if (!live) {
       } else if (!rc.pair) {
-        Powers b;
-        if constexpr (kCache) {
-          b = {b0[k], b1[k], b2[k]};
+        C u = {0.0, 0.0};
+        if constexpr (ROWL) {
+          if (w < nbin) u = rowl[(rc.idx * RB + r) * nbin + w];
         } else {
-          b = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-          if (w < nbin) b = first_values(a.dep, a.tw_q, a.k0, a.q_len, a.dq, j0 + r, w);
+          Powers b;
+          if constexpr (kCache) {
+            b = {b0[k], b1[k], b2[k]};
+          } else {
+            b = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+            if (w < nbin) b = first_values(a.dep, a.tw_q, a.k0, a.q_len, a.dq, j0 + r, w);
+          }
+          // arithmetic select: a branch or ?: on the runtime power is lowered to an indexed private array
+          const double s0 = rc.idx == 0, s1 = rc.idx == 1, s2 = rc.idx == 2;
+          u = {s0 * b.p0.x + s1 * b.p1.x + s2 * b.p2.x, s0 * b.p0.y + s1 * b.p1.y + s2 * b.p2.y};
         }
-        // arithmetic select: a branch or ?: on the runtime power is lowered to an indexed private array
-        const double s0 = rc.idx == 0, s1 = rc.idx == 1, s2 = rc.idx == 2;
-        const C u = {s0 * b.p0.x + s1 * b.p1.x + s2 * b.p2.x, s0 * b.p0.y + s1 * b.p1.y + s2 * b.p2.y};
         v = wmul(rc.wm, u, (double)(a.base + w) * a.fstep);
       } else if (w < nw2) {
         v = wmul(rc.wm, a.pair[rc.idx * fs2 + (size_t)w * a.q_len + j0 + r], (double)(2 * a.base + w) * a.fstep);

@@ -16,7 +16,11 @@ namespace {
 template <int T, int MODE>
 int t_go(const Args& a, hipStream_t st) {
   if (a.q_len % t_rows(T, MODE)) return 1;
-  hipLaunchKernelGGL((k_modal_t<T, MODE>), dim3(a.q_len / t_rows(T, MODE)), dim3(t_threads(T, MODE)), 0, st, a);
+  const int dyn = t_dyn_bytes<T, MODE>(a.span + 1);
+  if (dyn && hipFuncSetAttribute((const void*)k_modal_t<T, MODE>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn) !=
+                 hipSuccess)
+    return 2;
+  hipLaunchKernelGGL((k_modal_t<T, MODE>), dim3(a.q_len / t_rows(T, MODE)), dim3(t_threads(T, MODE)), dyn, st, a);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
