@@ -341,6 +341,9 @@ static int modal_run(const glw_modal_desc* d, const double* deposit, void* works
   a.src = (C*)(ws + L.src);
   a.pair = d->order == 3 ? (C*)(ws + L.pair) : nullptr;
   a.forcing = d->order == 3 ? (C*)(ws + L.forcing) : nullptr;
+  // e2 and p2 spectra ([2][2 span + 1][q_len]) live in the order-3 forcing
+  // array, which the order-3 time pass only writes after they are consumed
+  a.pair_scratch = a.forcing;
   a.rank = d->rank;
   a.lam2 = d->lambda2;
   a.lam_in = d->lambda_inner;
@@ -365,6 +368,8 @@ static int modal_run(const glw_modal_desc* d, const double* deposit, void* works
   if (r0 > 1)
     if (int rc = split_pre(a.src, d->q_len, glw::modal::out_ld(nw2), nw2, a.tw_q, st)) return rc;
   if (int rc = launched(glw::modal::launch_q_order2(a, nw2, r0, st), "modal pass (eta22)")) return rc;
+  if (d->order == 3)
+    if (int rc = launched(glw::modal::launch_q_pairs(a, nw2, r0, st), "modal pass (pair fields)")) return rc;
   if (r0 > 1) {
     if (int rc = split_sum(a, nw2, st)) return rc;
     if (d->order == 3)

@@ -81,6 +81,7 @@ struct Args {
   double* partial;       // [ncol][R0][4] per-sub-transform sums (R0 > 1)
   C* src;                // [2][Q][ldw2]  order-2 sources (d, k); eta20: forcing (d, k) at signed columns
   C* pair;               // [5][2 span + 1][Q]  pair fields in x
+  C* pair_scratch;       // [2][2 span + 1][Q]  e2, p2 spectra between the eta22 pass and k_modal_pairs
   C* forcing;            // [2][Q][ldw3]  order-3 forcing (d, k)
   C* coef;               // [nw] output coefficient per kept frequency of the modal pass
   double* energy;        // [nw][2] source energy (total, invalid) per kept frequency
@@ -273,6 +274,11 @@ __host__ __device__ constexpr int t_min_waves(int T, int MODE) { return t_one_ro
 #endif
 constexpr int kTPlan = GLW_MODAL_TPLAN;
 __host__ __device__ constexpr int q_threads(int Q) { return Q <= 512 ? 64 : (Q <= 2048 ? 256 : 512); }
+// The modal passes at Q = 4096 are held to 128 registers (four waves per SIMD,
+// two workgroups per CU; eta33 on MI355X: 0.186 ms against 0.213 ms at 168
+// registers and one workgroup per CU).
+template <int Q, int MODE>
+__host__ __device__ constexpr int q_min_waves() { return Q == 4096 ? 4 : 1; }
 
 // Row block of workgroup b.  A 128-byte line of a [w][Q] array holds 8
 // consecutive x rows; the G = 8 / RB workgroups that share those lines are
@@ -607,7 +613,7 @@ __device__ __forceinline__ void block_sum4(double (&v)[4], double* red) {
 // m = R0 m' + r), writes its sums to a.partial, and leaves the inverse
 // transforms of its pair spectra for k_modal_post to combine.
 template <int Q, int MODE>
-__global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
+__global__ __launch_bounds__(q_threads(Q), (q_min_waves<Q, MODE>())) void k_modal_q(Args a) {
   constexpr int NT = q_threads(Q);
   constexpr int SS = seq_stride<Q>();
   constexpr int PQ = (Q + NT - 1) / NT;
@@ -721,34 +727,45 @@ __global__ __launch_bounds__(q_threads(Q)) void k_modal_q(Args a) {
   }
   if constexpr (MODE == MODE_O2) {
     if (!pairs) return;
-    // pair fields in x (modal_time.py:285-292, _spectral_field = fft_t(ifft_q(.) P)):
-    // e2, e2 q, p2 q, p2 A, p2 q^2, each inverse-transformed over the modal axis
-    const size_t fs = (size_t)nw * a.q_len;
-    C p2[PQ];
+    // e2 and p2 spectra of this column to the pair scratch [2][ncol][q_len]
+    // (sub-transform rs at rs Q); k_modal_pairs transforms them
+    C* e2 = a.pair_scratch + (size_t)w * a.q_len + (size_t)rs * Q;
+    C* p2 = e2 + (size_t)gridDim.x * a.q_len;
 #pragma unroll
     for (int i = 0; i < PQ; ++i) {
       const int ml = threadIdx.x + i * NT;
-      p2[i] = ml < Q ? buf[lds_pad(ml)] : C{0.0, 0.0};  // own elements (block_sum4 synchronised)
-    }
-#pragma unroll 1
-    for (int f = 0; f < kPairFields; ++f) {
-      __syncthreads();
-#pragma unroll
-      for (int i = 0; i < PQ; ++i) {
-        const int ml = threadIdx.x + i * NT;
-        if (ml >= Q) continue;
-        const double q = (double)(2 * a.k0 + ml * r0 + rs) * a.dq;
-        double mul = 1.0;
-        if (f == 1 || f == 2) mul = q;
-        else if (f == 3) mul = q * tanh(q);
-        else if (f == 4) mul = q * q;
-        buf[lds_pad(ml)] = cmulr(f == 0 || f == 1 ? fd[i] : p2[i], mul);
+      if (ml < Q) {
+        e2[ml] = fd[i];
+        p2[ml] = buf[lds_pad(ml)];  // own elements (block_sum4 synchronised)
       }
-      __syncthreads();
-      ColStore st{a.pair + f * fs + (size_t)w * a.q_len + (size_t)rs * Q};
-      fft_io<double, Q, 1, SS, NT, +1, Q >= 3072 ? 1 : 0>(buf, a.tw_qs, none, st);
     }
   }
+}
+
+// Pair fields in x (modal_time.py:285-292, _spectral_field = fft_t(ifft_q(.) P)):
+// e2, e2 q, p2 q, p2 A, p2 q^2, each inverse-transformed over the modal axis,
+// one workgroup per (column, sub-transform, field): the 5 ncol transforms run
+// in parallel instead of five in a row per column.
+template <int Q>
+__global__ __launch_bounds__(q_threads(Q)) void k_modal_pairs(Args a) {
+  constexpr int NT = q_threads(Q);
+  constexpr int SS = seq_stride<Q>();
+  __shared__ C buf[SS];
+  const int w = blockIdx.x, rs = blockIdx.y, f = blockIdx.z, r0 = gridDim.y;
+  const C* src = a.pair_scratch + ((size_t)(f < 2 ? 0 : 1) * gridDim.x + w) * a.q_len + (size_t)rs * Q;
+  for (int ml = threadIdx.x; ml < Q; ml += NT) {
+    const double q = (double)(2 * a.k0 + ml * r0 + rs) * a.dq;
+    double mul = 1.0;
+    if (f == 1 || f == 2) mul = q;
+    else if (f == 3) mul = q * tanh(q);
+    else if (f == 4) mul = q * q;
+    buf[lds_pad(ml)] = cmulr(src[ml], mul);
+  }
+  __syncthreads();
+  const size_t fs = (size_t)gridDim.x * a.q_len;
+  ColStore st{a.pair + f * fs + (size_t)w * a.q_len + (size_t)rs * Q};
+  NoIO none;
+  fft_io<double, Q, 1, SS, NT, +1, Q >= 3072 ? 1 : 0>(buf, a.tw_qs, none, st);
 }
 
 // Split modal transform, forward half (R0 sub-transforms of length QS = Q / R0):

@@ -31,6 +31,13 @@ int q_go(const Args& a, int ncol, int r0, hipStream_t st) {
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
+template <int Q>
+int p_go(const Args& a, int ncol, int r0, hipStream_t st) {
+  if (a.q_len != Q * r0) return 1;
+  hipLaunchKernelGGL((k_modal_pairs<Q>), dim3(ncol, r0, kPairFields), dim3(q_threads(Q)), 0, st, a);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
 }  // namespace
 
 #define GLW_T_CASE(L) \
@@ -59,6 +66,18 @@ static int q_switch(const Args& a, int ncol, int r0, hipStream_t st) {
   }
 }
 #endif
+#if GLW_MODAL_SET == 2
+#define GLW_P_CASE(L) \
+  case L:             \
+    return p_go<L>(a, ncol, r0, st);
+static int p_switch(const Args& a, int ncol, int r0, hipStream_t st) {
+  if (r0 < 1 || a.q_len % r0) return 1;
+  switch (a.q_len / r0) {
+    GLW_MODAL_Q_LENGTHS(GLW_P_CASE)
+    default: return 1;
+  }
+}
+#endif
 
 #if GLW_MODAL_SET == 0
 int launch_t_order2(const Args& a, hipStream_t st) { return t_switch<MODE_O2>(a, st); }
@@ -67,6 +86,7 @@ int launch_t_eta20(const Args& a, hipStream_t st) { return t_switch<MODE_E20>(a,
 int launch_t_order3(const Args& a, hipStream_t st) { return t_switch<MODE_O3>(a, st); }
 #elif GLW_MODAL_SET == 2
 int launch_q_order2(const Args& a, int ncol, int r0, hipStream_t st) { return q_switch<MODE_O2>(a, ncol, r0, st); }
+int launch_q_pairs(const Args& a, int ncol, int r0, hipStream_t st) { return p_switch(a, ncol, r0, st); }
 #elif GLW_MODAL_SET == 3
 int launch_q_order3(const Args& a, int ncol, int r0, hipStream_t st) { return q_switch<MODE_O3>(a, ncol, r0, st); }
 int launch_q_eta20(const Args& a, int ncol, hipStream_t st) { return q_switch<MODE_E20>(a, ncol, 1, st); }

@@ -24,6 +24,8 @@ int launch_t_eta20(const Args& a, hipStream_t st);
 // r0 > 1: the split transform (a.q_len = r0 x an LDS length), one workgroup
 // per column and sub-transform.
 int launch_q_order2(const Args& a, int ncol, int r0, hipStream_t st);
+// order 3 after launch_q_order2: the five pair fields from a.pair_scratch
+int launch_q_pairs(const Args& a, int ncol, int r0, hipStream_t st);
 int launch_q_order3(const Args& a, int ncol, int r0, hipStream_t st);
 int launch_q_eta20(const Args& a, int ncol, hipStream_t st);
 
