@@ -26,7 +26,15 @@ constexpr int kMaxFields = 48;
 constexpr int NT = 256;
 // FFT stages read twiddles from the global table (a few KB, L1/L2-resident)
 // rather than an LDS copy: the LDS goes to data tiles (more workgroups per CU).
-constexpr bool kTwiddlesInLds = false;
+// Measured on MI355X (GL order 3, 512^2, 64 instants): LDS copies in the
+// column kernels 4.856 ms per step, in the row kernels 5.181 ms, neither 4.818 ms.
+#ifndef GLW_TW_LDS_COL
+#define GLW_TW_LDS_COL 0
+#endif
+#ifndef GLW_TW_LDS_ROW
+#define GLW_TW_LDS_ROW 0
+#endif
+constexpr bool kTwiddlesInLdsCol = GLW_TW_LDS_COL, kTwiddlesInLdsRow = GLW_TW_LDS_ROW;
 
 // Spectral multiplier codes (functions of qx, qy, nu, 1/nu, q^2, A on a bin).
 enum MulCode : int {
@@ -190,9 +198,9 @@ struct ColInvStore {
 template <typename T, int GY, int CW, int PLAN = 0, int NTH = NT>
 __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
-  __shared__ cpx<T> twl[kTwiddlesInLds ? GY : 1];
+  __shared__ cpx<T> twl[kTwiddlesInLdsCol ? GY : 1];
   const cpx<T>* twp = tw;
-  if constexpr (kTwiddlesInLds) {  // read from the second stage on, after a barrier
+  if constexpr (kTwiddlesInLdsCol) {  // read from the second stage on, after a barrier
     stage_twiddles<T, GY, NTH>(twl, tw);
     twp = twl;
   }
@@ -289,9 +297,9 @@ struct ColFwdStore {
 template <typename T, int GY, int CW, int PLAN = 0, int NTH = NT>
 __global__ __launch_bounds__(NTH) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
   __shared__ cpx<T> buf[CW * seq_stride<GY>()];
-  __shared__ cpx<T> twl[kTwiddlesInLds ? GY : 1];
+  __shared__ cpx<T> twl[kTwiddlesInLdsCol ? GY : 1];
   const cpx<T>* twp = tw;
-  if constexpr (kTwiddlesInLds) {
+  if constexpr (kTwiddlesInLdsCol) {
     stage_twiddles<T, GY, NTH>(twl, tw);
     twp = twl;
   }
@@ -366,10 +374,10 @@ template <typename T, int GX, int NIN, int NOUT, int RY, class Op, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[NIN * RY * SS];
-  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  __shared__ cpx<T> twl[kTwiddlesInLdsRow ? GX : 1];
   __shared__ int fmx0[NIN], fwx[NIN];
   const cpx<T>* twp = tw;
-  if constexpr (kTwiddlesInLds) {
+  if constexpr (kTwiddlesInLdsRow) {
     stage_twiddles<T, GX, NT>(twl, tw);
     twp = twl;
   }
@@ -465,9 +473,9 @@ template <typename T, int GX, int RY, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_rowinv(RowArgs a, cpx<T>* __restrict__ phys, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[RY * SS];
-  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  __shared__ cpx<T> twl[kTwiddlesInLdsRow ? GX : 1];
   const cpx<T>* twp = tw;
-  if constexpr (kTwiddlesInLds) {
+  if constexpr (kTwiddlesInLdsRow) {
     stage_twiddles<T, GX, NT>(twl, tw);
     twp = twl;
   }
@@ -505,9 +513,9 @@ template <typename T, int GX, int RY, int PLAN = 0>
 __global__ __launch_bounds__(NT) void k_rowfwd(RowArgs a, const cpx<T>* __restrict__ phys, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[RY * SS];
-  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  __shared__ cpx<T> twl[kTwiddlesInLdsRow ? GX : 1];
   const cpx<T>* twp = tw;
-  if constexpr (kTwiddlesInLds) {
+  if constexpr (kTwiddlesInLdsRow) {
     stage_twiddles<T, GX, NT>(twl, tw);
     twp = twl;
   }
@@ -557,9 +565,9 @@ template <typename T, int GX, int RY>
 __global__ __launch_bounds__(NT) void k_synth(SynthArgs a, const cpx<T>* __restrict__ tw) {
   constexpr int SS = seq_stride<GX>();
   __shared__ cpx<T> buf[RY * SS];
-  __shared__ cpx<T> twl[kTwiddlesInLds ? GX : 1];
+  __shared__ cpx<T> twl[kTwiddlesInLdsRow ? GX : 1];
   const cpx<T>* twp = tw;
-  if constexpr (kTwiddlesInLds) {
+  if constexpr (kTwiddlesInLdsRow) {
     stage_twiddles<T, GX, NT>(twl, tw);
     twp = twl;
   }

@@ -41,7 +41,7 @@ constexpr int row_ry() {
 
 template <typename T, int NIN, int L, int RY>
 constexpr bool row_fits() {
-  return (size_t)(NIN * RY * seq_stride<L>() + (kTwiddlesInLds ? L : 1)) * sizeof(cpx<T>) <= (size_t)kLdsMax;
+  return (size_t)(NIN * RY * seq_stride<L>() + (kTwiddlesInLdsRow ? L : 1)) * sizeof(cpx<T>) <= (size_t)kLdsMax;
 }
 
 // Rows per workgroup of the split-path transforms.
