@@ -20,7 +20,7 @@ values each): an equivalent rate, not a roofline fraction.  The measured HBM
 traffic of the dominant stage (rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/) is
 reported beside it when the stored schedule matches.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload order3|api|eta44|wit|batch|inverse|modal]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload order3|order2|api|eta44|wit|batch|inverse|modal]
 """
 
 from __future__ import annotations
@@ -163,6 +163,83 @@ def cpu_baseline(s, c, kxg, kyg, times_sample, gpu_fields=None):
             errs["eta33"] = max(errs["eta33"], float(np.linalg.norm(ge - s.depth * e) / np.linalg.norm(s.depth * e)))
             errs["psi33"] = max(errs["psi33"], float(np.linalg.norm(gp - pot * p) / np.linalg.norm(pot * p)))
     return len(times_sample) / elapsed, errs
+
+
+def bench_order2(args):
+    """Plain order 2 at 128^2 (eta22 GL rank 6 + dual-branch psi22), FP64: (eta22, psi22) field pairs per
+    second over 64 instants per step, the CPU restatement timed beside it on a bounded sample."""
+    import torch
+
+    from green_laplace_wave_reconstruction_amd import api, plan as PL, seastate
+    from green_laplace_wave_reconstruction_amd._native import DevicePlan
+
+    s = seastate.order2_plain_128()
+    c = api.gl_spectral_coefficients(2, s.gravity, s.depth, s.a, s.b, s.kx, s.ky, 0, 0)
+    kxg, kyg = api._wavenumber_grid(s.lx, s.ly, s.nx, s.ny)
+    spec, omega = api.deposit_analytic_input(c, s.lx, s.ly, s.nx, s.ny)
+    host = PL.build_plan(spec, s.depth * kxg, s.depth * kyg, depth=s.depth, gravity=s.gravity, order=2,
+                         peak_q=c["peak_depth"])
+    host.omega = host.box1.take(omega)
+    nt = INSTANTS
+    times = np.linspace(0.0, 10.0 * 10.0, nt)  # 64 instants over ten peak periods (Tp = 10 s)
+    dp = DevicePlan(host, host.box1.take(spec), max_instants=nt)
+    schedule = dp.autotune("order2", times, reps=3) if not args.no_tune else "default (plan 0, tile 0)"
+    out = None
+    for _ in range(args.warmup):
+        out = dp.run("order2", times, out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = dp.run("order2", times, out)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    dp.profile(True)
+    dp.stage_times()
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for _ in range(args.steps):
+        dp.run("order2", times, out)
+    torch.cuda.synchronize()
+    wall_profiled = time.perf_counter() - tp
+    stages = dp.stage_times()
+    dp.profile(False)
+    roof, pipe = stage_roofline(stages, args.steps, wall_profiled / args.steps * 1e3)
+    cpu = None
+    if not args.no_cpu:
+        from oracle import gl_oracle as O
+
+        sample = list(range(nt))  # every instant: about 5 s of CPU work
+        k = np.hypot(s.kx, s.ky)
+        om = np.sqrt(s.gravity * k * np.tanh(s.depth * k))
+        qx, qy = s.depth * kxg, s.depth * kyg
+        pot = s.depth * math.sqrt(s.gravity * s.depth)
+        eta, psi = out[0].cpu().numpy(), out[1].cpu().numpy()
+        elapsed, errs = 0.0, {"eta22": 0.0, "psi22": 0.0}
+        for i in sample:
+            sp = O.deposit(s.a, s.b, s.kx, s.ky, om, s.depth, s.lx, s.ly, s.nx, s.ny, float(times[i]))
+            t1 = time.perf_counter()
+            e22, _ = O.eta22(sp, qx, qy, 6, c["peak_depth"])
+            p22, _ = O.psi22(sp, qx, qy, False, c["peak_depth"])
+            elapsed += time.perf_counter() - t1
+            errs["eta22"] = max(errs["eta22"], rel_l2(eta[i], s.depth * e22))
+            errs["psi22"] = max(errs["psi22"], rel_l2(psi[i], pot * p22))
+        cpu = {"value": len(sample) / elapsed, "unit": "field pairs/s", "cores": 1, "kind": "port",
+               "sample": f"all {nt} instants of the step: FP64 NumPy oracle restatement of _green_laplace_eta22 "
+                         "(rank 6) and _green_laplace_psi22, numpy.fft single-threaded",
+               "gpu_rel_l2_vs_oracle": errs, **host_info()}
+    return {
+        "metric": "eta22+psi22 field pairs/s at 128^2 (plain order 2, FP64)", "value": nt * args.steps / wall,
+        "unit": "field pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic seeded JONSWAP sea state (numpy default_rng seed 1)",
+        "config": {"workload": f"GL order 2 (eta22 GL6 + psi22), 128x128, L=1000 m, h=50 m, Hs=4 m, Tp=10 s, "
+                               f"gamma=3.3, s=10, {s.count} parents, {nt} instants over [0, 100 s]",
+                   "instants_per_step": nt, "fft_schedule": schedule,
+                   "working_grids": {"quadratic": list(host.g2[::-1]) if host.g2 else None}},
+        "roofline": roof,
+        "pipeline": pipe,
+        "cpu_baseline": cpu,
+    }
 
 
 def bench_eta44(args):
@@ -717,7 +794,7 @@ def _wit_roofline(precision: str, quartets_per_s: float, key=None, kernel="k_wit
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="order3",
-                    choices=("order3", "api", "eta44", "wit", "batch", "inverse", "modal"))
+                    choices=("order3", "order2", "api", "eta44", "wit", "batch", "inverse", "modal"))
     ap.add_argument("--states", type=int, default=256, help="batch workload: sea states per step")
     ap.add_argument("--branches", type=int, default=4, help="batch workload: parallel graph branches")
     ap.add_argument("--rank", type=int, default=6, help="eta44 GL rank")
@@ -790,6 +867,9 @@ def main():
         sys.exit(1)
     if args.workload == "eta44":
         print(json.dumps(bench_eta44(args)))
+        return
+    if args.workload == "order2":
+        print(json.dumps(bench_order2(args)))
         return
     if args.workload == "wit":
         print(json.dumps(bench_wit(args) if args.sector == 44 else bench_wit_signed(args)))
