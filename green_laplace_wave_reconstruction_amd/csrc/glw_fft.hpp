@@ -288,24 +288,61 @@ struct NoIO {
   static constexpr bool SEQ_FAST = false;
 };
 
+// Compile-time shape of stage S: radix, butterflies per sequence and per
+// thread, twiddle step, which ends touch global memory and the lane mapping.
+template <int L, int NSEQ, int NT, int S, int NS, int PLAN, class LD, class ST>
+struct StageShape {
+  static constexpr int R = radix_plan<PLAN>(L, S);
+  static constexpr int NB = R > 0 ? L / R : 1;    // butterflies per sequence
+  static constexpr int TOTAL = NSEQ * NB;         // butterflies in the workgroup
+  static constexpr int PER = (TOTAL + NT - 1) / NT;
+  static constexpr int TWS = R > 0 ? L / (NS * R) : 1;  // twiddle index step
+  static constexpr bool GLOAD = S == 0 && !std::is_same<LD, NoIO>::value;
+  static constexpr bool GSTORE = R > 0 && radix_plan<PLAN>(L, S + 1) == 0 && !std::is_same<ST, NoIO>::value;
+  // Butterfly b -> (sequence, index).  With 8+ sequences in LDS the sequence
+  // runs fastest across lanes: neighbouring lanes then touch the same element
+  // of sequences an odd stride apart (no LDS bank conflicts) and read the
+  // same twiddle (broadcast).  A global end follows its functor's layout.
+  static constexpr bool SEQ_FAST = GLOAD ? LD::SEQ_FAST : (GSTORE ? ST::SEQ_FAST : NSEQ >= 8);
+  static constexpr int NTW = (R > 1 && NS > 1) ? R - 1 : 0;  // twiddles per butterfly
+  static constexpr int NTW1 = NTW > 0 ? NTW : 1;
+};
+
+// Twiddles of stage SH for every butterfly this thread owns (unconjugated).
+template <class SH, typename T, int NT, int NSEQ, int NS>
+__device__ __forceinline__ void stage_twiddle_regs(const cpx<T>* tw, cpx<T> (&w)[SH::PER][SH::NTW1]) {
+  if constexpr (SH::NTW > 0) {
+#pragma unroll
+    for (int p = 0; p < SH::PER; ++p) {
+      const int b = threadIdx.x + p * NT;
+      if (SH::TOTAL % NT == 0 || b < SH::TOTAL) {
+        const int j = SH::SEQ_FAST ? b / NSEQ : b % SH::NB;
+        const int k = j % NS;
+#pragma unroll
+        for (int r = 1; r < SH::R; ++r) w[p][r - 1] = tw[r * k * SH::TWS];
+      }
+    }
+  }
+}
+
 // One Stockham stage (radix R, NS = product of earlier radices), then the
-// remaining stages by compile-time recursion.
-template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS, int PLAN, class LD, class ST>
-__device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st) {
-  constexpr int R = radix_plan<PLAN>(L, S);
+// remaining stages by compile-time recursion.  PREFETCH: the twiddles of stage
+// S arrive in `wcur` (loaded during stage S - 1), and this stage issues the
+// global loads of stage S + 1's twiddles before its own work, so they are in
+// flight across its LDS barriers instead of exposed between a stage's LDS
+// reads and its twiddle multiply.  Only for twiddle tables in global memory
+// (an LDS table may not be staged yet when the next stage's loads issue).
+template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS, int PLAN, bool PREFETCH, bool FINAL,
+          class LD, class ST, class WCUR>
+__device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st, const WCUR& wcur) {
+  using SH = StageShape<L, NSEQ, NT, S, NS, PLAN, LD, ST>;
+  constexpr int R = SH::R;
   if constexpr (R > 0) {
-    constexpr int NB = L / R;           // butterflies per sequence
-    constexpr int TOTAL = NSEQ * NB;    // butterflies in the workgroup
-    constexpr int PER = (TOTAL + NT - 1) / NT;
-    constexpr int TWS = L / (NS * R);   // twiddle index step
-    constexpr bool GLOAD = S == 0 && !std::is_same<LD, NoIO>::value;
-    constexpr bool GSTORE = radix_plan<PLAN>(L, S + 1) == 0 && !std::is_same<ST, NoIO>::value;
-    // Butterfly b -> (sequence, index).  With 8+ sequences in LDS the sequence
-    // runs fastest across lanes: neighbouring lanes then touch the same element
-    // of sequences an odd stride apart (no LDS bank conflicts) and read the
-    // same twiddle (broadcast).  A global end follows its functor's layout.
-    constexpr bool SEQ_FAST = GLOAD ? LD::SEQ_FAST : (GSTORE ? ST::SEQ_FAST : NSEQ >= 8);
+    constexpr int NB = SH::NB, TOTAL = SH::TOTAL, PER = SH::PER, TWS = SH::TWS;
+    constexpr bool GLOAD = SH::GLOAD, GSTORE = SH::GSTORE, SEQ_FAST = SH::SEQ_FAST;
+    using SN = StageShape<L, NSEQ, NT, S + 1, NS * R, PLAN, LD, ST>;
     cpx<T> v[PER][R];
+    cpx<T> wnext[SN::PER][SN::NTW1];
     const int tid = threadIdx.x;
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
@@ -321,18 +358,30 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const 
 #pragma unroll
           for (int r = 0; r < R; ++r) v[p][r] = src[lds_pad(j + r * NB)];
         }
-        if constexpr (NS > 1) {
+      }
+    }
+    if constexpr (PREFETCH && SN::R > 0) stage_twiddle_regs<SN, T, NT, NSEQ, NS * R>(tw, wnext);
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int p = 0; p < PER; ++p) {
+        const int b = tid + p * NT;
+        if (TOTAL % NT == 0 || b < TOTAL) {
+          const int j = SEQ_FAST ? b / NSEQ : b % NB;
           const int k = j % NS;
 #pragma unroll
           for (int r = 1; r < R; ++r) {
-            cpx<T> w = tw[r * k * TWS];
+            cpx<T> w;
+            if constexpr (PREFETCH) w = wcur[p][r - 1];
+            else w = tw[r * k * TWS];
             if constexpr (SIGN > 0) w.y = -w.y;
             v[p][r] = cmul(v[p][r], w);
           }
         }
       }
     }
-    if constexpr (!GLOAD) __syncthreads();  // in place: every read before any write
+    // in place: every read before any write (a last stage that stores to global
+    // memory writes no LDS; FINAL says the caller does not reuse the buffer)
+    if constexpr (!GLOAD && !(FINAL && GSTORE)) __syncthreads();
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
       const int b = tid + p * NT;
@@ -353,7 +402,7 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const 
       }
     }
     if constexpr (!GSTORE) __syncthreads();
-    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R, PLAN>(buf, tw, ld, st);
+    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R, PLAN, PREFETCH, FINAL>(buf, tw, ld, st, wnext);
   }
 }
 
@@ -370,17 +419,27 @@ template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN = 0>
 __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* tw) {
   static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
   NoIO none;
-  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN>(buf, tw, none, none);
+  const cpx<T> w0[1][1] = {};
+  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN, false, false>(buf, tw, none, none, w0);
 }
 
 // Same transform with a global loader for the first stage and/or a storer
 // for the last (NoIO for an LDS end).  With a loader the caller stages
 // nothing (the first stage fills the LDS buffer); with a storer the results
-// are not left in LDS and the call ends without a barrier.
+// are not left in LDS and the call ends without a barrier.  PREFETCH (global
+// twiddle tables only): each stage's twiddles load one stage ahead.  FINAL
+// (with a storer): the caller never touches `buf` again, so the last stage
+// skips its read-before-write barrier.
+template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN, bool PREFETCH, bool FINAL = false, class LD,
+          class ST>
+__device__ __forceinline__ void fft_io_pf(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st) {
+  static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
+  const cpx<T> w0[1][1] = {};  // stage 0 has no twiddles (NS = 1)
+  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN, PREFETCH, FINAL>(buf, tw, ld, st, w0);
+}
 template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN, class LD, class ST>
 __device__ __forceinline__ void fft_io(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st) {
-  static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
-  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN>(buf, tw, ld, st);
+  fft_io_pf<T, L, NSEQ, SSTR, NT, SIGN, PLAN, false>(buf, tw, ld, st);
 }
 
 }  // namespace glw

@@ -35,6 +35,23 @@ constexpr int NT = 256;
 #define GLW_TW_LDS_ROW 0
 #endif
 constexpr bool kTwiddlesInLdsCol = GLW_TW_LDS_COL, kTwiddlesInLdsRow = GLW_TW_LDS_ROW;
+// Twiddle prefetch one stage ahead (global tables only) per kernel class, and
+// whether the last transform of a kernel skips its final read-before-write
+// barrier (the kernel ends there).
+#ifndef GLW_PF_COL
+#define GLW_PF_COL 1
+#endif
+#ifndef GLW_PF_ROW
+#define GLW_PF_ROW 1
+#endif
+#ifndef GLW_PF_SYNTH
+#define GLW_PF_SYNTH 0
+#endif
+#ifndef GLW_FINAL_NOBAR
+#define GLW_FINAL_NOBAR 1
+#endif
+constexpr bool kPfCol = GLW_PF_COL && !kTwiddlesInLdsCol, kPfRow = GLW_PF_ROW && !kTwiddlesInLdsRow;
+constexpr bool kPfSynth = GLW_PF_SYNTH && !kTwiddlesInLdsRow, kFinal = GLW_FINAL_NOBAR;
 
 // Spectral multiplier codes (functions of qx, qy, nu, 1/nu, q^2, A on a bin).
 enum MulCode : int {
@@ -229,7 +246,7 @@ __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __re
   st.ostride = a.ostride;
   st.col0 = ld.col0;
   st.wx = a.wx;
-  fft_io<T, GY, CW, seq_stride<GY>(), NTH, +1, PLAN>(buf, twp, ld, st);
+  fft_io_pf<T, GY, CW, seq_stride<GY>(), NTH, +1, PLAN, kPfCol, kFinal>(buf, twp, ld, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -321,7 +338,7 @@ __global__ __launch_bounds__(NTH) void k_colfwd(ColFwdArgs a, const cpx<T>* __re
   st.owy = a.owy;
   st.omy0 = a.omy0;
   st.gy = GY;
-  fft_io<T, GY, CW, seq_stride<GY>(), NTH, -1, PLAN>(buf, twp, ld, st);
+  fft_io_pf<T, GY, CW, seq_stride<GY>(), NTH, -1, PLAN, kPfCol, kFinal>(buf, twp, ld, st);
   if (a.fmax) block_max_atomic<NTH>(st.vmax, a.fmax + (size_t)task * a.nsrc + s);
 }
 
@@ -398,7 +415,7 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
   ld.y0 = y0;
   ld.istride = a.istride;
   NoIO none;
-  fft_io<T, GX, NIN * RY, SS, NT, +1, PLAN>(buf, twp, ld, none);
+  fft_io_pf<T, GX, NIN * RY, SS, NT, +1, PLAN, kPfRow>(buf, twp, ld, none);
   for (int i = threadIdx.x; i < RY * GX; i += NT) {
     const int r = i / GX;
     const int x = i - r * GX;
@@ -418,7 +435,7 @@ __global__ __launch_bounds__(NT) void k_row(RowArgs a, const cpx<T>* __restrict_
   st.ostride = a.ostride;
   st.omx0 = a.omx0;
   st.owx = a.owx;
-  fft_io<T, GX, NOUT * RY, SS, NT, -1, PLAN>(buf, twp, none, st);
+  fft_io_pf<T, GX, NOUT * RY, SS, NT, -1, PLAN, kPfRow, kFinal>(buf, twp, none, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -489,7 +506,7 @@ __global__ __launch_bounds__(NT) void k_rowinv(RowArgs a, cpx<T>* __restrict__ p
   ld.wx = a.fwx[f];
   FullRowStore<T, cpx<T>, GX> st;
   st.out = phys + (((size_t)task * a.nin + f) * a.gy + y0) * GX;
-  fft_io<T, GX, RY, SS, NT, +1, PLAN>(buf, twp, ld, st);
+  fft_io_pf<T, GX, RY, SS, NT, +1, PLAN, kPfRow, kFinal>(buf, twp, ld, st);
 }
 
 // in: [task][NIN][npts], out: [task][NOUT][npts]
@@ -529,7 +546,7 @@ __global__ __launch_bounds__(NT) void k_rowfwd(RowArgs a, const cpx<T>* __restri
   st.ostride = a.ostride;
   st.owx = a.out_per_field ? a.fwx[s] : a.owx;
   st.omx0 = a.out_per_field ? a.fmx0[s] : a.omx0;
-  fft_io<T, GX, RY, SS, NT, -1, PLAN>(buf, twp, ld, st);
+  fft_io_pf<T, GX, RY, SS, NT, -1, PLAN, kPfRow, kFinal>(buf, twp, ld, st);
 }
 
 // Adjoint of the pointwise products: v [task][NIN][npts] (forward inputs),
@@ -580,7 +597,7 @@ __global__ __launch_bounds__(NT) void k_synth(SynthArgs a, const cpx<T>* __restr
   ld.wx = a.wx;
   FullRowStore<T, cpx<double>, GX> st;
   st.out = a.out + ((size_t)seq * a.gy + y0) * GX;
-  fft_io<T, GX, RY, SS, NT, +1, 0>(buf, twp, ld, st);
+  fft_io_pf<T, GX, RY, SS, NT, +1, 0, kPfSynth, kFinal>(buf, twp, ld, st);
 }
 
 }  // namespace glw
