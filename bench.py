@@ -839,7 +839,20 @@ def main():
             e1.record()
             torch.cuda.synchronize()
             rates[name] = nbytes / (e0.elapsed_time(e1) / 10 * 1e-3) / 1e9
-        print(json.dumps({"calibration": "hbm", "GBps": rates, "frac_of_8TBps": {k: v / HBM_PEAK_GBS for k, v in rates.items()}}))
+        # the library's own streaming kernels (16 bytes per thread, many short workgroups)
+        import ctypes
+
+        from green_laplace_wave_reconstruction_amd import _native
+
+        lib = _native.load_library()
+        probe = {}
+        for name, kind, nbytes in (("copy (read + write)", 0, 2 * n), ("fill (write)", 1, n), ("read", 2, n)):
+            ms = ctypes.c_double()
+            _native._check(lib.glw_hbm_probe(kind, x.data_ptr(), y.data_ptr(), n, 10, ctypes.byref(ms)))
+            probe[name] = nbytes / (ms.value * 1e-3) / 1e9
+        print(json.dumps({"calibration": "hbm", "GBps_torch": rates, "GBps_glw_probe": probe,
+                          "frac_of_8TBps": {"torch": {k: v / HBM_PEAK_GBS for k, v in rates.items()},
+                                            "glw_probe": {k: v / HBM_PEAK_GBS for k, v in probe.items()}}}))
         return
     if args.calibrate_copy:
         import torch

@@ -66,7 +66,7 @@ EXPORTS = ("glw_plan_create", "glw_plan_destroy", "glw_linear_execute", "glw_ord
            "glw_adjoint_execute", "glw_plan_autotune", "glw_plan_get_schedule", "glw_plan_set_schedule", "glw_plan_hold",
            "glw_batch_create",
            "glw_batch_launch", "glw_batch_destroy", "glw_plan_profile", "glw_plan_stage_times", "glw_plan_buffer",
-           "glw_fft_length_supported", "glw_modal_length_supported", "glw_modal_workspace_size",
+           "glw_fft_length_supported", "glw_hbm_probe", "glw_modal_length_supported", "glw_modal_workspace_size",
            "glw_modal_execute", "glw_modal_stage_times", "glw_modal_execute_noalloc", "glw_eta20_workspace_size", "glw_eta20_execute", "glw_last_error", "glw_version",
            "glw_abi_sizeof")
 
@@ -126,6 +126,8 @@ def load_library(path: str | os.PathLike | None = None):
     lib.glw_plan_buffer.restype = c_int
     lib.glw_fft_length_supported.argtypes = [c_int32]
     lib.glw_fft_length_supported.restype = c_int
+    lib.glw_hbm_probe.argtypes = [c_int32, c_void_p, c_void_p, c_size_t, c_int32, _DP]
+    lib.glw_hbm_probe.restype = c_int
     lib.glw_last_error.restype = c_char_p
     lib.glw_version.restype = c_char_p
     if path is None:

@@ -288,6 +288,18 @@ struct NoIO {
   static constexpr bool SEQ_FAST = false;
 };
 
+// A functor may also gather or scatter all R elements of one butterfly at
+// once -- elements j, j + STEP, ..., j + (R - 1) STEP of sequence seq -- by
+// declaring BATCH = true and providing
+//   template <int R, int STEP> void load_batch(int seq, int j, cpx<T>* v) const;
+//   template <int R, int STEP> void store_batch(int seq, int j, const cpx<T>* v);
+// so that index wrapping and row addressing happen once per butterfly instead
+// of once per element.  The values are the same as R calls of operator().
+template <class F, class = void>
+struct io_batch : std::false_type {};
+template <class F>
+struct io_batch<F, std::void_t<decltype(F::BATCH)>> : std::integral_constant<bool, F::BATCH> {};
+
 // Compile-time shape of stage S: radix, butterflies per sequence and per
 // thread, twiddle step, which ends touch global memory and the lane mapping.
 template <int L, int NSEQ, int NT, int S, int NS, int PLAN, class LD, class ST>
@@ -350,7 +362,9 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const 
       if (TOTAL % NT == 0 || b < TOTAL) {
         const int seq = SEQ_FAST ? b % NSEQ : b / NB;
         const int j = SEQ_FAST ? b / NSEQ : b - seq * NB;
-        if constexpr (GLOAD) {
+        if constexpr (GLOAD && io_batch<LD>::value) {
+          ld.template load_batch<R, NB>(seq, j, v[p]);
+        } else if constexpr (GLOAD) {
 #pragma unroll
           for (int r = 0; r < R; ++r) v[p][r] = ld(seq, j + r * NB);
         } else {
@@ -391,7 +405,9 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const 
         const int k = j % NS;
         dft_reg<R, SIGN>(v[p]);
         const int base = (j / NS) * NS * R + k;
-        if constexpr (GSTORE) {
+        if constexpr (GSTORE && io_batch<ST>::value) {
+          st.template store_batch<R, NS>(seq, base, v[p]);
+        } else if constexpr (GSTORE) {
 #pragma unroll
           for (int r = 0; r < R; ++r) st(seq, base + r * NS, v[p][r]);
         } else {

@@ -2336,6 +2336,25 @@ double glw_plan_cost(const glw_plan* P, int kind, int nt) {
   return c * nt;
 }
 
+// Streaming kernels of the HBM calibration probe (glw_hbm_probe): one 16-byte
+// element per thread, so every launch has many short workgroups in flight.
+__global__ __launch_bounds__(256) void k_probe_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                    size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+__global__ __launch_bounds__(256) void k_probe_fill(double2* __restrict__ dst, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = make_double2(1.0, (double)i);
+}
+// read-only: the sum is written only if it equals a value no finite input sums to
+__global__ __launch_bounds__(256) void k_probe_read(const double2* __restrict__ src, double2* __restrict__ sink,
+                                                    size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const double v = i < n ? src[i].x + src[i].y : 0.0;
+  if (v == __builtin_inf()) sink[0] = make_double2(v, v);
+}
+
 extern "C" {
 
 int glw_plan_create(const glw_plan_desc* desc, glw_plan** plan) {
@@ -2524,6 +2543,34 @@ int glw_fft_length_supported(int32_t n) { return length_supported(n) ? 1 : 0; }
 const char* glw_last_error(void) { return g_err.c_str(); }
 
 const char* glw_version(void) { return "glw 0.1.0 gfx950"; }
+
+int glw_hbm_probe(int32_t kind, const double* src, double* dst, size_t bytes, int32_t reps, double* ms) {
+  if (!ms || reps < 1 || bytes < 16 || (kind != 1 && !src) || (kind != 2 && !dst) || kind < 0 || kind > 2)
+    return fail(GLW_EINVAL, "glw_hbm_probe: bad arguments");
+  const size_t n = bytes / 16;
+  const dim3 grid((unsigned)((n + 255) / 256));
+  if ((n + 255) / 256 > 0x7fffffffull) return fail(GLW_EINVAL, "glw_hbm_probe: buffer too large");
+  auto launch = [&]() {
+    if (kind == 0) hipLaunchKernelGGL(k_probe_copy, grid, dim3(256), 0, 0, (const double2*)src, (double2*)dst, n);
+    else if (kind == 1) hipLaunchKernelGGL(k_probe_fill, grid, dim3(256), 0, 0, (double2*)dst, n);
+    else hipLaunchKernelGGL(k_probe_read, grid, dim3(256), 0, 0, (const double2*)src, (double2*)dst, n);
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  GLW_HIP_CHECK(hipEventCreate(&e0));
+  GLW_HIP_CHECK(hipEventCreate(&e1));
+  launch();  // warm-up
+  GLW_HIP_CHECK(hipGetLastError());
+  GLW_HIP_CHECK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; ++r) launch();
+  GLW_HIP_CHECK(hipEventRecord(e1, 0));
+  GLW_HIP_CHECK(hipEventSynchronize(e1));
+  float t = 0.0f;
+  GLW_HIP_CHECK(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)t / reps;
+  return GLW_OK;
+}
 
 size_t glw_abi_sizeof(int32_t which) {
   switch (which) {

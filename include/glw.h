@@ -302,6 +302,13 @@ int glw_plan_buffer(glw_plan* plan, const char* name, void** ptr, size_t* bytes)
 /* Largest FFT length the kernels are instantiated for, and whether a length is one. */
 int glw_fft_length_supported(int32_t n);
 
+/* HBM calibration probe (measurement only; no counterpart in the reference):
+ * times `reps` launches of a plain streaming kernel over `bytes` of device
+ * memory on the default stream with HIP events and returns the mean launch
+ * time in *ms.  kind 0: copy src -> dst; 1: fill dst (write only); 2: read src
+ * (dst receives nothing; pass any 16-byte device buffer).  Synchronises. */
+int glw_hbm_probe(int32_t kind, const double* src, double* dst, size_t bytes, int32_t reps, double* ms);
+
 /* ---- Single-direction modal time path -------------------------------------
  *
  * glw_modal_execute replaces _modal_sum_components
