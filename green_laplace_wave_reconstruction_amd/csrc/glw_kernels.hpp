@@ -149,12 +149,12 @@ __device__ __forceinline__ cpx<double> apply_mul(int code, cpx<double> s, const 
   return {re * s.x - im * s.y, re * s.y + im * s.x};
 }
 
-// Streaming store (global_store ... nt): the column-inverse passes write each
+// Streaming store (global_store ... nt): the column passes write each
 // intermediate once, in whole 128-byte lines, for the next pass to read once.
 // Measured on MI355X (GL order 3, 512^2, 64 instants): the two column-inverse
 // stages 0.575 -> 0.444 ms and 0.761 -> 0.607 ms.  The same hint on the row
-// pass outputs changes nothing, and on the column-forward box outputs (rows not
-// line-aligned) it is 1.6x slower, so only k_colinv uses it.
+// pass outputs changes nothing, and on box outputs whose rows are not
+// line-aligned it is 1.6x slower (k_colfwd uses it only at a padded pitch).
 template <typename T>
 __device__ __forceinline__ void store_stream(cpx<T>* p, cpx<T> v) {
   using V = T __attribute__((ext_vector_type(2)));
@@ -296,8 +296,13 @@ struct ColFwdArgs {
   int omx0, owx, omy0, owy; // output box (columns = the row kernel's output box)
   const uint8_t* mask;      // [owy*owx]
   double scale;
-  cpx<double>* out;         // [task][nsrc][owy*owx]
+  cpx<double>* out;         // [task][nsrc][owy][opitch]
   uint64_t* fmax;           // [task][nsrc] or null
+  // Row pitch of the output boxes (0: owx).  A multiple of 8 makes every
+  // 8-column tile of a box row one whole 128-byte line, written with a
+  // streaming store; at pitch owx the rows are not line-aligned and two
+  // workgroups write parts of one line.
+  int opitch = 0;
 };
 
 template <int NTH = NT>
@@ -343,17 +348,21 @@ struct ColFwdStore {
   double scale;
   double vmax;
   int col0, owx, owy, omy0, gy;
-  __device__ __forceinline__ void put(int b, cpx<T> v) {
+  int opitch;               // output row pitch
+  bool stream;              // line-aligned rows: streaming stores
+  __device__ __forceinline__ void put(int r, int col, cpx<T> v) {
     cpx<double> d = {double(v.x) * scale, double(v.y) * scale};
-    if (!mask[b]) d = {0.0, 0.0};
+    if (!mask[r * owx + col]) d = {0.0, 0.0};
     else vmax = fmax(vmax, hypot(d.x, d.y));
-    out[b] = d;
+    cpx<double>* p = out + (size_t)r * opitch + col;
+    if (stream) store_stream(p, d);
+    else *p = d;
   }
   __device__ __forceinline__ void operator()(int c, int e, cpx<T> v) {
     const int col = col0 + c;
     const int r = wrap(e - omy0, gy);
     if (col >= owx || r >= owy) return;
-    put(r * owx + col, v);
+    put(r, col, v);
   }
   static constexpr bool BATCH = true;
   template <int R, int STEP>
@@ -365,7 +374,7 @@ struct ColFwdStore {
     for (int q = 0; q < R; ++q) {
       int r = r0 + q * STEP;
       if (r >= gy) r -= gy;
-      if (r < owy) put(r * owx + col, v[q]);
+      if (r < owy) put(r, col, v[q]);
     }
   }
 };
@@ -388,7 +397,10 @@ __global__ __launch_bounds__(NTH) void k_colfwd(ColFwdArgs a, const cpx<T>* __re
   ld.col0 = blockIdx.x * CW;
   ld.owx = a.owx;
   ColFwdStore<T> st;
-  st.out = a.out + ((size_t)task * a.nsrc + s) * a.owy * a.owx;
+  const int pitch = a.opitch ? a.opitch : a.owx;
+  st.out = a.out + ((size_t)task * a.nsrc + s) * a.owy * pitch;
+  st.opitch = pitch;
+  st.stream = pitch % 8 == 0;
   st.mask = a.mask;
   st.scale = a.scale;
   st.vmax = 0.0;

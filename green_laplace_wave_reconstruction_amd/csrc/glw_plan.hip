@@ -148,9 +148,24 @@ __device__ __forceinline__ cpx<double> floored(cpx<double> x, double level) {
 }
 __device__ __forceinline__ double bits_to_d(uint64_t b) { return __longlong_as_double((long long)b); }
 
+// Layout of spectra written by a column-forward pass at a padded row pitch:
+// spectrum k, box bin b (row-major over box width w) at k * slab + (b / w) * p + b % w.
+struct SpecLayout {
+  int w, p;     // box width, row pitch
+  size_t slab;  // elements per spectrum (box rows x p)
+  __device__ __forceinline__ size_t bin(int b) const {
+    const int r = b / w;
+    return (size_t)r * p + (b - r * w);
+  }
+};
+inline SpecLayout spec_layout(const glw_box& box) {
+  return SpecLayout{box.wx, sw(box.wx), (size_t)box.wy * sw(box.wx)};
+}
+
 // Inner GL accumulation of the pair state (_order3.py:240-259).
 struct AccInnerArgs {
-  const cpx<double>* spec;  // [nto*I][4][box]
+  const cpx<double>* spec;  // [nto*I][4] spectra in layout lay
+  SpecLayout lay;
   const uint64_t* fmax;     // [nto*I][4]
   double floor_level;
   int I, box;
@@ -170,12 +185,13 @@ __global__ __launch_bounds__(NT) void k_acc_inner(AccInnerArgs g) {
   const bool on = b < g.box && g.mask[b];
   if (on) {
     const double A = g.a[b];
+    const size_t bs = g.lay.bin(b);
     for (int i = 0; i < g.I; ++i) {
       const size_t task = (size_t)to * g.I + i;
       cpx<double> s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        s[k] = floored(g.spec[(task * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 4 + k]));
+        s[k] = floored(g.spec[(task * 4 + k) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 4 + k]));
       const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
       // e += c (-A sh sd + ch sk)
       const cpx<double> ze = {c * (-A * shv * s[0].x + chv * s[1].x), c * (-A * shv * s[0].y + chv * s[1].y)};
@@ -203,7 +219,8 @@ __global__ __launch_bounds__(NT) void k_acc_inner(AccInnerArgs g) {
 // Floors of the cubic forcing spectra and the maxima of the psi combinations
 // (_order3.py:103-122).
 struct ComboArgs {
-  const cpx<double>* spec;  // [nto][4][box]: F_K, F_D, C, C_t
+  const cpx<double>* spec;  // [nto][4] spectra in layout lay: F_K, F_D, C, C_t
+  SpecLayout lay;
   const uint64_t* fmax;     // [nto][4]
   double floor_level;
   int box;
@@ -217,10 +234,11 @@ __global__ __launch_bounds__(NT) void k_combo_outer(ComboArgs g) {
   const int to = blockIdx.y;
   double m0 = 0.0, m1 = 0.0;
   if (b < g.box && g.mask[b]) {
+    const size_t bs = g.lay.bin(b);
     cpx<double> s[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      s[k] = floored(g.spec[((size_t)to * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)to * 4 + k]));
+      s[k] = floored(g.spec[((size_t)to * 4 + k) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[(size_t)to * 4 + k]));
     const double A = g.a[b];
     const cpx<double> kp = {s[0].x + A * s[2].x, s[0].y + A * s[2].y};
     const cpx<double> dp = csub(s[1], s[3]);
@@ -234,7 +252,8 @@ __global__ __launch_bounds__(NT) void k_combo_outer(ComboArgs g) {
 
 // Outer GL accumulation of eta33, psi33 spectra (_order3.py:117-138).
 struct AccOuterArgs {
-  const cpx<double>* spec;  // [nt*O][4][box]
+  const cpx<double>* spec;  // [nt*O][4] spectra in layout lay
+  SpecLayout lay;
   const uint64_t* fmax;     // [nt*O][4]
   const uint64_t* cmax;     // [nt*O][2]
   double floor_level;
@@ -254,12 +273,13 @@ __global__ __launch_bounds__(NT) void k_acc_outer(AccOuterArgs g) {
   cpx<double> eta = {0, 0}, psi = {0, 0};
   if (g.mask[b]) {
     const double A = g.a[b];
+    const size_t bs = g.lay.bin(b);
     for (int o = 0; o < g.O; ++o) {
       const size_t to = (size_t)t * g.O + o;
       cpx<double> s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        s[k] = floored(g.spec[(to * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[to * 4 + k]));
+        s[k] = floored(g.spec[(to * 4 + k) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[to * 4 + k]));
       const cpx<double> kp = floored(cpx<double>{s[0].x + A * s[2].x, s[0].y + A * s[2].y},
                                      g.floor_level * bits_to_d(g.cmax[to * 2]));
       const cpx<double> dp = floored(csub(s[1], s[3]), g.floor_level * bits_to_d(g.cmax[to * 2 + 1]));
@@ -514,9 +534,10 @@ __global__ __launch_bounds__(NT) void k_adj_acc_outer(AccOuterArgs g, const cpx<
     const cpx<double> e = ge[b];
     const double A = g.a[b], c = g.coef[o], chv = g.ch[(size_t)o * g.box + b], shv = g.sh[(size_t)o * g.box + b];
     // eta += c (A sh F_D - i ch F_K)
-    if (kept(g.spec[((size_t)o * 4) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)o * 4])))
+    const size_t bs = g.lay.bin(b);
+    if (kept(g.spec[((size_t)o * 4) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[(size_t)o * 4])))
       gk = {-c * chv * e.y, c * chv * e.x};  // conj(-i c ch) e = i c ch e
-    if (kept(g.spec[((size_t)o * 4 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)o * 4 + 1])))
+    if (kept(g.spec[((size_t)o * 4 + 1) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[(size_t)o * 4 + 1])))
       gd = {c * A * shv * e.x, c * A * shv * e.y};
   }
   gs[((size_t)o * 2) * g.box + b] = gk;
@@ -539,9 +560,10 @@ __global__ __launch_bounds__(NT) void k_adj_acc_inner(AccInnerArgs g, const cpx<
     r[1] = {c * (chv * ge.x - shv * gph.y), c * (chv * ge.y + shv * gph.x)};
     r[2] = {c * (A * shv * get.y + chv * gpt.x), c * (-A * shv * get.x + chv * gpt.y)};
     r[3] = {c * (-chv * get.y - shv * gpt.x), c * (chv * get.x - shv * gpt.y)};
+    const size_t bs = g.lay.bin(b);
 #pragma unroll
     for (int k = 0; k < 4; ++k)
-      if (!kept(g.spec[((size_t)task * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[(size_t)task * 4 + k])))
+      if (!kept(g.spec[((size_t)task * 4 + k) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[(size_t)task * 4 + k])))
         r[k] = {0, 0};
   }
 #pragma unroll
@@ -961,13 +983,13 @@ int build(glw_plan* P) {
     const size_t wmax = sw(std::max(d.p1.wx, d.p2.wx));
     if ((rc = P->reserve("tin", nt * O * I * 12 * g2 * sw(d.p1.wx) * csz))) return rc;
     if ((rc = P->reserve("uin", nt * O * I * 4 * g2 * sw(d.p2.wx) * csz))) return rc;
-    if ((rc = P->reserve("sin", nt * O * I * 4 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("sin", nt * O * I * 4 * spec_layout(d.p2).slab * zsz))) return rc;
     if ((rc = P->reserve("max_in", nt * O * I * 4 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("pair", nt * O * 4 * b2 * zsz))) return rc;
     if ((rc = P->reserve("max_et", nt * O * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("tout", nt * O * 21 * g3 * wmax * csz))) return rc;
     if ((rc = P->reserve("uout", nt * O * 4 * g3 * sw(d.p3.wx) * csz))) return rc;
-    if ((rc = P->reserve("sout", nt * O * 4 * b3 * zsz))) return rc;
+    if ((rc = P->reserve("sout", nt * O * 4 * spec_layout(d.p3).slab * zsz))) return rc;
     if ((rc = P->reserve("max_out", nt * O * 4 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("max_combo", nt * O * 2 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("ep33", nt * 2 * b3 * zsz))) return rc;
@@ -1253,12 +1275,14 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g2x * d.g2y);
     c.out = P->buf<cpx<double>>("sin");
     c.fmax = P->buf<uint64_t>("max_in");
+    c.opitch = spec_layout(d.p2).p;
     if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, colfwd_bytes(c, d.g2y, n_in), launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_et"), (size_t)n_to, st))) return rc;
   {
     AccInnerArgs g;
     g.spec = P->buf<cpx<double>>("sin");
+    g.lay = spec_layout(d.p2);
     g.fmax = P->buf<uint64_t>("max_in");
     g.floor_level = d.floor_level;
     g.I = I;
@@ -1350,11 +1374,13 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = (double)d.nx * d.ny / ((double)d.g3x * d.g3y);
     c.out = P->buf<cpx<double>>("sout");
     c.fmax = P->buf<uint64_t>("max_out");
+    c.opitch = spec_layout(d.p3).p;
     if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, colfwd_bytes(c, d.g3y, n_to), launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
     ComboArgs g;
     g.spec = P->buf<cpx<double>>("sout");
+    g.lay = spec_layout(d.p3);
     g.fmax = P->buf<uint64_t>("max_out");
     g.floor_level = d.floor_level;
     g.box = b3;
@@ -1369,6 +1395,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
   {
     AccOuterArgs g;
     g.spec = P->buf<cpx<double>>("sout");
+    g.lay = spec_layout(d.p3);
     g.fmax = P->buf<uint64_t>("max_out");
     g.cmax = P->buf<uint64_t>("max_combo");
     g.floor_level = d.floor_level;
@@ -2113,6 +2140,7 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     if ((rc = gather(B("adj_u"), U, d.p3, 1.0, 0, B("adj_e33"), st))) return rc;
     AccOuterArgs g;
     g.spec = B("sout");
+    g.lay = spec_layout(d.p3);
     g.fmax = P->buf<uint64_t>("max_out");
     g.cmax = P->buf<uint64_t>("max_combo");
     g.floor_level = d.floor_level;
@@ -2179,6 +2207,7 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     }
     AccInnerArgs gi;
     gi.spec = B("sin");
+    gi.lay = spec_layout(d.p2);
     gi.fmax = P->buf<uint64_t>("max_in");
     gi.floor_level = d.floor_level;
     gi.I = I;
