@@ -299,10 +299,12 @@ struct ColFwdArgs {
   cpx<double>* out;         // [task][nsrc][owy][opitch]
   uint64_t* fmax;           // [task][nsrc] or null
   // Row pitch of the output boxes (0: owx).  A multiple of 8 makes every
-  // 8-column tile of a box row one whole 128-byte line, written with a
-  // streaming store; at pitch owx the rows are not line-aligned and two
-  // workgroups write parts of one line.
+  // 8-column tile of a box row one whole 128-byte line; at pitch owx the rows
+  // are not line-aligned and two workgroups write parts of one line.
   int opitch = 0;
+  // Streaming stores for line-aligned outputs too large to stay in the
+  // last-level cache for their consumer (the host decides by size).
+  bool stream = false;
 };
 
 template <int NTH = NT>
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(NTH) void k_colfwd(ColFwdArgs a, const cpx<T>* __re
   const int pitch = a.opitch ? a.opitch : a.owx;
   st.out = a.out + ((size_t)task * a.nsrc + s) * a.owy * pitch;
   st.opitch = pitch;
-  st.stream = pitch % 8 == 0;
+  st.stream = a.stream && pitch % 8 == 0;
   st.mask = a.mask;
   st.scale = a.scale;
   st.vmax = 0.0;

@@ -161,6 +161,11 @@ struct SpecLayout {
 inline SpecLayout spec_layout(const glw_box& box) {
   return SpecLayout{box.wx, sw(box.wx), (size_t)box.wy * sw(box.wx)};
 }
+// Column-forward outputs of at least this many bytes are written with streaming
+// stores: on MI355X that speeds up the GL order-3 passes at 512^2 (1.1 GB of
+// spectra per launch) and slows the small eta44 launches (their consumers then
+// miss the cache).
+constexpr size_t kStreamBytes = size_t(64) << 20;
 
 // Inner GL accumulation of the pair state (_order3.py:240-259).
 struct AccInnerArgs {
@@ -298,7 +303,8 @@ __global__ __launch_bounds__(NT) void k_acc_outer(AccOuterArgs g) {
 
 // eta22 accumulation (spatial.py:192-221).
 struct AccEta22Args {
-  const cpx<double>* spec;  // [nt*J][2][box]
+  const cpx<double>* spec;  // [nt*J][2] spectra in layout lay
+  SpecLayout lay;
   const uint64_t* fmax;     // [nt*J][2]
   double floor_level;
   int J, box;
@@ -314,11 +320,13 @@ __global__ __launch_bounds__(NT) void k_acc_eta22(AccEta22Args g) {
   const int t = blockIdx.y;
   if (b >= g.box) return;
   const double A = g.a[b];
+  const size_t bs = g.lay.bin(b);
   cpx<double> acc = {0, 0};
   for (int j = 0; j < g.J; ++j) {
     const size_t task = (size_t)t * g.J + j;
-    const cpx<double> sd = floored(g.spec[(task * 2) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2]));
-    const cpx<double> sk = floored(g.spec[(task * 2 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2 + 1]));
+    const cpx<double> sd = floored(g.spec[(task * 2) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 2]));
+    const cpx<double> sk =
+        floored(g.spec[(task * 2 + 1) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 2 + 1]));
     const double c = g.coef[j], chv = g.ch[(size_t)j * g.box + b], shv = g.sh[(size_t)j * g.box + b];
     acc.x += c * (-A * shv * sd.x + chv * sk.x);
     acc.y += c * (-A * shv * sd.y + chv * sk.y);
@@ -328,8 +336,9 @@ __global__ __launch_bounds__(NT) void k_acc_eta22(AccEta22Args g) {
 
 // psi22 accumulation (spatial.py:276-308).
 struct AccPsi22Args {
-  const cpx<double>* spec;    // [nt*4][8][box]  (branch-node task, pair p: sd = 2p, sk = 2p+1)
-  const cpx<double>* contact; // [nt][1][box]
+  const cpx<double>* spec;    // [nt*4][8] spectra in layout lay (branch-node task, pair p: sd = 2p, sk = 2p+1)
+  const cpx<double>* contact; // [nt][1] spectra in layout lay
+  SpecLayout lay;
   int box;
   const double* inu;
   const uint8_t* mask;
@@ -345,21 +354,22 @@ __global__ __launch_bounds__(NT) void k_acc_psi22(AccPsi22Args g) {
   if (b >= g.box) return;
   cpx<double> res = {0, 0};
   const double inu = g.inu[b];
+  const size_t bs = g.lay.bin(b);
   for (int bn = 0; bn < 4; ++bn) {
     const double sgn = g.sign[bn / 2];
     const size_t task = (size_t)t * 4 + bn;
     cpx<double> resp = {0, 0};
     for (int p = 0; p < 4; ++p) {
       const double f = g.factor[((size_t)bn * 4 + p) * g.box + b];
-      const cpx<double> sd = g.spec[(task * 8 + 2 * p) * g.box + b];
-      const cpx<double> sk = g.spec[(task * 8 + 2 * p + 1) * g.box + b];
+      const cpx<double> sd = g.spec[(task * 8 + 2 * p) * g.lay.slab + bs];
+      const cpx<double> sk = g.spec[(task * 8 + 2 * p + 1) * g.lay.slab + bs];
       resp.x += f * (sd.x + sgn * sk.x * inu);
       resp.y += f * (sd.y + sgn * sk.y * inu);
     }
     const double c = g.coef[bn];
     res = {res.x - c * resp.y, res.y + c * resp.x};
   }
-  const cpx<double> ct = g.contact[(size_t)t * g.box + b];
+  const cpx<double> ct = g.contact[(size_t)t * g.lay.slab + bs];
   cpx<double> v = cadd(res, ct);
   if (!g.mask[b]) v = {0, 0};
   g.out[((size_t)t * 2 + 1) * g.box + b] = v;
@@ -370,7 +380,8 @@ __global__ __launch_bounds__(NT) void k_acc_psi22(AccPsi22Args g) {
 // Order-4 accumulations (paper/order4/cpp/gl_eta44_fftw.cpp)
 // ---------------------------------------------------------------------------
 struct Acc4Args {
-  const cpx<double>* spec;  // [nset*J][nsrc][box]
+  const cpx<double>* spec;  // [nset*J][nsrc] spectra in layout lay
+  SpecLayout lay;
   const uint64_t* fmax;     // [nset*J][nsrc]
   double floor_level;
   int J, box, nsrc, nacc;
@@ -392,12 +403,13 @@ __global__ __launch_bounds__(NT) void k_acc_pair44(Acc4Args g) {
   const bool on = b < g.box && g.mask[b];
   if (on) {
     const double A = g.a[b];
+    const size_t bs = g.lay.bin(b);
     for (int i = 0; i < g.J; ++i) {
       const size_t task = (size_t)set * g.J + i;
       cpx<double> s[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k)
-        s[k] = floored(g.spec[(task * 6 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 6 + k]));
+        s[k] = floored(g.spec[(task * 6 + k) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 6 + k]));
       const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
       // es += c (-A sh sd + ch sk)
       acc[0].x += c * (-A * shv * s[0].x + chv * s[1].x);
@@ -432,12 +444,13 @@ __global__ __launch_bounds__(NT) void k_acc_third44(Acc4Args g) {
   const bool on = b < g.box && g.mask[b];
   if (on) {
     const double A = g.a[b];
+    const size_t bs = g.lay.bin(b);
     for (int i = 0; i < g.J; ++i) {
       const size_t task = (size_t)set * g.J + i;
       cpx<double> s[4];
 #pragma unroll
       for (int k = 0; k < 4; ++k)
-        s[k] = floored(g.spec[(task * 4 + k) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 4 + k]));
+        s[k] = floored(g.spec[(task * 4 + k) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 4 + k]));
       const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
       // es += c (A sh fd - i ch fk)
       es.x += c * (A * shv * s[1].x + chv * s[0].y);
@@ -470,11 +483,12 @@ __global__ __launch_bounds__(NT) void k_acc_quartic44(Acc4Args g) {
   const bool on = b < g.box && g.mask[b];
   if (on) {
     const double A = g.a[b];
+    const size_t bs = g.lay.bin(b);
     for (int i = 0; i < g.J; ++i) {
       const size_t task = (size_t)set * g.J + i;
-      const cpx<double> fk = floored(g.spec[(task * 2) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2]));
+      const cpx<double> fk = floored(g.spec[(task * 2) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 2]));
       const cpx<double> fd =
-          floored(g.spec[(task * 2 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2 + 1]));
+          floored(g.spec[(task * 2 + 1) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 2 + 1]));
       const double c = g.coef[i], chv = g.ch[(size_t)i * g.box + b], shv = g.sh[(size_t)i * g.box + b];
       o.x += c * (A * shv * fd.x + chv * fk.y);
       o.y += c * (A * shv * fd.y - chv * fk.x);
@@ -515,9 +529,10 @@ __global__ __launch_bounds__(NT) void k_adj_acc_eta22(AccEta22Args g, const cpx<
   if (mask[b]) {
     const cpx<double> e = ge[b];
     const double A = g.a[b], c = g.coef[j], chv = g.ch[(size_t)j * g.box + b], shv = g.sh[(size_t)j * g.box + b];
-    if (kept(g.spec[(task * 2) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2])))
+    const size_t bs = g.lay.bin(b);
+    if (kept(g.spec[(task * 2) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 2])))
       gd = {-c * A * shv * e.x, -c * A * shv * e.y};
-    if (kept(g.spec[(task * 2 + 1) * g.box + b], g.floor_level * bits_to_d(g.fmax[task * 2 + 1])))
+    if (kept(g.spec[(task * 2 + 1) * g.lay.slab + bs], g.floor_level * bits_to_d(g.fmax[task * 2 + 1])))
       gk = {c * chv * e.x, c * chv * e.y};
   }
   gs[(task * 2) * g.box + b] = gd;
@@ -954,14 +969,14 @@ int build(glw_plan* P) {
     const size_t g2 = (size_t)d.g2y;
     if ((rc = P->reserve("e22_t", nt * J * 8 * g2 * sw(d.p1.wx) * csz))) return rc;
     if ((rc = P->reserve("e22_u", nt * J * 2 * g2 * sw(d.p2.wx) * csz))) return rc;
-    if ((rc = P->reserve("e22_s", nt * J * 2 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("e22_s", nt * J * 2 * spec_layout(d.p2).slab * zsz))) return rc;
     if ((rc = P->reserve("e22_max", nt * J * 2 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("p22_t", nt * 4 * 16 * g2 * sw(d.p1.wx) * csz))) return rc;
     if ((rc = P->reserve("p22_u", nt * 4 * 8 * g2 * sw(d.p2.wx) * csz))) return rc;
-    if ((rc = P->reserve("p22_s", nt * 4 * 8 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("p22_s", nt * 4 * 8 * spec_layout(d.p2).slab * zsz))) return rc;
     if ((rc = P->reserve("c22_t", nt * 2 * g2 * sw(d.p1.wx) * csz))) return rc;
     if ((rc = P->reserve("c22_u", nt * 1 * g2 * sw(d.p2.wx) * csz))) return rc;
-    if ((rc = P->reserve("c22_s", nt * 1 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("c22_s", nt * 1 * spec_layout(d.p2).slab * zsz))) return rc;
     if ((rc = P->reserve("ep22", nt * 2 * b2 * zsz))) return rc;
   }
   if (d.order >= 3) {
@@ -1032,7 +1047,7 @@ int build(glw_plan* P) {
     const size_t np = nt * J * J * J;  // pair tasks of the third level (the larger pair batch)
     if ((rc = P->reserve("o4_pt", np * 16 * g2 * sw(d.p1.wx) * csz))) return rc;
     if ((rc = P->reserve("o4_pu", np * 6 * g2 * sw(d.p2.wx) * csz))) return rc;
-    if ((rc = P->reserve("o4_ps", np * 6 * b2 * zsz))) return rc;
+    if ((rc = P->reserve("o4_ps", np * 6 * spec_layout(d.p2).slab * zsz))) return rc;
     if ((rc = P->reserve("o4_pm", np * 6 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_set3", nt * J * J * 5 * b2 * zsz))) return rc;
     if ((rc = P->reserve("o4_max3", nt * J * J * 5 * sizeof(uint64_t)))) return rc;
@@ -1040,13 +1055,13 @@ int build(glw_plan* P) {
     if ((rc = P->reserve("o4_max4", nt * J * 5 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_tt", nt * J * J * 31 * g3 * w12 * csz))) return rc;
     if ((rc = P->reserve("o4_tu", nt * J * J * 4 * g3 * sw(d.p3.wx) * csz))) return rc;
-    if ((rc = P->reserve("o4_ts", nt * J * J * 4 * b3 * zsz))) return rc;
+    if ((rc = P->reserve("o4_ts", nt * J * J * 4 * spec_layout(d.p3).slab * zsz))) return rc;
     if ((rc = P->reserve("o4_tm", nt * J * J * 4 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_th", nt * J * 3 * b3 * zsz))) return rc;
     if ((rc = P->reserve("o4_thm", nt * J * 3 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_qt", nt * J * 45 * g4 * w123 * csz))) return rc;
     if ((rc = P->reserve("o4_qu", nt * J * 2 * g4 * sw(o.p4.wx) * csz))) return rc;
-    if ((rc = P->reserve("o4_qs", nt * J * 2 * b4 * zsz))) return rc;
+    if ((rc = P->reserve("o4_qs", nt * J * 2 * spec_layout(o.p4).slab * zsz))) return rc;
     if ((rc = P->reserve("o4_qm", nt * J * 2 * sizeof(uint64_t)))) return rc;
     if ((rc = P->reserve("o4_out", nt * b4 * zsz))) return rc;
     if ((rc = P->reserve("o4_outm", nt * sizeof(uint64_t)))) return rc;
@@ -1276,6 +1291,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.out = P->buf<cpx<double>>("sin");
     c.fmax = P->buf<uint64_t>("max_in");
     c.opitch = spec_layout(d.p2).p;
+    c.stream = (size_t)n_in * 4 * spec_layout(d.p2).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, colfwd_bytes(c, d.g2y, n_in), launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_et"), (size_t)n_to, st))) return rc;
@@ -1375,6 +1391,7 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.out = P->buf<cpx<double>>("sout");
     c.fmax = P->buf<uint64_t>("max_out");
     c.opitch = spec_layout(d.p3).p;
+    c.stream = (size_t)n_to * 4 * spec_layout(d.p3).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, colfwd_bytes(c, d.g3y, n_to), launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
@@ -1447,7 +1464,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.gy = d.g2y;
     return r;
   };
-  auto fwd_args = [&](const char* uin, int nsrc, const char* sout, uint64_t* fmax, const uint8_t* mask) {
+  auto fwd_args = [&](const char* uin, int nsrc, const char* sout, uint64_t* fmax, const uint8_t* mask, int ntask) {
     ColFwdArgs c;
     c.in = P->ws.at(uin).first;
     c.nsrc = nsrc;
@@ -1460,6 +1477,8 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.scale = up;
     c.out = P->buf<cpx<double>>(sout);
     c.fmax = fmax;
+    c.opitch = spec_layout(d.p2).p;
+    c.stream = (size_t)ntask * nsrc * spec_layout(d.p2).slab * sizeof(cpx<double>) >= kStreamBytes;
     return c;
   };
   static const int e_codes[8] = {M_ONE, M_NU, M_NU2, M_QX_NU, M_QY_NU, M_Q2_NU, M_QX, M_QY};
@@ -1481,11 +1500,12 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     if ((rc = GLW_STAGE(P, "o2_eta_row", st, row_bytes(row_args("e22_t", 8, "e22_u", 2), nt * J), (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("e22_max"), (size_t)nt * J * 2, st))) return rc;
-    if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, colfwd_bytes(fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), d.g2y, nt * J), launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
+    if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, colfwd_bytes(fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2, nt * J), d.g2y, nt * J), launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2, nt * J), nt * J,
                                P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
     AccEta22Args g;
     g.spec = P->buf<cpx<double>>("e22_s");
+    g.lay = spec_layout(d.p2);
     g.fmax = P->buf<uint64_t>("e22_max");
     g.floor_level = d.floor_level;
     g.J = J;
@@ -1526,7 +1546,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     }
     if ((rc = GLW_STAGE(P, "o2_psi_row", st, row_bytes(row_args("p22_t", 16, "p22_u", 8), nt * 4), (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, colfwd_bytes(fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), d.g2y, nt * 4), launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
+    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, colfwd_bytes(fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2, nt * 4), d.g2y, nt * 4), launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2, nt * 4), nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   // ---- psi22 Taylor contact: tasks t
@@ -1545,12 +1565,13 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, colinv_bytes(a, d.g2y, nt), launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
     if ((rc = GLW_STAGE(P, "o2_contact_row", st, row_bytes(row_args("c22_t", 2, "c22_u", 1), nt), (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, colfwd_bytes(fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), d.g2y, nt), launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
+    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, colfwd_bytes(fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2, nt), d.g2y, nt), launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2, nt), nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   {
     AccPsi22Args g;
     g.spec = P->buf<cpx<double>>("p22_s");
+    g.lay = spec_layout(d.p2);
     g.contact = P->buf<cpx<double>>("c22_s");
     g.box = b2;
     g.inu = P->t2.inu;
@@ -1634,12 +1655,15 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.mask = P->mask2;
     c.scale = nn / ((double)d.g2x * d.g2y);
     c.out = P->buf<cpx<double>>("o4_ps");
+    c.opitch = spec_layout(d.p2).p;
     c.fmax = P->buf<uint64_t>("o4_pm");
+    c.stream = (size_t)ntask * 6 * spec_layout(d.p2).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc2 = GLW_STAGE(P, "o4_pair_colfwd", st, colfwd_bytes(c, d.g2y, ntask), launch_colfwd<T>(d.g2y, c, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
       return rc2;
     if ((rc2 = fill_zero(P->buf<uint64_t>(setmax), (size_t)nset * 5, st))) return rc2;
     Acc4Args g;
     g.spec = P->buf<cpx<double>>("o4_ps");
+    g.lay = spec_layout(d.p2);
     g.fmax = P->buf<uint64_t>("o4_pm");
     g.floor_level = d.floor_level;
     g.J = J;
@@ -1722,12 +1746,15 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.mask = P->mask3;
     c.scale = nn / ((double)d.g3x * d.g3y);
     c.out = P->buf<cpx<double>>("o4_ts");
+    c.opitch = spec_layout(d.p3).p;
     c.fmax = P->buf<uint64_t>("o4_tm");
+    c.stream = (size_t)ntask * 4 * spec_layout(d.p3).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o4_third_colfwd", st, colfwd_bytes(c, d.g3y, ntask), launch_colfwd<T>(d.g3y, c, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_thm"), (size_t)nt * J * 3, st))) return rc;
     Acc4Args g;
     g.spec = P->buf<cpx<double>>("o4_ts");
+    g.lay = spec_layout(d.p3);
     g.fmax = P->buf<uint64_t>("o4_tm");
     g.floor_level = d.floor_level;
     g.J = J;
@@ -1820,12 +1847,15 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.mask = P->mask4;
     c.scale = nn / ((double)o.g4x * o.g4y);
     c.out = P->buf<cpx<double>>("o4_qs");
+    c.opitch = spec_layout(o.p4).p;
     c.fmax = P->buf<uint64_t>("o4_qm");
+    c.stream = (size_t)ntask * 2 * spec_layout(o.p4).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o4_quartic_colfwd", st, colfwd_bytes(c, o.g4y, ntask), launch_colfwd<T>(o.g4y, c, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_outm"), (size_t)nt, st))) return rc;
     Acc4Args g;
     g.spec = P->buf<cpx<double>>("o4_qs");
+    g.lay = spec_layout(o.p4);
     g.fmax = P->buf<uint64_t>("o4_qm");
     g.floor_level = d.floor_level;
     g.J = J;
@@ -2096,6 +2126,7 @@ int adjoint_run(glw_plan* P, const double* s0, const double* times, int orders, 
     if ((rc = gather(B("adj_u"), U, d.p2, 1.0, 0, B("adj_e22"), st))) return rc;
     AccEta22Args g;
     g.spec = B("e22_s");
+    g.lay = spec_layout(d.p2);
     g.fmax = P->buf<uint64_t>("e22_max");
     g.floor_level = d.floor_level;
     g.J = J;
