@@ -155,6 +155,12 @@ __device__ __forceinline__ cpx<double> apply_mul(int code, cpx<double> s, const 
 // stages 0.575 -> 0.444 ms and 0.761 -> 0.607 ms.  The same hint on the row
 // pass outputs changes nothing, and on box outputs whose rows are not
 // line-aligned it is 1.6x slower (k_colfwd uses it only at a padded pitch).
+// The launchers set it by output size (kStreamBytes).
+// Column-pass outputs of at least this many bytes per launch use streaming
+// stores; smaller ones stay cacheable for their consumer (an eta44 launch at
+// 256^2, or the small sea states of a batch, slow down when their consumer
+// misses the cache).
+constexpr size_t kStreamBytes = size_t(64) << 20;
 template <typename T>
 __device__ __forceinline__ void store_stream(cpx<T>* p, cpx<T> v) {
   using V = T __attribute__((ext_vector_type(2)));
@@ -188,6 +194,7 @@ struct ColInvArgs {
   double fscale[kMaxFields];
   void* out;                // cpx<T> [task][out_nfield][gy][ostride]
   int out_nfield, out_foff, ostride;
+  bool stream = false;      // streaming stores (set by the launcher for large outputs)
 };
 
 // Box column (damped, floored, multiplied, scaled) as the first stage's input;
@@ -235,8 +242,13 @@ struct ColInvStore {
   static constexpr bool SEQ_FAST = true;
   cpx<T>* out;
   int ostride, col0, wx;
+  bool stream;
+  __device__ __forceinline__ void put(cpx<T>* p, cpx<T> v) {
+    if (stream) store_stream(p, v);
+    else *p = v;
+  }
   __device__ __forceinline__ void operator()(int c, int e, cpx<T> v) {
-    if (col0 + c < wx) store_stream(out + (size_t)e * ostride + col0 + c, v);
+    if (col0 + c < wx) put(out + (size_t)e * ostride + col0 + c, v);
   }
   static constexpr bool BATCH = true;
   template <int R, int STEP>
@@ -244,7 +256,7 @@ struct ColInvStore {
     if (col0 + c >= wx) return;
     cpx<T>* p = out + (size_t)j * ostride + col0 + c;
 #pragma unroll
-    for (int q = 0; q < R; ++q) store_stream(p + (size_t)q * STEP * ostride, v[q]);
+    for (int q = 0; q < R; ++q) put(p + (size_t)q * STEP * ostride, v[q]);
   }
 };
 
@@ -282,6 +294,7 @@ __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __re
   st.ostride = a.ostride;
   st.col0 = ld.col0;
   st.wx = a.wx;
+  st.stream = a.stream;
   fft_io_pf<T, GY, CW, seq_stride<GY>(), NTH, +1, PLAN, kPfCol, kFinal>(buf, twp, ld, st);
 }
 
@@ -303,7 +316,7 @@ struct ColFwdArgs {
   // are not line-aligned and two workgroups write parts of one line.
   int opitch = 0;
   // Streaming stores for line-aligned outputs too large to stay in the
-  // last-level cache for their consumer (the host decides by size).
+  // last-level cache for their consumer (set by the launcher by size).
   bool stream = false;
 };
 

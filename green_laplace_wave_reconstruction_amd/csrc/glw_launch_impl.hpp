@@ -52,16 +52,31 @@ constexpr int split_ry() {
 }
 
 // ---------------------------------------------------------------- columns
+// Streaming stores for outputs of kStreamBytes or more per launch, and only when
+// a workgroup's row segment (CW columns) fills whole 128-byte lines: narrower
+// tiles (CW = 2 at 768 and 1024 points) leave partial lines that the L2 must
+// merge across workgroups (streamed, k_colinv<1024, 2> ran 2x slower).
+template <typename T, int CW>
+constexpr bool whole_lines() {
+  return (CW * sizeof(cpx<T>)) % 128 == 0;
+}
+
 template <typename T, int L, int CW, int PLAN, int NTH>
 void colinv_go(const ColInvArgs& a, int ntask, const void* tw, hipStream_t st) {
+  ColInvArgs b = a;
+  b.stream = whole_lines<T, CW>() && a.ostride % 8 == 0 &&
+             (size_t)ntask * a.nfield * L * a.ostride * sizeof(cpx<T>) >= kStreamBytes;
   hipLaunchKernelGGL((k_colinv<T, L, CW, PLAN, NTH>), dim3((a.wx + CW - 1) / CW, a.nfield, ntask), dim3(NTH), 0, st,
-                     a, (const cpx<T>*)tw);
+                     b, (const cpx<T>*)tw);
 }
 
 template <typename T, int L, int CW, int PLAN, int NTH>
 void colfwd_go(const ColFwdArgs& a, int ntask, const void* tw, hipStream_t st) {
+  ColFwdArgs b = a;
+  const int pitch = a.opitch ? a.opitch : a.owx;
+  b.stream = whole_lines<double, CW>() && (size_t)ntask * a.nsrc * a.owy * pitch * sizeof(cpx<double>) >= kStreamBytes;
   hipLaunchKernelGGL((k_colfwd<T, L, CW, PLAN, NTH>), dim3((a.owx + CW - 1) / CW, a.nsrc, ntask), dim3(NTH), 0, st,
-                     a, (const cpx<T>*)tw);
+                     b, (const cpx<T>*)tw);
 }
 
 // Column tiles: 0 = widest tile, 256 threads; 1 = half tile, 256 threads;

@@ -161,11 +161,7 @@ struct SpecLayout {
 inline SpecLayout spec_layout(const glw_box& box) {
   return SpecLayout{box.wx, sw(box.wx), (size_t)box.wy * sw(box.wx)};
 }
-// Column-forward outputs of at least this many bytes are written with streaming
-// stores: on MI355X that speeds up the GL order-3 passes at 512^2 (1.1 GB of
-// spectra per launch) and slows the small eta44 launches (their consumers then
-// miss the cache).
-constexpr size_t kStreamBytes = size_t(64) << 20;
+
 
 // Inner GL accumulation of the pair state (_order3.py:240-259).
 struct AccInnerArgs {
@@ -1291,7 +1287,6 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.out = P->buf<cpx<double>>("sin");
     c.fmax = P->buf<uint64_t>("max_in");
     c.opitch = spec_layout(d.p2).p;
-    c.stream = (size_t)n_in * 4 * spec_layout(d.p2).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o3_inner_colfwd", st, colfwd_bytes(c, d.g2y, n_in), launch_colfwd<T>(d.g2y, c, n_in, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   if ((rc = fill_zero(P->buf<uint64_t>("max_et"), (size_t)n_to, st))) return rc;
@@ -1391,7 +1386,6 @@ int order3_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.out = P->buf<cpx<double>>("sout");
     c.fmax = P->buf<uint64_t>("max_out");
     c.opitch = spec_layout(d.p3).p;
-    c.stream = (size_t)n_to * 4 * spec_layout(d.p3).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o3_outer_colfwd", st, colfwd_bytes(c, d.g3y, n_to), launch_colfwd<T>(d.g3y, c, n_to, P->tw(d.g3y, f32), P->exec(STAGE_COL), st)))) return rc;
   }
   {
@@ -1464,7 +1458,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     r.gy = d.g2y;
     return r;
   };
-  auto fwd_args = [&](const char* uin, int nsrc, const char* sout, uint64_t* fmax, const uint8_t* mask, int ntask) {
+  auto fwd_args = [&](const char* uin, int nsrc, const char* sout, uint64_t* fmax, const uint8_t* mask) {
     ColFwdArgs c;
     c.in = P->ws.at(uin).first;
     c.nsrc = nsrc;
@@ -1478,7 +1472,6 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     c.out = P->buf<cpx<double>>(sout);
     c.fmax = fmax;
     c.opitch = spec_layout(d.p2).p;
-    c.stream = (size_t)ntask * nsrc * spec_layout(d.p2).slab * sizeof(cpx<double>) >= kStreamBytes;
     return c;
   };
   static const int e_codes[8] = {M_ONE, M_NU, M_NU2, M_QX_NU, M_QY_NU, M_Q2_NU, M_QX, M_QY};
@@ -1500,7 +1493,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     if ((rc = GLW_STAGE(P, "o2_eta_row", st, row_bytes(row_args("e22_t", 8, "e22_u", 2), nt * J), (launch_row<T, 8, 2, Eta22Op>(d.g2x, row_args("e22_t", 8, "e22_u", 2), nt * J, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("e22_max"), (size_t)nt * J * 2, st))) return rc;
-    if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, colfwd_bytes(fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2, nt * J), d.g2y, nt * J), launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2, nt * J), nt * J,
+    if ((rc = GLW_STAGE(P, "o2_eta_colfwd", st, colfwd_bytes(fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), d.g2y, nt * J), launch_colfwd<T>(d.g2y, fwd_args("e22_u", 2, "e22_s", P->buf<uint64_t>("e22_max"), P->mask2), nt * J,
                                P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
     AccEta22Args g;
@@ -1546,7 +1539,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     }
     if ((rc = GLW_STAGE(P, "o2_psi_row", st, row_bytes(row_args("p22_t", 16, "p22_u", 8), nt * 4), (launch_row<T, 16, 8, Psi22Op>(d.g2x, row_args("p22_t", 16, "p22_u", 8), nt * 4, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, colfwd_bytes(fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2, nt * 4), d.g2y, nt * 4), launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2, nt * 4), nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
+    if ((rc = GLW_STAGE(P, "o2_psi_colfwd", st, colfwd_bytes(fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), d.g2y, nt * 4), launch_colfwd<T>(d.g2y, fwd_args("p22_u", 8, "p22_s", nullptr, P->mask2), nt * 4, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   // ---- psi22 Taylor contact: tasks t
@@ -1565,7 +1558,7 @@ int order2_chunk(glw_plan* P, int nt, double* eta, double* psi, hipStream_t st) 
     if ((rc = GLW_STAGE(P, "o2_contact_colinv", st, colinv_bytes(a, d.g2y, nt), launch_colinv<T>(d.g2y, a, nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))) return rc;
     if ((rc = GLW_STAGE(P, "o2_contact_row", st, row_bytes(row_args("c22_t", 2, "c22_u", 1), nt), (launch_row<T, 2, 1, ContactOp>(d.g2x, row_args("c22_t", 2, "c22_u", 1), nt, P->tw(d.g2x, f32), P->exec(STAGE_ROW), st)))
      )) return rc;
-    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, colfwd_bytes(fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2, nt), d.g2y, nt), launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2, nt), nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
+    if ((rc = GLW_STAGE(P, "o2_contact_colfwd", st, colfwd_bytes(fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), d.g2y, nt), launch_colfwd<T>(d.g2y, fwd_args("c22_u", 1, "c22_s", nullptr, P->mask2), nt, P->tw(d.g2y, f32), P->exec(STAGE_COL), st)))
      ) return rc;
   }
   {
@@ -1657,7 +1650,6 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.out = P->buf<cpx<double>>("o4_ps");
     c.opitch = spec_layout(d.p2).p;
     c.fmax = P->buf<uint64_t>("o4_pm");
-    c.stream = (size_t)ntask * 6 * spec_layout(d.p2).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc2 = GLW_STAGE(P, "o4_pair_colfwd", st, colfwd_bytes(c, d.g2y, ntask), launch_colfwd<T>(d.g2y, c, ntask, P->tw(d.g2y, f32), P->exec(STAGE_COL), st))))
       return rc2;
     if ((rc2 = fill_zero(P->buf<uint64_t>(setmax), (size_t)nset * 5, st))) return rc2;
@@ -1748,7 +1740,6 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.out = P->buf<cpx<double>>("o4_ts");
     c.opitch = spec_layout(d.p3).p;
     c.fmax = P->buf<uint64_t>("o4_tm");
-    c.stream = (size_t)ntask * 4 * spec_layout(d.p3).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o4_third_colfwd", st, colfwd_bytes(c, d.g3y, ntask), launch_colfwd<T>(d.g3y, c, ntask, P->tw(d.g3y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_thm"), (size_t)nt * J * 3, st))) return rc;
@@ -1849,7 +1840,6 @@ int order4_chunk(glw_plan* P, int nt, double* eta, hipStream_t st) {
     c.out = P->buf<cpx<double>>("o4_qs");
     c.opitch = spec_layout(o.p4).p;
     c.fmax = P->buf<uint64_t>("o4_qm");
-    c.stream = (size_t)ntask * 2 * spec_layout(o.p4).slab * sizeof(cpx<double>) >= kStreamBytes;
     if ((rc = GLW_STAGE(P, "o4_quartic_colfwd", st, colfwd_bytes(c, o.g4y, ntask), launch_colfwd<T>(o.g4y, c, ntask, P->tw(o.g4y, f32), P->exec(STAGE_COL), st))))
       return rc;
     if ((rc = fill_zero(P->buf<uint64_t>("o4_outm"), (size_t)nt, st))) return rc;

@@ -7,7 +7,11 @@ The reference has no gradient, so the adjoint is pinned by
   only reproducible to ~1e-9 there: FFT roundoff amplified by the GL node
   factors; the gradient inherits it);
 * Euler homogeneity: phi_n = <w, Re eta_nn> is homogeneous of degree n in
-  (a, b), so <grad phi_n, (a, b)> = n phi_n (exact up to rounding, 1e-9);
+  (a, b), so <grad phi_n, (a, b)> = n phi_n (exact up to rounding: 1e-9 for
+  orders 1 and 2; for order 3 the residual is ~1e-9 relative and moves from
+  build to build with the forward's rounding -- the compiler picks FMA
+  contractions per build, and the roundoff floors amplify rounding-level
+  changes of the order-3 graph, test_conditioning.py -- so it is held to 1e-8);
 * central finite differences of the misfit along 32 seeded directions at
   256^2, Richardson-extrapolated (relative 1e-5: forward roundoff / eps
   dominates; the truncation is O(eps^4)).
@@ -100,7 +104,7 @@ def test_adjoint_euler_homogeneity(torch):
         ga, gb = m.gradient(w, orders=mask, recompute=True)
         euler = float((ga * a).sum() + (gb * b).sum())
         print(f"order {n}: <grad, a> = {euler:.6e}, n phi = {n * phi:.6e}", flush=True)
-        assert abs(euler - n * phi) < 1e-9 * abs(n * phi), n
+        assert abs(euler - n * phi) < (1e-8 if n == 3 else 1e-9) * abs(n * phi), n
 
 
 def test_adjoint_central_differences_32_directions(torch):
