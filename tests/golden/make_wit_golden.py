@@ -16,7 +16,11 @@
   input) in sectors 44, 40 and 42 -- about 3, 8 and 11 minutes on 8 cores --
   or reuses DIR/out_s{44,40,42}.bin from an earlier run of
   ``wit_eta4 tests/golden/wit_n0200_input.bin DIR/out_sS.bin --sector S``,
-  and keeps all five arrays on their nonzero bins.
+  and keeps all five arrays on their nonzero bins;
+* with ``--m400 [DIR]`` only: the same for n0400 (the 400-mode runtime input) in
+  sector 44 -- 1.08e9 quartets, about 71 minutes on 6 cores -- or reuses
+  DIR/out_s44.bin from an earlier ``wit_eta4 tests/golden/wit_n0400_input.bin
+  DIR/out_s44.bin --sector 44``.
 """
 
 import shutil
@@ -34,8 +38,9 @@ from green_laplace_wave_reconstruction_amd import formats  # noqa: E402
 
 ARGS = [x for x in sys.argv[1:] if not x.startswith("--")]
 M200 = "--m200" in sys.argv
+M400 = "--m400" in sys.argv
 SECTORS = [int(x) for a in sys.argv if a.startswith("--sectors=") for x in a.split("=", 1)[1].split(",")] or [44, 40, 42]
-REF = Path(ARGS[0] if ARGS and not M200 else "/root/reference")
+REF = Path(ARGS[0] if ARGS and not (M200 or M400) else "/root/reference")
 EXE = ROOT / "oracle" / "_ref" / "wit_eta4"
 
 
@@ -63,6 +68,18 @@ def save_all(path, out):
                         **{k: out[k].ravel()[idx] for k in ARRAYS})
     return idx.size
 
+
+if M400:
+    given = Path(ARGS[0]) if ARGS else None
+    reuse = given / "out_s44.bin" if given else None
+    with tempfile.TemporaryDirectory() as td:
+        out_path = reuse if reuse and reuse.exists() else Path(td) / "out.bin"
+        if out_path != reuse:
+            subprocess.run([str(EXE), str(HERE / "wit_n0400_input.bin"), str(out_path), "--sector", "44"], check=True)
+        out = formats.read_dir4out(out_path)
+    n = save_all(HERE / "wit_n0400_sector44.npz", out)
+    print("n0400 sector 44 nonzero bins", n, "kept", out["kept"], "near", out["near"])
+    sys.exit(0)
 
 if M200:
     given = Path(ARGS[0]) if ARGS else None

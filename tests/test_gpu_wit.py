@@ -86,6 +86,26 @@ def test_wit_m200_matches_reference_outputs(witmod, sector):
     _check_all_arrays(witmod, "n0200", sector)
 
 
+def test_wit_m400_matches_reference_outputs(witmod):
+    """The 512^2, 400-mode runtime input (the reference's M = 400 timing case, 1.08e9 quartets, 71 min on
+    6 cores for the reference executable in the build container) in sector 44 against its outputs: every
+    array and counter, in FP64 and with FP32 evaluation (bound 1e-6, as for the smaller inputs)."""
+    _check_all_arrays(witmod, "n0400", 44)
+    g = np.load(GOLDEN / "wit_n0400_sector44.npz")
+    inp = formats.read_dir4in(GOLDEN / "wit_n0400_input.bin")
+    plan = witmod.WitPlan(inp["modes"], nx=inp["nx"], ny=inp["ny"], gravity=inp["gravity"], depth=inp["depth"],
+                          dkx=inp["dkx"], dky=inp["dky"], threshold=inp["threshold"], precision="fp32")
+    r = plan.sector44(inp["amplitudes"])
+    plan.close()
+    for k in ("eta", "psi"):
+        ref = np.zeros(int(np.prod(g["shape"])), complex)
+        ref[g["index"]] = g[k]
+        err = rel(r[k], ref.reshape(tuple(g["shape"])))
+        print(f"WIT n0400 FP32 {k}: {err:.2e}", flush=True)
+        assert err < 1e-6, k
+    assert r["kept"] == int(g["counters"][1])
+
+
 @pytest.mark.parametrize("name", ["n0050", "n0100"])
 def test_wit_global_atomic_path(witmod, name):
     """Every quartet added with global atomics (the path of LDS windows above 128 KB) against the

@@ -147,3 +147,20 @@ def test_signed_restatement_outer_split_is_additive():
         b = wit.signed(inp, sector, (n // 3, n))
         assert rel(a["eta"] + b["eta"], full["eta"]) < 1e-14
         assert a["kept"] + b["kept"] == full["kept"]
+
+
+@pytest.mark.parametrize("name,sectors", [("n0200", (44, 40, 42)), ("n0400", (44,))])
+def test_runtime_golden_fixtures_are_consistent(name, sectors):
+    """The reference executable's outputs for the runtime inputs (too large for the CPU restatement
+    here): sector 44 sums every unordered quartet with repetition, C(M + 3, 4) of them; the arrays
+    sit on the 512^2 grid of the input and hold finite values."""
+    from math import comb
+
+    inp = formats.read_dir4in(GOLDEN / f"wit_{name}_input.bin")
+    m = inp["modes"].shape[0]
+    for sector in sectors:
+        g = np.load(GOLDEN / f"wit_{name}_sector{sector}.npz")
+        assert tuple(g["shape"])[-2:] == (inp["ny"], inp["nx"])
+        assert np.all(np.isfinite(g["eta"])) and np.all(np.isfinite(g["psi"]))
+        if sector == 44:
+            assert int(g["counters"][0]) == comb(m + 3, 4)
