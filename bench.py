@@ -134,6 +134,25 @@ def traffic_from_profiles(kernel: str, summary: str = "pmc_traffic.json", schedu
         return None
 
 
+def sq_from_profiles(kernel: str, schedule: str | None = None, summary: str = "sq_counters.json"):
+    """SQ-counter figures of `kernel` (VALU busy, waves per SIMD, ...) from a committed summary, or None
+    (None also when the summary was taken with another plan/tile for that stage)."""
+    p = ROOT / "profiles" / summary
+    if not p.exists():
+        return None
+    try:
+        data = json.loads(p.read_text())
+        if schedule is not None:
+            def entry(text):
+                return dict(e.split("=") for e in text.split(";") if "=" in e).get(kernel, "0/0")
+            if entry(data.get("_schedule", "")) != entry(schedule):
+                return None
+        e = data.get(kernel)
+        return {k: v for k, v in e.items() if k != "kernel"} if e else None
+    except Exception:
+        return None
+
+
 def measured_hbm(traffic, avg_launch_ms):
     """Measured HBM rate of the dominant kernel (PMC bytes per dispatch / launch time) and its
     fraction of the 8 TB/s peak, next to the algorithmic figure; Nones without a PMC summary."""
@@ -973,6 +992,9 @@ def main():
     traffic = traffic_from_profiles(roof["kernel"], schedule=schedule)
     roof["traffic"] = traffic
     roof.update(measured_hbm(traffic, roof["avg_launch_ms"]))
+    # the dominant stage's SQ counters (profiles/sq_counters.json, same schedule): a row kernel is
+    # compute/latency-bound (VALU half busy at 2-4 waves per SIMD), not HBM-bound
+    roof["sq_counters"] = sq_from_profiles(roof["kernel"], schedule=schedule)
     n2 = s.nx * s.ny
 
     sample_idx = np.linspace(0, INSTANTS - 1, max(1, args.cpu_sample)).round().astype(int)
