@@ -15,7 +15,11 @@ Pinning (what checks each function besides re-evaluating its own formula):
   reference's three MATLAB-generated 64x64 fixtures at the reference's own
   tolerances, the linear closed form, exact homogeneity of degree 2 and 3, and
   (eta22) the ordered-pair brute force of the same GL kernel.
-* ``wit``: see ``oracle/wit_ref.c`` and its header.
+* ``wit`` (``oracle/wit_oracle.cpp``, sectors 44, 40 and 42): the reference
+  executable built from its own source into ``oracle/_ref`` (n0050, n0100,
+  n0382 inputs), the reference's shipped n0382 output, and the reference
+  executable's n0050 / n0200 outputs of the signed sectors; every counter
+  exact.
 * ``modal_oracle`` (single-direction time path): the reference's two
   MATLAB-generated time records (eta22/eta33 at 3e-13, eta20 at 8e-11), the
   on-grid identity with the spatial graph at 3e-14, the off-grid ladder
