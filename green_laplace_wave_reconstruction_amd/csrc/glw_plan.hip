@@ -2595,7 +2595,7 @@ const char* glw_last_error(void) { return g_err.c_str(); }
 const char* glw_version(void) { return "glw 0.1.0 gfx950"; }
 
 int glw_hbm_probe(int32_t kind, const double* src, double* dst, size_t bytes, int32_t reps, double* ms) {
-  if (!ms || reps < 1 || bytes < 16 || (kind != 1 && !src) || (kind != 2 && !dst) || kind < 0 || kind > 2)
+  if (!ms || reps < 1 || bytes < 16 || (kind != 1 && !src) || !dst || kind < 0 || kind > 2)
     return fail(GLW_EINVAL, "glw_hbm_probe: bad arguments");
   const size_t n = bytes / 16;
   const dim3 grid((unsigned)((n + 255) / 256));

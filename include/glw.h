@@ -306,7 +306,8 @@ int glw_fft_length_supported(int32_t n);
  * times `reps` launches of a plain streaming kernel over `bytes` of device
  * memory on the default stream with HIP events and returns the mean launch
  * time in *ms.  kind 0: copy src -> dst; 1: fill dst (write only); 2: read src
- * (dst receives nothing; pass any 16-byte device buffer).  Synchronises. */
+ * (dst must still be a device buffer of at least 16 bytes: the read kernel
+ * writes one value there only if a sum is infinite).  Synchronises. */
 int glw_hbm_probe(int32_t kind, const double* src, double* dst, size_t bytes, int32_t reps, double* ms);
 
 /* ---- Single-direction modal time path -------------------------------------
