@@ -29,11 +29,26 @@ constexpr int col_width() {
   return p;
 }
 
-// Rows per workgroup of the fused row kernel: as many as fit kRowLds (at most 4).
-template <typename T, int NIN, int L>
+// LDS budget of a fused row tile for an operator: kRowLds unless the operator
+// sets kRowLdsKb.  The eta44 third-order operator (31 fields of 64 points)
+// takes 96 KB, two rows per workgroup: its products and forward transforms then
+// keep more lanes busy (eta44 GL6/8/10 at 256^2: +4-6 % on MI355X), while the
+// order-2 and order-3 operators are faster at 64 KB (-2.5 to -4 % at 96 KB).
+template <class Op, class = void>
+struct row_lds_of {
+  static constexpr int value = kRowLds;
+};
+template <class Op>
+struct row_lds_of<Op, std::void_t<decltype(Op::kRowLdsKb)>> {
+  static constexpr int value = Op::kRowLdsKb * 1024;
+};
+
+// Rows per workgroup of the fused row kernel: as many as fit the operator's
+// LDS budget (at most 4).
+template <typename T, int NIN, int L, class Op = void>
 constexpr int row_ry() {
   int per = NIN * seq_stride<L>() * (int)sizeof(cpx<T>);
-  int ry = kRowLds / per;
+  int ry = row_lds_of<Op>::value / per;
   if (ry >= 4) return 4;
   if (ry >= 2) return 2;
   return 1;
@@ -227,7 +242,7 @@ int row_split(int gx, const RowArgs& a, int ntask, const void* tw, const Exec& x
 template <typename T, int NIN, int NOUT, class Op, int L>
 int row_len(const RowArgs& a, int ntask, const void* tw, const Exec& x, hipStream_t st) {
   static_assert(NOUT <= NIN, "the fused row kernel reuses the input tile for the outputs");
-  constexpr int RY0 = row_ry<T, NIN, L>();
+  constexpr int RY0 = row_ry<T, NIN, L, Op>();
   constexpr int P1 = canonical_plan<L, 1>();
   int tile = x.s.tile;
   if (tile == 0 && !row_fits<T, NIN, L, RY0>()) tile = 1;

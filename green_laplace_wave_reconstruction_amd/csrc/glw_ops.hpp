@@ -221,6 +221,7 @@ struct Pair44Op {
 //           26 ptz 27 px_t 28 py_t 29 pzz_t 30 ptz_t
 // out: fk, fd, fkt, fdt
 struct Third44Op {
+  static constexpr int kRowLdsKb = 96;  // two 64-point rows of 31 fields per fused row tile
   template <typename T>
   __device__ __forceinline__ static void apply(const cpx<T>* f, cpx<T>* o) {
     const cpx<T> e = f[0], ex = f[1], ey = f[2], px = f[3], py = f[4], pz = f[5], pxz = f[6], pyz = f[7];
