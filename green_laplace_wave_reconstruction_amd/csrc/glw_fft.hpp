@@ -42,6 +42,9 @@ __device__ __forceinline__ cpx<T> cmul_i(cpx<T> a) {
 // power-of-two strides of the Stockham writes, and the odd sequence stride
 // keeps lanes that walk across sequences (column tiles) on distinct banks.
 __host__ __device__ constexpr int lds_pad(int e) { return e + (e >> 4); }
+// Element offset within a sequence: padded (the layout above) or dense.
+template <bool PAD>
+__host__ __device__ constexpr int elem_at(int e) { return PAD ? lds_pad(e) : e; }
 template <int L>
 __host__ __device__ constexpr int seq_stride() {
   return (L + (L >> 4)) | 1;
@@ -345,7 +348,7 @@ __device__ __forceinline__ void stage_twiddle_regs(const cpx<T>* tw, cpx<T> (&w)
 // reads and its twiddle multiply.  Only for twiddle tables in global memory
 // (an LDS table may not be staged yet when the next stage's loads issue).
 template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int S, int NS, int PLAN, bool PREFETCH, bool FINAL,
-          class LD, class ST, class WCUR>
+          bool PAD, class LD, class ST, class WCUR>
 __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st, const WCUR& wcur) {
   using SH = StageShape<L, NSEQ, NT, S, NS, PLAN, LD, ST>;
   constexpr int R = SH::R;
@@ -370,7 +373,7 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const 
         } else {
           const cpx<T>* src = buf + seq * SSTR;
 #pragma unroll
-          for (int r = 0; r < R; ++r) v[p][r] = src[lds_pad(j + r * NB)];
+          for (int r = 0; r < R; ++r) v[p][r] = src[elem_at<PAD>(j + r * NB)];
         }
       }
     }
@@ -413,12 +416,12 @@ __device__ __forceinline__ void fft_stages(cpx<T>* buf, const cpx<T>* tw, const 
         } else {
           cpx<T>* dst = buf + seq * SSTR;
 #pragma unroll
-          for (int r = 0; r < R; ++r) dst[lds_pad(base + r * NS)] = v[p][r];
+          for (int r = 0; r < R; ++r) dst[elem_at<PAD>(base + r * NS)] = v[p][r];
         }
       }
     }
     if constexpr (!GSTORE) __syncthreads();
-    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R, PLAN, PREFETCH, FINAL>(buf, tw, ld, st, wnext);
+    fft_stages<T, L, NSEQ, SSTR, NT, SIGN, S + 1, NS * R, PLAN, PREFETCH, FINAL, PAD>(buf, tw, ld, st, wnext);
   }
 }
 
@@ -436,7 +439,7 @@ __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* tw) {
   static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
   NoIO none;
   const cpx<T> w0[1][1] = {};
-  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN, false, false>(buf, tw, none, none, w0);
+  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN, false, false, true>(buf, tw, none, none, w0);
 }
 
 // Same transform with a global loader for the first stage and/or a storer
@@ -446,12 +449,12 @@ __device__ __forceinline__ void fft_lds(cpx<T>* buf, const cpx<T>* tw) {
 // twiddle tables only): each stage's twiddles load one stage ahead.  FINAL
 // (with a storer): the caller never touches `buf` again, so the last stage
 // skips its read-before-write barrier.
-template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN, bool PREFETCH, bool FINAL = false, class LD,
-          class ST>
+template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN, bool PREFETCH, bool FINAL = false,
+          bool PAD = true, class LD, class ST>
 __device__ __forceinline__ void fft_io_pf(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st) {
   static_assert(fft_length_ok(L), "FFT length must be 2^a 3^b >= 8");
   const cpx<T> w0[1][1] = {};  // stage 0 has no twiddles (NS = 1)
-  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN, PREFETCH, FINAL>(buf, tw, ld, st, w0);
+  fft_stages<T, L, NSEQ, SSTR, NT, SIGN, 0, 1, PLAN, PREFETCH, FINAL, PAD>(buf, tw, ld, st, w0);
 }
 template <typename T, int L, int NSEQ, int SSTR, int NT, int SIGN, int PLAN, class LD, class ST>
 __device__ __forceinline__ void fft_io(cpx<T>* buf, const cpx<T>* tw, const LD& ld, ST& st) {

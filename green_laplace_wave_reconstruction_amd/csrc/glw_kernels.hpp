@@ -24,6 +24,42 @@ namespace glw {
 
 constexpr int kMaxFields = 48;
 constexpr int NT = 256;
+
+// Workgroups are dealt to the 8 XCDs round robin, and each XCD has its own
+// L2.  xcd_block renumbers dispatch slot w of an n-workgroup grid so that XCD
+// k runs one consecutive range of logical workgroups (range sizes differ by
+// at most one; a bijection on [0, n)).  Neighbouring logical workgroups --
+// the column tiles and fields of one task, which read the same spectrum lines
+// -- then share one L2 instead of each XCD fetching those lines again.
+#ifndef GLW_XCD_COL
+#define GLW_XCD_COL 1
+#endif
+__device__ __forceinline__ int xcd_block(int w, int n) {
+  const int q = n >> 3, r = n & 7, k = w & 7;
+  return k * q + (k < r ? k : r) + (w >> 3);
+}
+
+// LDS layout of the column kernels' tiles.  Their lanes run across the CW
+// columns (sequences) of a tile at one element, so the 16-element padding of
+// lds_pad buys nothing there; what decides bank conflicts is the sequence
+// stride S.  Measured on MI355X with ds_read_b128 / ds_write_b128 over 8
+// sequences x consecutive elements: reads are served 32 lanes at a time over
+// a 512-byte bank period, writes 8 lanes at a time over 128 bytes.  Odd S
+// keeps the writes conflict-free; among odd S, S = 23 or 25 (mod 32) halves
+// the read conflicts of other residues (1.55x the conflict-free time instead
+// of 2.5x); no stride makes both conflict-free.  GLW_COL_SSTR_RES < 0 keeps
+// the padded row layout (seq_stride).
+#ifndef GLW_COL_SSTR_RES
+#define GLW_COL_SSTR_RES 23
+#endif
+constexpr bool kColPad = GLW_COL_SSTR_RES < 0;
+template <int L>
+__host__ __device__ constexpr int col_stride() {
+  if (GLW_COL_SSTR_RES < 0) return seq_stride<L>();
+  int s = L;
+  while (s % 32 != GLW_COL_SSTR_RES) ++s;
+  return s;
+}
 // FFT stages read twiddles from the global table (a few KB, L1/L2-resident)
 // rather than an LDS copy: the LDS goes to data tiles (more workgroups per CU).
 // Measured on MI355X (GL order 3, 512^2, 64 instants): LDS copies in the
@@ -262,15 +298,22 @@ struct ColInvStore {
 
 template <typename T, int GY, int CW, int PLAN = 0, int NTH = NT>
 __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __restrict__ tw) {
-  __shared__ cpx<T> buf[CW * seq_stride<GY>()];
+  __shared__ cpx<T> buf[CW * col_stride<GY>()];
   __shared__ cpx<T> twl[kTwiddlesInLdsCol ? GY : 1];
   const cpx<T>* twp = tw;
   if constexpr (kTwiddlesInLdsCol) {  // read from the second stage on, after a barrier
     stage_twiddles<T, GY, NTH>(twl, tw);
     twp = twl;
   }
-  const int f = blockIdx.y;
-  const int task = blockIdx.z;
+  int tile = blockIdx.x, f = blockIdx.y, task = blockIdx.z;
+  if constexpr (GLW_XCD_COL) {
+    int w = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                      gridDim.x * gridDim.y * gridDim.z);
+    tile = w % gridDim.x;
+    w /= gridDim.x;
+    f = w % gridDim.y;
+    task = w / gridDim.y;
+  }
   const int set = task / a.sdiv;
   const int src = a.fsrc[f];
   const int box = a.wx * a.wy;
@@ -284,7 +327,7 @@ __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __re
   }
   ld.scale = a.fscale[f];
   ld.mul = a.tab.mul + (size_t)a.fcode[f] * box;
-  ld.col0 = blockIdx.x * CW;
+  ld.col0 = tile * CW;
   ld.wx = a.wx;
   ld.wy = a.wy;
   ld.my0 = a.my0;
@@ -295,7 +338,7 @@ __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __re
   st.col0 = ld.col0;
   st.wx = a.wx;
   st.stream = a.stream;
-  fft_io_pf<T, GY, CW, seq_stride<GY>(), NTH, +1, PLAN, kPfCol, kFinal>(buf, twp, ld, st);
+  fft_io_pf<T, GY, CW, col_stride<GY>(), NTH, +1, PLAN, kPfCol, kFinal, kColPad>(buf, twp, ld, st);
 }
 
 // ----------------------------------------------------------------------------
@@ -396,7 +439,7 @@ struct ColFwdStore {
 
 template <typename T, int GY, int CW, int PLAN = 0, int NTH = NT>
 __global__ __launch_bounds__(NTH) void k_colfwd(ColFwdArgs a, const cpx<T>* __restrict__ tw) {
-  __shared__ cpx<T> buf[CW * seq_stride<GY>()];
+  __shared__ cpx<T> buf[CW * col_stride<GY>()];
   __shared__ cpx<T> twl[kTwiddlesInLdsCol ? GY : 1];
   const cpx<T>* twp = tw;
   if constexpr (kTwiddlesInLdsCol) {
@@ -424,7 +467,7 @@ __global__ __launch_bounds__(NTH) void k_colfwd(ColFwdArgs a, const cpx<T>* __re
   st.owy = a.owy;
   st.omy0 = a.omy0;
   st.gy = GY;
-  fft_io_pf<T, GY, CW, seq_stride<GY>(), NTH, -1, PLAN, kPfCol, kFinal>(buf, twp, ld, st);
+  fft_io_pf<T, GY, CW, col_stride<GY>(), NTH, -1, PLAN, kPfCol, kFinal, kColPad>(buf, twp, ld, st);
   if (a.fmax) block_max_atomic<NTH>(st.vmax, a.fmax + (size_t)task * a.nsrc + s);
 }
 

@@ -23,7 +23,7 @@ constexpr int kLdsMax = 160 * 1024;    // gfx950 LDS per workgroup
 // Widest power-of-two column tile (<= 32 columns) within kLdsBudget.
 template <typename T, int L>
 constexpr int col_width() {
-  int cw = kLdsBudget / (seq_stride<L>() * (int)sizeof(cpx<T>));
+  int cw = kLdsBudget / (col_stride<L>() * (int)sizeof(cpx<T>));
   int p = 1;
   while (p * 2 <= cw && p < 32) p *= 2;
   return p;
