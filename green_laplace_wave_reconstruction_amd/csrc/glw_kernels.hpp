@@ -60,6 +60,29 @@ __host__ __device__ constexpr int col_stride() {
   while (s % 32 != GLW_COL_SSTR_RES) ++s;
   return s;
 }
+
+// Columns the column-inverse pass writes per row: the box width w rounded up
+// to whole 128-byte lines within the row pitch, when its tiles span whole
+// lines and the last tile is at least half full (the padding columns get the
+// loader's zeros; no consumer reads past a field's width).  A line written in
+// part costs HBM time out of proportion: the inner column-inverse pass at 512^2
+// (45 of 48 columns per row) took 0.42 ms with the box columns only and
+// 0.33 ms with whole lines on MI355X.  Measured and not taken: a last tile
+// holding 1 of 8 columns writing all 8 (89 -> 96 columns: 5 % slower), whole
+// lines for the row kernels' output rows (89 -> 96 and 133 -> 136: 3-4 %
+// slower, and the column-forward passes that read them 2-3 % slower) and for
+// the column-forward spectra.  GLW_FULL_LINES=0 writes the box columns only.
+#ifndef GLW_FULL_LINES
+#define GLW_FULL_LINES 1
+#endif
+template <typename T, int CW>
+__host__ __device__ inline int fill_width_cols(int w, int pitch) {
+  constexpr int kLine = 128 / (int)sizeof(cpx<T>);
+  const int part = w % kLine;
+  if (!GLW_FULL_LINES || CW < kLine || (part != 0 && 2 * part < kLine)) return w;
+  const int wl = (w + kLine - 1) / kLine * kLine;
+  return wl < pitch ? wl : (w > pitch ? w : pitch);
+}
 // FFT stages read twiddles from the global table (a few KB, L1/L2-resident)
 // rather than an LDS copy: the LDS goes to data tiles (more workgroups per CU).
 // Measured on MI355X (GL order 3, 512^2, 64 instants): LDS copies in the
@@ -277,7 +300,8 @@ template <typename T>
 struct ColInvStore {
   static constexpr bool SEQ_FAST = true;
   cpx<T>* out;
-  int ostride, col0, wx;
+  int ostride, col0;
+  int wx;  // columns written: fill_width_cols of the box (the loader's zeros past it)
   bool stream;
   __device__ __forceinline__ void put(cpx<T>* p, cpx<T> v) {
     if (stream) store_stream(p, v);
@@ -336,7 +360,7 @@ __global__ __launch_bounds__(NTH) void k_colinv(ColInvArgs a, const cpx<T>* __re
   st.out = reinterpret_cast<cpx<T>*>(a.out) + ((size_t)task * a.out_nfield + a.out_foff + f) * GY * a.ostride;
   st.ostride = a.ostride;
   st.col0 = ld.col0;
-  st.wx = a.wx;
+  st.wx = fill_width_cols<T, CW>(a.wx, a.ostride);
   st.stream = a.stream;
   fft_io_pf<T, GY, CW, col_stride<GY>(), NTH, +1, PLAN, kPfCol, kFinal, kColPad>(buf, twp, ld, st);
 }
