@@ -819,8 +819,9 @@ def main():
     ap.add_argument("--rank", type=int, default=6, help="eta44 GL rank")
     ap.add_argument("--modes", type=int, default=200, help="WIT mode count (200, 400, 800)")
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 20 for order3, whose step is ~4 ms, 10 for the other workloads)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps first (default: 5 for order3, else 2)")
     ap.add_argument("--precision", default="fp64", choices=("fp64", "fp32"))
     ap.add_argument("--sector", type=int, default=44, choices=(40, 42, 44), help="wit: WIT sector")
     ap.add_argument("--cpu-sample", type=int, default=3)
@@ -840,6 +841,10 @@ def main():
                     help="time plain 4 GiB copy (read + write), fill (write) and sum (read) kernels: the HBM rates "
                          "a streaming kernel reaches on this device")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 20 if args.workload == "order3" else 10
+    if args.warmup is None:
+        args.warmup = 5 if args.workload == "order3" else 2
     if args.calibrate_hbm:
         import torch
 
