@@ -206,6 +206,52 @@ def describe(plan: HostPlan, max_instants: int) -> tuple[CPlanDesc, list]:
     return d, keep
 
 
+# Device -> host copies of large outputs.  torch copies a device tensor into
+# pageable host memory at about 4 GB/s on MI355X (a 268 MB [64][512][512]
+# complex128 field in 65 ms), against about 50 GB/s into pinned memory.  The
+# outputs are staged through two reusable pinned chunks instead: the copy of
+# chunk k + 1 runs while chunk k is copied into the caller's NumPy array.
+_STAGE_BYTES = 32 << 20
+_stage = {}
+
+
+def to_host(x) -> np.ndarray:
+    """A contiguous NumPy copy of device tensor ``x`` (same dtype and shape)."""
+    import torch
+
+    if isinstance(x, np.ndarray):
+        return x
+    if x.device.type != "cuda" or x.numel() * x.element_size() <= _STAGE_BYTES:
+        return x.cpu().numpy()
+    x = x.contiguous()
+    flat = x.view(-1)
+    out = np.empty(tuple(x.shape), dtype=torch.empty((), dtype=x.dtype).numpy().dtype)
+    outf = out.reshape(-1)
+    key = (x.device.index, x.dtype)
+    if key not in _stage:
+        n = _STAGE_BYTES // x.element_size()
+        _stage[key] = ([torch.empty(n, dtype=x.dtype, pin_memory=True) for _ in range(2)],
+                       [torch.cuda.Event() for _ in range(2)])
+    bufs, events = _stage[key]
+    chunk = bufs[0].numel()
+    stream = torch.cuda.current_stream(x.device)
+    pending = []  # (slot, offset, count) copied and not yet drained
+    for i, off in enumerate(range(0, flat.numel(), chunk)):
+        cnt = min(chunk, flat.numel() - off)
+        slot = i % 2
+        if len(pending) == 2:  # the slot about to be refilled holds the oldest chunk: drain it first
+            s0, o0, c0 = pending.pop(0)
+            events[s0].synchronize()
+            np.copyto(outf[o0:o0 + c0], bufs[s0][:c0].numpy())
+        bufs[slot][:cnt].copy_(flat[off:off + cnt], non_blocking=True)
+        events[slot].record(stream)
+        pending.append((slot, off, cnt))
+    for s0, o0, c0 in pending:
+        events[s0].synchronize()
+        np.copyto(outf[o0:o0 + c0], bufs[s0][:c0].numpy())
+    return out
+
+
 class DevicePlan:
     """A glw plan on the current HIP device plus the t=0 parent spectrum.
 

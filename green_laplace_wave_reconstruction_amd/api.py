@@ -23,7 +23,7 @@ from typing import Any
 import numpy as np
 
 from . import plan as _plan
-from ._native import DevicePlan
+from ._native import DevicePlan, to_host
 
 API_ID = "green-laplace-spectral-v1"
 
@@ -157,8 +157,8 @@ def _validate_coefficients(coeffs) -> None:
         raise GreenLaplaceError("green_laplace:Coefficients", "Input must come from gl_spectral_coefficients.")
 
 
-def _evaluate(host: _plan.HostPlan, s0_box: np.ndarray, times: np.ndarray, order: int, *, return_device: bool,
-              max_instants: int = 16):
+def _evaluate(host: _plan.HostPlan, s0_box: np.ndarray, times: np.ndarray, order: int, *, max_instants: int = 16):
+    """Device tensors of every *_plus component of orders 1..order at the given instants."""
     dp = DevicePlan(host, s0_box, max_instants=max(1, min(max_instants, len(times))))
     comps = {}
     try:
@@ -175,8 +175,6 @@ def _evaluate(host: _plan.HostPlan, s0_box: np.ndarray, times: np.ndarray, order
 
         torch.cuda.synchronize()
         dp.close()
-    if not return_device:
-        comps = {k: v.cpu().numpy() for k, v in comps.items()}
     return comps
 
 
@@ -199,23 +197,32 @@ def _check_order3_finite(comps: dict) -> None:
         raise GreenLaplaceError("", "Direct Green--Laplace Psi33 output is not finite.")
 
 
-def _finish(comps: dict, order: int, single: bool) -> dict:
+def _finish(comps: dict, order: int, single: bool, to_numpy: bool) -> dict:
+    """Output dictionary: every *_plus component, its real part, and the summed eta/psi.
+
+    The sums run on the device in the reference's order (orders 1, 2, 3 added
+    left to right), then every field is copied to host once when ``to_numpy``.
+    """
     keys = ["eta11", "psi11"] + (["eta22", "psi22"] if order >= 2 else []) + (["eta33", "psi33"] if order >= 3 else [])
-    out: dict = {}
+    plus: dict = {}
     eta_plus = None
     psi_plus = None
     for k in keys:
         v = comps[k + "_plus"]
-        if single:
-            v = v[0]
-        out[k + "_plus"] = v
-        out[k] = v.real
+        plus[k + "_plus"] = v
         if k.startswith("eta"):
             eta_plus = v if eta_plus is None else eta_plus + v
         else:
             psi_plus = v if psi_plus is None else psi_plus + v
-    out["eta_plus"], out["psi_plus"] = eta_plus, psi_plus
-    out["eta"], out["psi"] = eta_plus.real, psi_plus.real
+    plus["eta_plus"], plus["psi_plus"] = eta_plus, psi_plus
+    out: dict = {}
+    for k, v in plus.items():
+        if single:
+            v = v[0]
+        out[k] = to_host(v) if to_numpy else v
+    for k in keys:
+        out[k] = out[k + "_plus"].real
+    out["eta"], out["psi"] = out["eta_plus"].real, out["psi_plus"].real
     return out
 
 
@@ -249,9 +256,9 @@ def gl_spectral_surface(coeffs: dict, Lx: float, Ly: float, Nx: int, Ny: int, t,
                             allow_below_model_parent_domain=coeffs["options"]["allow_below_model_parent_domain"],
                             precision=_precision(precision))
     host.omega = host.box1.take(omega)
-    comps = _evaluate(host, host.box1.take(spec), times, order, return_device=return_device)
+    comps = _evaluate(host, host.box1.take(spec), times, order)
     _check_order3_finite(comps)
-    out = _finish(comps, order, single)
+    out = _finish(comps, order, single, to_numpy=not return_device)
     X, Y = np.meshgrid(np.arange(Nx) * Lx / Nx, np.arange(Ny) * Ly / Ny)
     included = ["eta11", "psi11"] + (["eta22", "psi22"] if order >= 2 else []) + (["eta33", "psi33"] if order >= 3 else [])
     audit = {
@@ -331,8 +338,8 @@ def reconstruct_directional_field(eta1, *, lx: float, ly: float, depth: float, g
     host = _plan.build_plan(spec, depth * kx, depth * ky, depth=depth, gravity=gravity, order=order,
                             eta22_rank=eta22_rank, allow_below_model_parent_domain=allow_below_model_parent_domain,
                             precision=_precision(precision))
-    comps = _evaluate(host, host.box1.take(spec), np.array([float(time)]), order, return_device=False)
-    out = _finish(comps, order, True)
+    comps = _evaluate(host, host.box1.take(spec), np.array([float(time)]), order)
+    out = _finish(comps, order, True, to_numpy=True)
     x, y = np.meshgrid(np.arange(nx, dtype=float) * lx / nx, np.arange(ny, dtype=float) * ly / ny)
     audit = {
         "implementation": "green-laplace-mi355x-directional-spatial", "order": order, "grid": [ny, nx],

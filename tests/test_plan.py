@@ -139,7 +139,7 @@ def test_matlab_api_nonfinite_order3_output(monkeypatch):
     the device evaluation is replaced by fields with one NaN."""
     c = api.gl_spectral_coefficients(3, 1.0, 1.0, [.01, .006], [0, -.002], [2, 3], [0, 1], 0, 0)
 
-    def fake(host, s0, times, order, *, return_device, max_instants=16):
+    def fake(host, s0, times, order, *, max_instants=16):
         z = np.zeros((len(times), 32, 32), complex)
         out = {k: z.copy() for k in ("eta11_plus", "psi11_plus", "eta22_plus", "psi22_plus",
                                      "eta33_plus", "psi33_plus")}
