@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_size_t, c_uint8, c_void_p
 from pathlib import Path
 
@@ -213,6 +214,7 @@ def describe(plan: HostPlan, max_instants: int) -> tuple[CPlanDesc, list]:
 # chunk k + 1 runs while chunk k is copied into the caller's NumPy array.
 _STAGE_BYTES = 32 << 20
 _stage = {}
+_stage_lock = threading.Lock()  # the chunks are shared by every caller in the process
 
 
 def to_host(x) -> np.ndarray:
@@ -223,7 +225,11 @@ def to_host(x) -> np.ndarray:
         return x
     if x.device.type != "cuda" or x.numel() * x.element_size() <= _STAGE_BYTES:
         return x.cpu().numpy()
-    x = x.contiguous()
+    with _stage_lock:
+        return _staged_copy(torch, x.contiguous())
+
+
+def _staged_copy(torch, x) -> np.ndarray:
     flat = x.view(-1)
     out = np.empty(tuple(x.shape), dtype=torch.empty((), dtype=x.dtype).numpy().dtype)
     outf = out.reshape(-1)
