@@ -122,10 +122,59 @@ def box_of(mask: np.ndarray) -> Box:
     return Box(int(mxs.min()), int(mys.min()), int(mxs.max() - mxs.min() + 1), int(mys.max() - mys.min() + 1))
 
 
+def _smooth_length(n: int) -> int:
+    """Smallest 2^a 3^b 5^c >= n (a fast pocketfft length)."""
+    m = n
+    while True:
+        r = m
+        for f in (2, 3, 5):
+            while r % f == 0:
+                r //= f
+        if r == 1:
+            return m
+        m += 1
+
+
+def minkowski_sum(masks: list) -> np.ndarray:
+    """Minkowski sum of masks on their periodic [ny, nx] grid: the thresholded
+    convolution count ``ifft2(prod fft2(mask)) > 0.5`` (_order3.py:38-40).
+
+    When the sum of the masks' bounding boxes fits the grid, no bin of the sum
+    wraps onto another, so the same count is formed on a grid of the boxes'
+    size and placed back (a few thousand points instead of N^2: the host plan
+    at 512^2 spent half its time in these transforms); otherwise on the full
+    grid.
+    """
+    ny, nx = masks[0].shape
+    if not all(np.any(m) for m in masks):
+        return np.zeros((ny, nx), dtype=bool)
+    boxes = [box_of(m) for m in masks]
+    ex = sum(b.wx - 1 for b in boxes) + 1
+    ey = sum(b.wy - 1 for b in boxes) + 1
+    small = ex <= nx and ey <= ny
+    if small:
+        gy, gx = _smooth_length(ey), _smooth_length(ex)
+    spectra = {}  # one transform per distinct mask
+    acc = None
+    for m, b in zip(masks, boxes):
+        f = spectra.get(id(m))
+        if f is None:
+            if small:
+                grid = np.zeros((gy, gx))
+                grid[:b.wy, :b.wx] = b.take(m)
+            else:
+                grid = m.astype(float)
+            f = spectra[id(m)] = np.fft.fft2(grid)
+        acc = f if acc is None else acc * f
+    count = np.real(np.fft.ifft2(acc)) > 0.5
+    if not small:
+        return count
+    return Box(sum(b.mx0 for b in boxes), sum(b.my0 for b in boxes), ex, ey).put(count[:ey, :ex], (ny, nx))
+
+
 def sum_support(mask: np.ndarray, n: int) -> np.ndarray:
     """n-fold Minkowski sum of a mask, thresholded convolution count (_order3.py:38-40)."""
-    t = np.fft.fft2(mask.astype(float))
-    return np.real(np.fft.ifft2(t**n)) > 0.5
+    return minkowski_sum([mask] * n)
 
 
 def _sinh_ratio(sa: np.ndarray, tau: float) -> np.ndarray:
@@ -295,8 +344,7 @@ def _order2_tables(plan: HostPlan, spectrum, support, s2, q, nu_full, rank, allo
             masks.append(m)
             vals = nu_full[m]
             shifts.append(0.5 * (float(np.min(vals)) + float(np.max(vals))))
-    ts = [np.fft.fft2(m.astype(float)) for m in masks]
-    pair_sup = [[np.real(np.fft.ifft2(ts[i] * ts[j])) > 0.5 for j in range(len(masks))] for i in range(len(masks))]
+    pair_sup = [[minkowski_sum([masks[i], masks[j]]) for j in range(len(masks))] for i in range(len(masks))]
     pn = math.sqrt(qp * math.tanh(qp))
     pon = math.sqrt(2.0 * qp * math.tanh(2.0 * qp))
     scales = (2.0 * pn - pon, 2.0 * pn + pon)
