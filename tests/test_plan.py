@@ -171,3 +171,33 @@ def test_mode_numbers_exact_on_radix3_grids():
     mask = np.zeros((192, 192), dtype=bool)
     mask[[-28, 0, 28], [28, 1, 27]] = True
     assert PL.box_of(mask) == PL.Box(1, -28, 28, 57)
+
+
+def test_minkowski_sum_matches_full_grid_count():
+    """plan.minkowski_sum is the thresholded full-grid count ifft2(prod fft2(mask)) > 0.5
+    (_order3.py:38-40): on bounding-box grids when the summed box fits, else on the full grid.
+    Seeded random masks cover boxes that wrap, repeated masks (n-fold sums) and empty masks."""
+    rng = np.random.default_rng(7)
+
+    def full(masks):
+        t = None
+        for m in masks:
+            f = np.fft.fft2(m.astype(float))
+            t = f if t is None else t * f
+        return np.real(np.fft.ifft2(t)) > 0.5
+
+    for _ in range(120):
+        ny, nx = int(rng.choice([16, 24, 48, 64, 96])), int(rng.choice([16, 32, 48, 80, 128]))
+        n = int(rng.integers(1, 4))
+        masks = []
+        for _ in range(n):
+            m = np.zeros((ny, nx), bool)
+            h, w = int(rng.integers(1, ny)), int(rng.integers(1, nx))
+            y0, x0 = int(rng.integers(-ny // 2, ny // 2)), int(rng.integers(-nx // 2, nx // 2))
+            m[np.ix_(np.arange(y0, y0 + h) % ny, np.arange(x0, x0 + w) % nx)] = rng.random((h, w)) < rng.uniform(0.05, 0.9)
+            masks.append(m)
+        if rng.random() < 0.3:
+            masks = [masks[0]] * n
+        assert np.array_equal(PL.minkowski_sum(masks), full(masks))
+    empty = np.zeros((32, 32), bool)
+    assert not PL.minkowski_sum([empty, empty]).any()
