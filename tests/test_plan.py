@@ -114,6 +114,23 @@ def test_matlab_api_error_identifiers(args, ident):
     assert e.value.identifier == ident
 
 
+def test_deposit_shared_bins_and_first_failing_component():
+    """src/gl_spectral_surface.m:123-149 walks the components in order: components on one
+    bin add, the last one sets the bin's frequency, and the first failing one names the error."""
+    c = dict(api.gl_spectral_coefficients(2, 9.81, 1, [.01, .02], [.003, 0], [2.0, 2.0], [1.0, 1.0], 0, 0))
+    c["omega"] = np.array([3.0, 5.0])
+    spec, omega = api.deposit_analytic_input(c, 2 * np.pi, 2 * np.pi, 32, 32)
+    assert np.count_nonzero(spec) == 1 and np.count_nonzero(omega) == 1
+    assert spec[1, 2] == pytest.approx(32 * 32 * complex(.03, .003) / c["h"], rel=1e-15)
+    assert omega[1, 2] == 5.0
+    nyq, off = 16.0, 2.0000001
+    for kx, ident in (([nyq, off], "green_laplace:Nyquist"), ([off, nyq], "green_laplace:OffGridComponent")):
+        c = api.gl_spectral_coefficients(2, 9.81, 1, [.01, .01], [0, 0], kx, [0, 0], 0, 0)
+        with pytest.raises(GreenLaplaceError) as e:
+            api.deposit_analytic_input(c, 2 * np.pi, 2 * np.pi, 32, 32)
+        assert e.value.identifier == ident
+
+
 def test_matlab_api_option_errors():
     with pytest.raises(GreenLaplaceError) as e:
         api.gl_spectral_coefficients(2, 9.81, 1, [.01], [0], [2], [0], 0, 0, {"nope": 1})
